@@ -1,0 +1,1031 @@
+// lbm_capi.hip -- host side of liblbm_hip.so: the C ABI of include/lbm_hip.h.
+//
+// Replaces the per-worker state and time loop of the reference
+// (src/MainCore.jl:165-379): device buffers (worker_alloc, init_bb_buffers!,
+// src/Initialization.jl:89-130), the motion/mask/link rebuild (motion!,
+// src/Motion.jl:41-122) and the per-step operator sequence.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm_types.h"
+
+using namespace lbm;
+
+namespace {
+
+constexpr int kChunk = 64;  // steps whose poses are uploaded with one copy
+
+}  // namespace
+
+struct lbm_ctx {
+    lbm_config cfg{};
+    Grid g{};
+    Phys<double> pd{};
+    Phys<float> pf{};
+    bool forced = false;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    void* fbuf[2] = {nullptr, nullptr};
+    int cur = 0;               // buffer holding f* (pending stream)
+    size_t fbytes = 0;
+    uint8_t* flags = nullptr;
+
+    // static boundary-node list (solidity from flags)
+    int32_t* list = nullptr;
+    int32_t* list_count = nullptr;  // device scalar
+    int64_t list_n = 0;
+    int64_t list_cap = 0;
+    int64_t nlinks = 0;
+    bool list_valid = false;
+    int32_t* cnt_a = nullptr;
+    int32_t* cnt_b = nullptr;
+    int64_t cnt_cap = 0;
+
+    // bodies
+    int nb = 0;
+    bool have_laws = false;
+    std::vector<lbm_motion> laws;
+    std::vector<lbm_body> pending;
+    bool pending_set = false;
+    bool flags_only = false;      // geometry from lbm_set_flags (no analytic bodies)
+    std::vector<DevBody> mask_bodies;  // pose that flags reflect
+    std::vector<DevBody> step_bodies;  // pose + kinematics of the current step
+    std::vector<float> reach;          // bounding half-extent per body
+    bool geom_valid = false;
+
+    // per-step records: [kChunk] x (StepGeom | DevBody new[nb] | DevBody old[nb])
+    size_t rec_bytes = 0;
+    char* d_rec = nullptr;
+    char* h_rec[2] = {nullptr, nullptr};
+    hipEvent_t rec_ev[2] = {nullptr, nullptr};
+    bool rec_ev_used[2] = {false, false};
+    int parity = 0;
+    int nrec_cap_bodies = -1;
+
+    double* d_partials = nullptr;
+    int64_t partials_cap = 0;  // doubles
+    double* d_forces = nullptr;  // [kChunk][nb][6]
+    double* h_forces[2] = {nullptr, nullptr};
+
+    // last step (for getters)
+    const char* last_rec = nullptr;
+    int last_grid = 0;
+    bool last_rec_valid = false;
+
+    int64_t step = 0;
+    bool initialised = false;
+    bool raw_state = false;  // step 0: fbuf[1-cur] holds f_old_0
+
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double bulk_ms = 0.0, link_ms = 0.0;
+    int64_t bulk_launches = 0, link_launches = 0;
+
+    int64_t device_bytes = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(lbm_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail((ctx), LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+int dev_alloc(lbm_ctx* c, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(c, LBM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->device_bytes += (int64_t)bytes;
+    return LBM_OK;
+}
+
+void dev_free(lbm_ctx* c, void* p, size_t bytes) {
+    if (p) {
+        (void)hipFree(p);
+        c->device_bytes -= (int64_t)bytes;
+    }
+}
+
+// Host pose evaluation with no contraction, so the FP32 mask pose is the same
+// bit pattern a plain C evaluation of the formula produces.
+void rotation_at(const double om[3], double t, const double R0[9], double out[9]) {
+#pragma clang fp contract(off)
+    const double w = std::sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+    double Rr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (w > 0.0) {
+        const double k[3] = {om[0] / w, om[1] / w, om[2] / w};
+        const double th = w * t, s = std::sin(th), c1 = 1.0 - std::cos(th);
+        const double K[9] = {0, -k[2], k[1], k[2], 0, -k[0], -k[1], k[0], 0};
+        double K2[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double a = 0.0;
+                for (int m = 0; m < 3; ++m) a += K[i * 3 + m] * K[m * 3 + j];
+                K2[i * 3 + j] = a;
+            }
+        for (int i = 0; i < 9; ++i) Rr[i] += s * K[i] + c1 * K2[i];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double a = 0.0;
+            for (int m = 0; m < 3; ++m) a += Rr[i * 3 + m] * R0[m * 3 + j];
+            out[i * 3 + j] = a;
+        }
+}
+
+// motion_at (src/Motion.jl:26-36) for step n with dt = 1: mask pose at n-1,
+// kinematics at n-1/2.
+lbm_body body_from_law(const lbm_motion& m, int64_t n) {
+#pragma clang fp contract(off)
+    lbm_body b;
+    std::memset(&b, 0, sizeof(b));
+    b.shape = m.shape;
+    const double t0 = (double)(n - 1), th = (double)(n - 1) + 0.5;
+    double R[9];
+    rotation_at(m.omega, t0, m.rot0, R);
+    for (int k = 0; k < 3; ++k) {
+        b.center[k] = (float)(m.center0[k] + m.vel[k] * t0);
+        b.semi[k] = (float)m.semi[k];
+        b.center_half[k] = m.center0[k] + m.vel[k] * th;
+        b.vel[k] = m.vel[k];
+        b.omega[k] = m.omega[k];
+    }
+    for (int k = 0; k < 9; ++k) b.rot[k] = (float)R[k];
+    return b;
+}
+
+DevBody to_dev(const lbm_body& b) {
+#pragma clang fp contract(off)
+    DevBody d;
+    std::memset(&d, 0, sizeof(d));
+    d.shape = b.shape;
+    for (int k = 0; k < 3; ++k) {
+        d.c[k] = b.center[k];
+        d.ch[k] = b.center_half[k];
+        d.v[k] = b.vel[k];
+        d.om[k] = b.omega[k];
+    }
+    for (int k = 0; k < 9; ++k) d.rot[k] = b.rot[k];
+    if (b.shape == LBM_SPHERE) {
+        const float R = b.semi[0];
+        d.r2 = R * R;
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            const float a = b.semi[k];
+            d.ia2[k] = 1.0f / (a * a);
+        }
+    }
+    return d;
+}
+
+float reach_of(const lbm_body& b) {
+    if (b.shape == LBM_SPHERE) return b.semi[0];
+    return std::max(b.semi[0], std::max(b.semi[1], b.semi[2]));
+}
+
+bool same_mask_pose(const DevBody& a, const DevBody& b) {
+    if (a.shape != b.shape || std::memcmp(a.c, b.c, sizeof(a.c)) != 0) return false;
+    if (a.shape == LBM_SPHERE) return a.r2 == b.r2;  // a sphere's mask ignores its orientation
+    return std::memcmp(a.ia2, b.ia2, sizeof(a.ia2)) == 0 && std::memcmp(a.rot, b.rot, sizeof(a.rot)) == 0;
+}
+
+
+// Face slabs of the non-periodic axes, disjoint by construction: axis a takes
+// the full extent of later axes and the interior of earlier non-periodic ones.
+void add_face_slabs(const lbm_ctx* c, StepGeom& sg) {
+    const int L[3] = {c->g.nx, c->g.ny, c->g.nz};
+    for (int a = 0; a < 3; ++a) {
+        if ((c->g.per >> a) & 1) continue;
+        for (int side = 0; side < 2; ++side) {
+            Box b{};
+            for (int o = 0; o < 3; ++o) {
+                if (o == a) {
+                    b.lo[o] = side ? L[o] - 1 : 0;
+                    b.ext[o] = 1;
+                } else if (o < a && !((c->g.per >> o) & 1)) {
+                    b.lo[o] = 1;
+                    b.ext[o] = L[o] - 2;
+                } else {
+                    b.lo[o] = 0;
+                    b.ext[o] = L[o];
+                }
+            }
+            b.kind = 0;
+            sg.box[sg.nbox++] = b;
+        }
+    }
+}
+
+void index_box(const lbm_ctx* c, float lo_f, float hi_f, int axis, int& lo, int& ext) {
+    const int L = axis == 0 ? c->g.nx : (axis == 1 ? c->g.ny : c->g.nz);
+    int l = (int)std::floor(lo_f) - 2, h = (int)std::ceil(hi_f) + 1;
+    if ((c->g.per >> axis) & 1) {
+        if (h - l + 1 >= L) {
+            l = 0;
+            h = L - 1;
+        } else {
+            // bring lo into [0, L): the kernel wraps lo + i < 2L with one subtraction
+            const int sh = ((l % L) + L) % L - l;
+            l += sh;
+            h += sh;
+        }
+    } else {
+        l = std::max(l, 0);
+        h = std::min(h, L - 1);
+    }
+    lo = l;
+    ext = std::max(h - l + 1, 0);
+}
+
+void finish_boxes(StepGeom& sg) {
+    int blocks = 0;
+    for (int i = 0; i < sg.nbox; ++i) {
+        sg.box[i].first_block = blocks;
+        const int64_t vol = (int64_t)sg.box[i].ext[0] * sg.box[i].ext[1] * sg.box[i].ext[2];
+        blocks += (int)nblocks_of(vol);
+    }
+    sg.nblocks = blocks;
+}
+
+LaunchReq base_req(const lbm_ctx* c, int kind, int mode) {
+    LaunchReq r;
+    std::memset(&r, 0, sizeof(r));
+    r.kind = kind;
+    r.mode = mode;
+    r.collision = c->cfg.collision;
+    r.forced = c->forced ? 1 : 0;
+    r.g = c->g;
+    r.p = c->pd;
+    r.stream = c->stream;
+    r.nb = c->nb;
+    r.flags = c->flags;
+    r.list = c->list;
+    r.list_count = c->list_count;
+    r.partials = c->d_partials;
+    return r;
+}
+
+int do_launch(lbm_ctx* c, const LaunchReq& r) {
+    int e = 0;
+    switch (c->cfg.storage) {
+        case LBM_FP64: e = launch<StoreF64>(r); break;
+        case LBM_FP32: e = launch<StoreF32>(r); break;
+        case LBM_FP16S: e = launch<StoreF16S>(r); break;
+        case LBM_BF16S: e = launch<StoreBF16S>(r); break;
+        default: return fail(c, LBM_ERR_ARG, "unknown storage");
+    }
+    if (e) return fail(c, LBM_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString((hipError_t)e));
+    return LBM_OK;
+}
+
+int launch_bulk(lbm_ctx* c, int mode, const void* fin, void* fout, double* gout, double* rho, double* u) {
+    LaunchReq r = base_req(c, K_BULK, mode);
+    r.fin = fin;
+    r.fout = fout;
+    r.gout = gout;
+    r.rho_out = rho;
+    r.u_out = u;
+    return do_launch(c, r);
+}
+
+int launch_fixup(lbm_ctx* c, int mode, int grid_blocks, const void* fin, void* fout, double* gout, double* rho,
+                 double* u, const char* rec, int write_flags) {
+    if (grid_blocks <= 0) return LBM_OK;
+    LaunchReq r = base_req(c, K_FIXUP, mode);
+    r.fin = fin;
+    r.fout = fout;
+    r.gout = gout;
+    r.rho_out = rho;
+    r.u_out = u;
+    r.grid_blocks = grid_blocks;
+    r.rec = rec;
+    r.write_flags = write_flags;
+    return do_launch(c, r);
+}
+
+#define GEOCHK(ctx, call)                                                                                   \
+    do {                                                                                                    \
+        int e_ = (call);                                                                                    \
+        if (e_) return fail((ctx), LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString((hipError_t)e_)); \
+    } while (0)
+
+int ensure_records(lbm_ctx* c) {
+    if (c->nrec_cap_bodies == c->nb && c->d_rec) return LBM_OK;
+    if (c->d_rec) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dev_free(c, c->d_rec, (size_t)kChunk * c->rec_bytes);
+        for (int i = 0; i < 2; ++i) {
+            (void)hipHostFree(c->h_rec[i]);
+            (void)hipHostFree(c->h_forces[i]);
+            c->h_rec[i] = nullptr;
+            c->h_forces[i] = nullptr;
+        }
+        dev_free(c, c->d_forces, 0);
+        c->d_forces = nullptr;
+    }
+    c->rec_bytes = (sizeof(StepGeom) + 2 * (size_t)std::max(c->nb, 1) * sizeof(DevBody) + 255) / 256 * 256;
+    int rc = dev_alloc(c, (void**)&c->d_rec, (size_t)kChunk * c->rec_bytes);
+    if (rc) return rc;
+    const size_t fbytes = (size_t)kChunk * std::max(c->nb, 1) * 6 * sizeof(double);
+    rc = dev_alloc(c, (void**)&c->d_forces, fbytes);
+    if (rc) return rc;
+    for (int i = 0; i < 2; ++i) {
+        HIPCHK(c, hipHostMalloc((void**)&c->h_rec[i], (size_t)kChunk * c->rec_bytes, hipHostMallocDefault));
+        HIPCHK(c, hipHostMalloc((void**)&c->h_forces[i], fbytes, hipHostMallocDefault));
+        std::memset(c->h_rec[i], 0, (size_t)kChunk * c->rec_bytes);
+    }
+    c->nrec_cap_bodies = c->nb;
+    c->last_rec_valid = false;
+    return LBM_OK;
+}
+
+int ensure_partials(lbm_ctx* c, int64_t blocks) {
+    const int64_t need = blocks * std::max(c->nb, 1) * 6;
+    if (need <= c->partials_cap) return LBM_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_free(c, c->d_partials, (size_t)c->partials_cap * sizeof(double));
+    c->d_partials = nullptr;
+    const int64_t cap = std::max(need, c->partials_cap * 2);
+    int rc = dev_alloc(c, (void**)&c->d_partials, (size_t)cap * sizeof(double));
+    if (rc) return rc;
+    c->partials_cap = cap;
+    return LBM_OK;
+}
+
+// Ordered compaction of the boundary nodes (and optionally the links) from flags.
+int build_list(lbm_ctx* c, int32_t* d_link_cell, int8_t* d_link_dir, int64_t* nlinks_out) {
+    const int64_t nblk = nblocks_of(c->g.N);
+    if (nblk + 1 > c->cnt_cap) {
+        dev_free(c, c->cnt_a, (size_t)c->cnt_cap * 4);
+        dev_free(c, c->cnt_b, (size_t)c->cnt_cap * 4);
+        c->cnt_a = c->cnt_b = nullptr;
+        c->cnt_cap = nblk + 1;
+        int rc = dev_alloc(c, (void**)&c->cnt_a, (size_t)c->cnt_cap * 4);
+        if (rc) return rc;
+        rc = dev_alloc(c, (void**)&c->cnt_b, (size_t)c->cnt_cap * 4);
+        if (rc) return rc;
+    }
+    GEOCHK(c, launch_bb_count(c->flags, c->g, c->cnt_a, c->cnt_b, c->stream));
+    GEOCHK(c, launch_scan2(c->cnt_a, c->cnt_b, (int)nblk, c->stream));
+    int32_t tot[2];
+    HIPCHK(c, hipMemcpyAsync(&tot[0], c->cnt_a + nblk, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&tot[1], c->cnt_b + nblk, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t nn = tot[0];
+    if (nn > c->list_cap) {
+        dev_free(c, c->list, (size_t)c->list_cap * 4);
+        c->list = nullptr;
+        c->list_cap = std::max<int64_t>(nn, 1024);
+        int rc = dev_alloc(c, (void**)&c->list, (size_t)c->list_cap * 4);
+        if (rc) return rc;
+    }
+    if (d_link_cell && *nlinks_out < tot[1]) {
+        *nlinks_out = tot[1];
+        return LBM_ERR_CAPACITY;
+    }
+    GEOCHK(c, launch_bb_fill(c->flags, c->g, c->cnt_a, c->cnt_b, c->list, d_link_cell, d_link_dir, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->list_count, c->cnt_a + nblk, 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipGetLastError());
+    c->list_n = nn;
+    c->nlinks = tot[1];
+    if (nlinks_out) *nlinks_out = tot[1];
+    return LBM_OK;
+}
+
+// Full build of flags (from the analytic bodies) and of the static node list.
+int full_geometry(lbm_ctx* c, const std::vector<DevBody>& bodies) {
+    DevBody* d_b = nullptr;
+    int rc = dev_alloc(c, (void**)&d_b, std::max<size_t>(bodies.size(), 1) * sizeof(DevBody));
+    if (rc) return rc;
+    if (!bodies.empty())
+        HIPCHK(c, hipMemcpyAsync(d_b, bodies.data(), bodies.size() * sizeof(DevBody), hipMemcpyHostToDevice, c->stream));
+    GEOCHK(c, launch_flags_full(c->flags, d_b, (int)bodies.size(), c->g, c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_free(c, d_b, std::max<size_t>(bodies.size(), 1) * sizeof(DevBody));
+    c->mask_bodies = bodies;
+    c->geom_valid = true;
+    rc = build_list(c, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    c->list_valid = true;
+    return LBM_OK;
+}
+
+// Bodies for step n (1-based) from the law or the pending set.
+void bodies_for_step(lbm_ctx* c, int64_t n, std::vector<DevBody>& out) {
+    out.clear();
+    if (c->have_laws) {
+        for (const auto& m : c->laws) out.push_back(to_dev(body_from_law(m, n)));
+    } else if (c->pending_set) {
+        for (const auto& b : c->pending) out.push_back(to_dev(b));
+    } else {
+        out = c->step_bodies;
+    }
+}
+
+constexpr int kStopChunk = 1;
+
+// Fills the record of one step.  A step that needs a synchronous rebuild (full
+// flags, or the static list after a motion phase) returns kStopChunk unless
+// allow_sync, so that the rebuild sees the flags of every earlier step.
+// Returns the fix-up grid size via *grid.
+int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, bool allow_sync) {
+    StepGeom sg;
+    std::memset(&sg, 0, sizeof(sg));
+    add_face_slabs(c, sg);
+    sg.nbodies = c->nb;
+    std::vector<DevBody> bnew;
+    if (!c->flags_only) bodies_for_step(c, n, bnew);
+    else bnew = c->step_bodies;
+    std::vector<DevBody> bold;
+    bool moving = false;
+    if (c->nb > 0 && !c->flags_only) {
+        if (!c->geom_valid) {
+            if (!allow_sync) return kStopChunk;
+            int rc = full_geometry(c, bnew);
+            if (rc) return rc;
+        } else {
+            for (int i = 0; i < c->nb; ++i)
+                if (!same_mask_pose(bnew[i], c->mask_bodies[i])) moving = true;
+        }
+    }
+    if (!moving && c->nb > 0 && !c->list_valid && !allow_sync) return kStopChunk;
+    if (moving) {
+        sg.mode = 1;
+        bold = c->mask_bodies;
+        // body regions: union of the old and new bounding boxes, dilated
+        for (int i = 0; i < c->nb; ++i) {
+            Box b{};
+            const float rr = c->reach[i];
+            for (int a = 0; a < 3; ++a) {
+                const float lo_f = std::min(bnew[i].c[a], bold[i].c[a]) - rr;
+                const float hi_f = std::max(bnew[i].c[a], bold[i].c[a]) + rr;
+                index_box(c, lo_f, hi_f, a, b.lo[a], b.ext[a]);
+            }
+            b.kind = 1;
+            if ((int64_t)b.ext[0] * b.ext[1] * b.ext[2] > 0) sg.box[sg.nbox++] = b;
+        }
+        finish_boxes(sg);
+        sg.list_blocks = 0;
+        c->mask_bodies = bnew;
+        c->list_valid = false;
+    } else {
+        sg.mode = 0;
+        if (c->nb > 0 && !c->list_valid) {
+            int rc = build_list(c, nullptr, nullptr, nullptr);
+            if (rc) return rc;
+            c->list_valid = true;
+        }
+        finish_boxes(sg);
+        sg.list_blocks = c->nb > 0 ? (int)nblocks_of(c->list_n) : 0;
+        bold = bnew;
+    }
+    c->step_bodies = bnew;
+    std::memcpy(rec, &sg, sizeof(sg));
+    DevBody* bn = (DevBody*)(rec + sizeof(StepGeom));
+    for (int i = 0; i < c->nb; ++i) {
+        bn[i] = bnew[i];
+        bn[c->nb + i] = bold[i];
+    }
+    *grid = sg.nblocks + sg.list_blocks;
+    return LBM_OK;
+}
+
+hipEvent_t next_event(lbm_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
+    // the pinned buffers of this parity are free once their last copy finished
+    const int par = c->parity;
+    if (c->rec_ev_used[par]) HIPCHK(c, hipEventSynchronize(c->rec_ev[par]));
+    std::vector<int> grids(nsteps);
+    int nprep = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = prepare_step(c, c->step + 1 + s, c->h_rec[par] + (size_t)s * c->rec_bytes, &grids[s], s == 0);
+        if (rc == kStopChunk) break;
+        if (rc) return rc;
+        ++nprep;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_rec, c->h_rec[par], (size_t)nprep * c->rec_bytes, hipMemcpyHostToDevice, c->stream));
+    int maxgrid = 0;
+    for (int s = 0; s < nprep; ++s) maxgrid = std::max(maxgrid, grids[s]);
+    int rc = ensure_partials(c, std::max(maxgrid, 1));
+    if (rc) return rc;
+    const int n6 = c->nb * 6;
+    for (int s = 0; s < nprep; ++s) {
+        const char* rec = c->d_rec + (size_t)s * c->rec_bytes;
+        void* fin = c->fbuf[c->cur];
+        void* fout = c->fbuf[1 - c->cur];
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+        if (c->profiling) {
+            e0 = next_event(c);
+            e1 = next_event(c);
+            e2 = next_event(c);
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+        }
+        int lr = launch_bulk(c, MODE_STEP, fin, fout, nullptr, nullptr, nullptr);
+        if (lr) return lr;
+        if (c->profiling) HIPCHK(c, hipEventRecord(e1, c->stream));
+        lr = launch_fixup(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, 1);
+        if (lr) return lr;
+        if (c->profiling) HIPCHK(c, hipEventRecord(e2, c->stream));
+        if (n6 > 0) {
+            if (grids[s] > 0) {
+                GEOCHK(c, launch_force_finalize(c->d_partials, grids[s], n6, c->d_forces + (size_t)s * n6, c->stream));
+            } else {
+                HIPCHK(c, hipMemsetAsync(c->d_forces + (size_t)s * n6, 0, (size_t)n6 * sizeof(double), c->stream));
+            }
+        }
+        HIPCHK(c, hipGetLastError());
+        c->cur = 1 - c->cur;
+        c->last_rec = rec;
+        c->last_grid = grids[s];
+        c->last_rec_valid = true;
+        c->raw_state = false;
+    }
+    if (force_out && n6 > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->h_forces[par], c->d_forces, (size_t)nprep * n6 * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipEventRecord(c->rec_ev[par], c->stream));
+    c->rec_ev_used[par] = true;
+    c->parity = 1 - par;
+    c->step += nprep;
+    c->pending_set = false;
+    *done = nprep;
+    return LBM_OK;
+}
+
+int collide_only(lbm_ctx* c) {
+    return launch_bulk(c, MODE_COLLIDE_ONLY, c->fbuf[1 - c->cur], c->fbuf[c->cur], nullptr, nullptr, nullptr);
+}
+
+// Pulled state f_old and moments of the last completed step.
+int get_state(lbm_ctx* c, double* d_f, double* d_rho, double* d_u) {
+    if (!c->initialised) return fail(c, LBM_ERR_STATE, "not initialised");
+    const void* prev = c->fbuf[1 - c->cur];
+    if (c->raw_state) return launch_bulk(c, MODE_GET_RAW, prev, nullptr, d_f, d_rho, d_u);
+    int rc = launch_bulk(c, MODE_GET, prev, nullptr, d_f, d_rho, d_u);
+    if (rc == LBM_OK && c->last_rec_valid)
+        rc = launch_fixup(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
+    if (!cfg || !out) return LBM_ERR_ARG;
+    *out = nullptr;
+    if (cfg->collision != LBM_BGK && cfg->collision != LBM_TRT) return LBM_ERR_ARG;
+    if (cfg->storage < LBM_FP64 || cfg->storage > LBM_BF16S) return LBM_ERR_ARG;
+    const int L[3] = {cfg->nx, cfg->ny, cfg->nz};
+    for (int a = 0; a < 3; ++a) {
+        const int k0 = cfg->faces[2 * a].kind, k1 = cfg->faces[2 * a + 1].kind;
+        if (k0 < 0 || k0 > 3 || k1 < 0 || k1 > 3) return LBM_ERR_ARG;
+        if ((k0 == LBM_FACE_PERIODIC) != (k1 == LBM_FACE_PERIODIC)) return LBM_ERR_ARG;
+        if (L[a] < (k0 == LBM_FACE_PERIODIC ? 2 : 3)) return LBM_ERR_ARG;
+    }
+    const int64_t N = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+    if (N >= ((int64_t)1 << 31)) return LBM_ERR_ARG;  // cell indices are int32 in the lists
+    if (!(cfg->omega_plus > 0.0 && cfg->omega_plus < 2.0)) return LBM_ERR_ARG;
+    if (cfg->collision == LBM_TRT && !(cfg->omega_minus > 0.0 && cfg->omega_minus < 2.0)) return LBM_ERR_ARG;
+
+    lbm_ctx* c = new lbm_ctx();
+    c->cfg = *cfg;
+    c->g.nx = cfg->nx;
+    c->g.ny = cfg->ny;
+    c->g.nz = cfg->nz;
+    c->g.N = N;
+    c->g.nxy = (int64_t)cfg->nx * cfg->ny;
+    c->g.per = 0;
+    for (int a = 0; a < 3; ++a)
+        if (cfg->faces[2 * a].kind == LBM_FACE_PERIODIC) c->g.per |= 1 << a;
+    for (int f = 0; f < 6; ++f) {
+        c->g.face_kind[f] = cfg->faces[f].kind;
+        for (int k = 0; k < 3; ++k) c->g.face_u[f][k] = cfg->faces[f].u[k];
+    }
+    c->g.wall_point = cfg->wall_point;
+    c->g.refill = cfg->refill;
+    c->g.rho0 = cfg->rho0;
+    c->forced = cfg->g[0] != 0.0 || cfg->g[1] != 0.0 || cfg->g[2] != 0.0;
+    c->pd.wp = cfg->omega_plus;
+    c->pd.wm = cfg->collision == LBM_TRT ? cfg->omega_minus : cfg->omega_plus;
+    c->pf.wp = (float)c->pd.wp;
+    c->pf.wm = (float)c->pd.wm;
+    for (int k = 0; k < 3; ++k) {
+        c->pd.g[k] = cfg->g[k];
+        c->pf.g[k] = (float)cfg->g[k];
+    }
+    const int bytes = cfg->storage == LBM_FP64 ? 8 : (cfg->storage == LBM_FP32 ? 4 : 2);
+    c->fbytes = (size_t)19 * N * bytes;
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LBM_ERR_HIP;
+    }
+    int rc = LBM_OK;
+    for (int i = 0; i < 2 && rc == LBM_OK; ++i) rc = dev_alloc(c, &c->fbuf[i], c->fbytes);
+    if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->flags, (size_t)N);
+    if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->list_count, 16);
+    if (rc == LBM_OK) {
+        if (hipMemsetAsync(c->flags, 0, (size_t)N, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->list_count, 0, 16, c->stream) != hipSuccess)
+            rc = LBM_ERR_HIP;
+    }
+    for (int i = 0; i < 2 && rc == LBM_OK; ++i)
+        if (hipEventCreateWithFlags(&c->rec_ev[i], hipEventDisableTiming) != hipSuccess) rc = LBM_ERR_HIP;
+    if (rc == LBM_OK) rc = ensure_records(c);
+    if (rc == LBM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = LBM_ERR_HIP;
+    if (rc != LBM_OK) {
+        lbm_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx* c) {
+    if (!c) return LBM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 2; ++i) {
+        if (c->fbuf[i]) (void)hipFree(c->fbuf[i]);
+        if (c->h_rec[i]) (void)hipHostFree(c->h_rec[i]);
+        if (c->h_forces[i]) (void)hipHostFree(c->h_forces[i]);
+        if (c->rec_ev[i]) (void)hipEventDestroy(c->rec_ev[i]);
+    }
+    void* ptrs[] = {c->flags, c->list, c->list_count, c->cnt_a, c->cnt_b, c->d_rec, c->d_partials, c->d_forces};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LBM_OK;
+}
+
+int lbm_last_error(const lbm_ctx* c, char* buf, size_t n) {
+    if (!c || !buf || n == 0) return LBM_ERR_ARG;
+    std::snprintf(buf, n, "%s", c->err.c_str());
+    return LBM_OK;
+}
+
+static int set_body_count(lbm_ctx* c, int n) {
+    if (n < 0 || n > LBM_MAX_BODIES) return fail(c, LBM_ERR_ARG, "body count out of range");
+    if (n != c->nb) {
+        c->nb = n;
+        c->geom_valid = false;
+        c->list_valid = false;
+        int rc = ensure_records(c);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
+int lbm_set_bodies(lbm_ctx* c, int32_t n, const lbm_body* bodies) {
+    if (!c || (n > 0 && !bodies)) return LBM_ERR_ARG;
+    for (int i = 0; i < n; ++i) {
+        if (bodies[i].shape != LBM_SPHERE && bodies[i].shape != LBM_ELLIPSOID)
+            return fail(c, LBM_ERR_ARG, "unknown body shape");
+        const int ns = bodies[i].shape == LBM_SPHERE ? 1 : 3;
+        for (int k = 0; k < ns; ++k)
+            if (!(bodies[i].semi[k] > 0.0f)) return fail(c, LBM_ERR_ARG, "body size must be positive");
+    }
+    if (c->flags_only) {
+        c->flags_only = false;
+        c->geom_valid = false;
+    }
+    int rc = set_body_count(c, n);
+    if (rc) return rc;
+    c->have_laws = false;
+    c->laws.clear();
+    c->pending.assign(bodies, bodies + n);
+    c->pending_set = true;
+    c->reach.clear();
+    for (int i = 0; i < n; ++i) c->reach.push_back(reach_of(bodies[i]));
+    return LBM_OK;
+}
+
+int lbm_set_motion(lbm_ctx* c, int32_t n, const lbm_motion* laws) {
+    if (!c || (n > 0 && !laws)) return LBM_ERR_ARG;
+    for (int i = 0; i < n; ++i) {
+        if (laws[i].shape != LBM_SPHERE && laws[i].shape != LBM_ELLIPSOID)
+            return fail(c, LBM_ERR_ARG, "unknown body shape");
+        const int ns = laws[i].shape == LBM_SPHERE ? 1 : 3;
+        for (int k = 0; k < ns; ++k)
+            if (!(laws[i].semi[k] > 0.0)) return fail(c, LBM_ERR_ARG, "body size must be positive");
+    }
+    if (c->flags_only) {
+        c->flags_only = false;
+        c->geom_valid = false;
+    }
+    int rc = set_body_count(c, n);
+    if (rc) return rc;
+    c->laws.assign(laws, laws + n);
+    c->have_laws = n > 0;
+    c->pending.clear();
+    c->pending_set = false;
+    c->reach.clear();
+    for (int i = 0; i < n; ++i) {
+        const lbm_motion& m = laws[i];
+        c->reach.push_back(m.shape == LBM_SPHERE ? (float)m.semi[0]
+                                                 : (float)std::max(m.semi[0], std::max(m.semi[1], m.semi[2])));
+    }
+    return LBM_OK;
+}
+
+int lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
+    if (!c || !flags) return LBM_ERR_ARG;
+    int maxf = 0;
+    for (int64_t i = 0; i < c->g.N; ++i) maxf = std::max<int>(maxf, flags[i]);
+    if (maxf > LBM_MAX_BODIES) return fail(c, LBM_ERR_ARG, "flag value above the body limit");
+    int rc = set_body_count(c, maxf);
+    if (rc) return rc;
+    c->have_laws = false;
+    c->laws.clear();
+    c->pending.clear();
+    c->pending_set = false;
+    c->flags_only = true;
+    // bodies without kinematics: at rest, centre at the origin
+    c->reach.assign(maxf, 0.0f);
+    c->step_bodies.assign(maxf, DevBody{});
+    c->mask_bodies = c->step_bodies;
+    HIPCHK(c, hipMemcpyAsync(c->flags, flags, (size_t)c->g.N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->geom_valid = true;
+    rc = build_list(c, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    c->list_valid = true;
+    c->last_rec_valid = false;
+    return LBM_OK;
+}
+
+int lbm_update_geometry(lbm_ctx* c) {
+    if (!c) return LBM_ERR_ARG;
+    if (c->flags_only || c->nb == 0) return LBM_OK;
+    std::vector<DevBody> b;
+    bodies_for_step(c, c->step + 1, b);
+    int rc = full_geometry(c, b);
+    if (rc) return rc;
+    c->step_bodies = b;
+    c->last_rec_valid = false;
+    return LBM_OK;
+}
+
+int lbm_get_flags(lbm_ctx* c, uint8_t* flags) {
+    if (!c || !flags) return LBM_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(flags, c->flags, (size_t)c->g.N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_get_links(lbm_ctx* c, int32_t* cell, int8_t* dir, int64_t cap, int64_t* nnz) {
+    if (!c) return LBM_ERR_ARG;
+    // count first
+    int64_t need = 0;
+    int rc = build_list(c, nullptr, nullptr, &need);
+    if (rc) return rc;
+    c->list_valid = true;
+    if (nnz) *nnz = need;
+    if (!cell && !dir) return LBM_OK;
+    if (need > cap) return fail(c, LBM_ERR_CAPACITY, "link buffer too small");
+    int32_t* d_cell = nullptr;
+    int8_t* d_dir = nullptr;
+    rc = dev_alloc(c, (void**)&d_cell, (size_t)std::max<int64_t>(need, 1) * 4);
+    if (rc) return rc;
+    rc = dev_alloc(c, (void**)&d_dir, (size_t)std::max<int64_t>(need, 1));
+    if (rc) return rc;
+    int64_t have = need;
+    rc = build_list(c, d_cell, d_dir, &have);
+    if (rc == LBM_OK) {
+        if (cell && need) HIPCHK(c, hipMemcpyAsync(cell, d_cell, (size_t)need * 4, hipMemcpyDeviceToHost, c->stream));
+        if (dir && need) HIPCHK(c, hipMemcpyAsync(dir, d_dir, (size_t)need, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    dev_free(c, d_cell, (size_t)std::max<int64_t>(need, 1) * 4);
+    dev_free(c, d_dir, (size_t)std::max<int64_t>(need, 1));
+    return rc;
+}
+
+int lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* u) {
+    if (!c) return LBM_ERR_ARG;
+    const int64_t N = c->g.N;
+    double* d_rho = nullptr;
+    double* d_u = nullptr;
+    int rc = LBM_OK;
+    if (rho) rc = dev_alloc(c, (void**)&d_rho, (size_t)N * 8);
+    if (rc == LBM_OK && u) rc = dev_alloc(c, (void**)&d_u, (size_t)3 * N * 8);
+    if (rc) return rc;
+    if (rho) HIPCHK(c, hipMemcpyAsync(d_rho, rho, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+    if (u) HIPCHK(c, hipMemcpyAsync(d_u, u, (size_t)3 * N * 8, hipMemcpyHostToDevice, c->stream));
+    {
+        LaunchReq r = base_req(c, K_EQUILIBRIUM, 0);
+        r.fout = c->fbuf[1 - c->cur];
+        r.src_rho = d_rho;
+        r.src_u = d_u;
+        r.rho0 = c->cfg.rho0;
+        for (int k = 0; k < 3; ++k) r.u0[k] = c->cfg.u0[k];
+        rc = do_launch(c, r);
+    }
+    if (rc == LBM_OK) rc = collide_only(c);
+    if (rc == LBM_OK) HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_free(c, d_rho, rho ? (size_t)N * 8 : 0);
+    dev_free(c, d_u, u ? (size_t)3 * N * 8 : 0);
+    if (rc) return rc;
+    c->step = 0;
+    c->initialised = true;
+    c->raw_state = true;
+    c->last_rec_valid = false;
+    return LBM_OK;
+}
+
+int lbm_set_populations(lbm_ctx* c, const double* f) {
+    if (!c || !f) return LBM_ERR_ARG;
+    const int64_t N = c->g.N;
+    double* d = nullptr;
+    int rc = dev_alloc(c, (void**)&d, (size_t)19 * N * 8);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(d, f, (size_t)19 * N * 8, hipMemcpyHostToDevice, c->stream));
+    {
+        LaunchReq r = base_req(c, K_FROM_DOUBLE, 0);
+        r.fout = c->fbuf[1 - c->cur];
+        r.src_f = d;
+        rc = do_launch(c, r);
+    }
+    if (rc == LBM_OK) rc = collide_only(c);
+    if (rc == LBM_OK) HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_free(c, d, (size_t)19 * N * 8);
+    if (rc) return rc;
+    c->step = 0;
+    c->initialised = true;
+    c->raw_state = true;
+    c->last_rec_valid = false;
+    return LBM_OK;
+}
+
+int lbm_step(lbm_ctx* c, int32_t nsteps, double* force_out) {
+    if (!c || nsteps < 0) return LBM_ERR_ARG;
+    if (!c->initialised) return fail(c, LBM_ERR_STATE, "lbm_step before lbm_init_equilibrium");
+    int64_t done_total = 0;
+    const int n6 = c->nb * 6;
+    int64_t chunk_off[2] = {-1, -1};  // first step (within this call) of the chunk held by each parity
+    int chunk_cnt[2] = {0, 0};
+    int rc = [&]() -> int {
+        while (done_total < nsteps) {
+            const int par = c->parity;
+            if (chunk_off[par] >= 0) {
+                HIPCHK(c, hipEventSynchronize(c->rec_ev[par]));
+                if (force_out && n6 > 0)
+                    std::memcpy(force_out + (size_t)chunk_off[par] * n6, c->h_forces[par],
+                                (size_t)chunk_cnt[par] * n6 * sizeof(double));
+                chunk_off[par] = -1;
+            }
+            const int k = (int)std::min<int64_t>(kChunk, nsteps - done_total);
+            int64_t done = 0;
+            int r = run_chunk(c, k, force_out, &done);
+            if (r) return r;
+            chunk_off[par] = done_total;
+            chunk_cnt[par] = (int)done;
+            done_total += done;
+            if (c->profiling) {
+                // per-kernel event pairs of this chunk: read after it drained
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                for (size_t i = 0; i + 3 <= c->ev_used; i += 3) {
+                    float a = 0.f, b = 0.f;
+                    HIPCHK(c, hipEventElapsedTime(&a, c->ev_pool[i], c->ev_pool[i + 1]));
+                    HIPCHK(c, hipEventElapsedTime(&b, c->ev_pool[i + 1], c->ev_pool[i + 2]));
+                    c->bulk_ms += a;
+                    ++c->bulk_launches;
+                    c->link_ms += b;
+                    ++c->link_launches;
+                }
+                c->ev_used = 0;
+            }
+        }
+        return (int)LBM_OK;
+    }();
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int par = 0; par < 2; ++par) {
+        if (chunk_off[par] >= 0 && force_out && n6 > 0)
+            std::memcpy(force_out + (size_t)chunk_off[par] * n6, c->h_forces[par],
+                        (size_t)chunk_cnt[par] * n6 * sizeof(double));
+    }
+    return LBM_OK;
+}
+
+int lbm_get_macros(lbm_ctx* c, double* rho, double* u) {
+    if (!c) return LBM_ERR_ARG;
+    const int64_t N = c->g.N;
+    double *d_rho = nullptr, *d_u = nullptr;
+    int rc = dev_alloc(c, (void**)&d_rho, (size_t)N * 8);
+    if (rc == LBM_OK) rc = dev_alloc(c, (void**)&d_u, (size_t)3 * N * 8);
+    if (rc == LBM_OK) rc = get_state(c, nullptr, d_rho, d_u);
+    if (rc == LBM_OK) {
+        if (rho) HIPCHK(c, hipMemcpyAsync(rho, d_rho, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
+        if (u) HIPCHK(c, hipMemcpyAsync(u, d_u, (size_t)3 * N * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    dev_free(c, d_rho, (size_t)N * 8);
+    dev_free(c, d_u, (size_t)3 * N * 8);
+    return rc;
+}
+
+int lbm_get_populations(lbm_ctx* c, double* f) {
+    if (!c || !f) return LBM_ERR_ARG;
+    const int64_t N = c->g.N;
+    double* d = nullptr;
+    int rc = dev_alloc(c, (void**)&d, (size_t)19 * N * 8);
+    if (rc == LBM_OK) rc = get_state(c, d, nullptr, nullptr);
+    if (rc == LBM_OK) {
+        HIPCHK(c, hipMemcpyAsync(f, d, (size_t)19 * N * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    dev_free(c, d, (size_t)19 * N * 8);
+    return rc;
+}
+
+int lbm_conservation(lbm_ctx* c, double* mass, double* mom3) {
+    if (!c) return LBM_ERR_ARG;
+    const int64_t N = c->g.N;
+    const int64_t nblk = nblocks_of(N);
+    double *d_rho = nullptr, *d_u = nullptr, *d_p = nullptr;
+    int rc = dev_alloc(c, (void**)&d_rho, (size_t)N * 8);
+    if (rc == LBM_OK) rc = dev_alloc(c, (void**)&d_u, (size_t)3 * N * 8);
+    if (rc == LBM_OK) rc = dev_alloc(c, (void**)&d_p, (size_t)nblk * 4 * 8);
+    if (rc == LBM_OK) rc = get_state(c, nullptr, d_rho, d_u);
+    std::vector<double> h((size_t)nblk * 4);
+    if (rc == LBM_OK) {
+        GEOCHK(c, launch_conservation(d_rho, d_u, c->flags, N, d_p, c->stream));
+        HIPCHK(c, hipMemcpyAsync(h.data(), d_p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double s[4] = {0, 0, 0, 0};
+        for (int64_t b = 0; b < nblk; ++b)
+            for (int k = 0; k < 4; ++k) s[k] += h[(size_t)b * 4 + k];
+        if (mass) *mass = s[0];
+        if (mom3) {
+            mom3[0] = s[1];
+            mom3[1] = s[2];
+            mom3[2] = s[3];
+        }
+    }
+    dev_free(c, d_rho, (size_t)N * 8);
+    dev_free(c, d_u, (size_t)3 * N * 8);
+    dev_free(c, d_p, (size_t)nblk * 4 * 8);
+    return rc;
+}
+
+int lbm_info(lbm_ctx* c, int64_t* ncells, int64_t* nbb, int64_t* nlinks, int64_t* steps, int64_t* device_bytes) {
+    if (!c) return LBM_ERR_ARG;
+    if (ncells) *ncells = c->g.N;
+    if (nbb) *nbb = c->list_valid ? c->list_n : -1;
+    if (nlinks) *nlinks = c->list_valid ? c->nlinks : -1;
+    if (steps) *steps = c->step;
+    if (device_bytes) *device_bytes = c->device_bytes;
+    return LBM_OK;
+}
+
+int lbm_profile(lbm_ctx* c, int32_t enable) {
+    if (!c) return LBM_ERR_ARG;
+    c->profiling = enable != 0;
+    c->bulk_ms = c->link_ms = 0.0;
+    c->bulk_launches = c->link_launches = 0;
+    return LBM_OK;
+}
+
+int lbm_profile_read(lbm_ctx* c, double* bulk_ms, int64_t* bulk_launches, double* link_ms, int64_t* link_launches) {
+    if (!c) return LBM_ERR_ARG;
+    if (bulk_ms) *bulk_ms = c->bulk_ms;
+    if (bulk_launches) *bulk_launches = c->bulk_launches;
+    if (link_ms) *link_ms = c->link_ms;
+    if (link_launches) *link_launches = c->link_launches;
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_ctx* c) {
+    if (!c) return LBM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+}  // extern "C"

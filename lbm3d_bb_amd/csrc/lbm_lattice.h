@@ -1,0 +1,73 @@
+// lbm_lattice.h -- D3Q19 tables and population storage formats for gfx950.
+//
+// Direction order is the reference's (src/Utilities.jl:950-960) minus one, so a
+// population index d here is the reference's d+1 and bbOpp maps to kOpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace lbm {
+
+constexpr int Q = 19;
+constexpr int kCx[Q] = {1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0};
+constexpr int kCy[Q] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 0};
+constexpr int kCz[Q] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 0};
+constexpr int kOpp[Q] = {1, 0, 3, 2, 5, 4, 7, 6, 9, 8, 11, 10, 13, 12, 15, 14, 17, 16, 18};
+
+// Weights in the compute type, each the correctly rounded value of 1/18, 1/36 or
+// 1/3 (the reference stores Float32.(1//18) etc., src/Utilities.jl:954-958).
+template <typename C>
+__host__ __device__ constexpr C weight(int d) {
+    return d < 6 ? C(1) / C(18) : (d < 18 ? C(1) / C(36) : C(1) / C(3));
+}
+
+// ---------------------------------------------------------------------------
+// Storage formats.  Each exposes the stored type T, the compute type C and
+// load/store of one population.  The shifted 16-bit formats keep f_d - w_d (the
+// deviation from the rest state at rho0 = 1), so the 16-bit mantissa spends its
+// bits on the O(Ma) deviation instead of on w_d itself.
+// ---------------------------------------------------------------------------
+struct StoreF64 {
+    using T = double;
+    using C = double;
+    static constexpr int kBytes = 8;
+    __device__ static inline C load(const T* p, int d) { (void)d; return *p; }
+    __device__ static inline void store(T* p, int d, C v) { (void)d; *p = v; }
+};
+
+struct StoreF32 {
+    using T = float;
+    using C = float;
+    static constexpr int kBytes = 4;
+    __device__ static inline C load(const T* p, int d) { (void)d; return *p; }
+    __device__ static inline void store(T* p, int d, C v) { (void)d; *p = v; }
+};
+
+struct StoreF16S {
+    using T = __half;
+    using C = float;
+    static constexpr int kBytes = 2;
+    __device__ static inline C load(const T* p, int d) { return __half2float(*p) + weight<float>(d); }
+    // __float2half_rn: round to nearest even (not the round-toward-zero of the
+    // packed v_cvt_pkrtz form, which would bias the deviations).
+    __device__ static inline void store(T* p, int d, C v) { *p = __float2half_rn(v - weight<float>(d)); }
+};
+
+struct StoreBF16S {
+    using T = uint16_t;
+    using C = float;
+    static constexpr int kBytes = 2;
+    __device__ static inline float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+    __device__ static inline uint16_t f32_to_bf16_rne(float f) {
+        uint32_t u = __float_as_uint(f);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);  // quiet NaN
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return uint16_t(u >> 16);
+    }
+    __device__ static inline C load(const T* p, int d) { return bf16_to_f32(*p) + weight<float>(d); }
+    __device__ static inline void store(T* p, int d, C v) { *p = f32_to_bf16_rne(v - weight<float>(d)); }
+};
+
+}  // namespace lbm

@@ -1,0 +1,73 @@
+// lbm_launch.h -- launch<S>(): the kernels of one storage format.  Included
+// once per storage by lbm_store_*.hip, which instantiates it.
+#pragma once
+
+#include "lbm_device.h"
+
+namespace lbm {
+
+template <class S, int MODE, int COLL, bool FORCED>
+static void launch_bulk_t(const LaunchReq& r, const Phys<typename S::C>& p) {
+    using T = typename S::T;
+    hipLaunchKernelGGL((k_bulk<S, COLL, FORCED, MODE>), dim3((unsigned)nblocks_of(r.g.N)), dim3(kBlock), 0, r.stream,
+                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p);
+}
+
+template <class S, int MODE, int COLL, bool FORCED>
+static void launch_fixup_t(const LaunchReq& r, const Phys<typename S::C>& p) {
+    using T = typename S::T;
+    const StepGeom* sg = (const StepGeom*)r.rec;
+    const DevBody* bn = (const DevBody*)(r.rec + sizeof(StepGeom));
+    const DevBody* bo = bn + r.nb;
+    hipLaunchKernelGGL((k_fixup<S, COLL, FORCED, MODE>), dim3((unsigned)r.grid_blocks), dim3(kBlock), 0, r.stream,
+                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, sg, bn, bo, r.flags, r.list,
+                       r.list_count, r.partials, r.write_flags);
+}
+
+template <class S, int MODE>
+static void launch_mode(const LaunchReq& r, const Phys<typename S::C>& p) {
+    const bool trt = r.collision == LBM_TRT;
+    if (r.kind == K_BULK) {
+        if (trt) r.forced ? launch_bulk_t<S, MODE, LBM_TRT, true>(r, p) : launch_bulk_t<S, MODE, LBM_TRT, false>(r, p);
+        else r.forced ? launch_bulk_t<S, MODE, LBM_BGK, true>(r, p) : launch_bulk_t<S, MODE, LBM_BGK, false>(r, p);
+    } else if (MODE == MODE_STEP || MODE == MODE_GET) {
+        if (r.grid_blocks <= 0) return;
+        if (trt) r.forced ? launch_fixup_t<S, MODE, LBM_TRT, true>(r, p) : launch_fixup_t<S, MODE, LBM_TRT, false>(r, p);
+        else r.forced ? launch_fixup_t<S, MODE, LBM_BGK, true>(r, p) : launch_fixup_t<S, MODE, LBM_BGK, false>(r, p);
+    }
+}
+
+template <class S>
+int launch(const LaunchReq& r) {
+    using C = typename S::C;
+    using T = typename S::T;
+    Phys<C> p;
+    p.wp = (C)r.p.wp;
+    p.wm = (C)r.p.wm;
+    for (int k = 0; k < 3; ++k) p.g[k] = (C)r.p.g[k];
+    const dim3 grid((unsigned)nblocks_of(r.g.N)), block(kBlock);
+    switch (r.kind) {
+        case K_BULK:
+        case K_FIXUP:
+            switch (r.mode) {
+                case MODE_STEP: launch_mode<S, MODE_STEP>(r, p); break;
+                case MODE_GET: launch_mode<S, MODE_GET>(r, p); break;
+                case MODE_COLLIDE_ONLY: launch_mode<S, MODE_COLLIDE_ONLY>(r, p); break;
+                case MODE_GET_RAW: launch_mode<S, MODE_GET_RAW>(r, p); break;
+                default: return (int)hipErrorInvalidValue;
+            }
+            break;
+        case K_EQUILIBRIUM:
+            hipLaunchKernelGGL(k_equilibrium<S>, grid, block, 0, r.stream, (T*)r.fout, r.src_rho, r.src_u, r.rho0,
+                               r.u0[0], r.u0[1], r.u0[2], r.g.N);
+            break;
+        case K_FROM_DOUBLE:
+            hipLaunchKernelGGL(k_from_double<S>, grid, block, 0, r.stream, (T*)r.fout, r.src_f, r.g.N);
+            break;
+        default: return (int)hipErrorInvalidValue;
+    }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : (int)e;
+}
+
+}  // namespace lbm

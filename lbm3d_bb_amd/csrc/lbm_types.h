@@ -1,0 +1,123 @@
+// lbm_types.h -- data shared by the host code and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lbm_hip.h"
+
+namespace lbm {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxBoxes = 6 + LBM_MAX_BODIES;
+
+// Body as the kernels see it: FP32 mask pose (inside test) and FP64 half-step
+// kinematics (moving-wall velocity and torque arm).
+struct DevBody {
+    int shape;
+    float c[3];
+    float r2;
+    float ia2[3];
+    float rot[9];
+    float pad;
+    double ch[3];
+    double v[3];
+    double om[3];
+};
+
+// A box of cells: lo may lie outside [0, L) on a periodic axis (wrapped).
+// kind 0 = face slab, 1 = body region.
+struct Box {
+    int lo[3];
+    int ext[3];
+    int kind;
+    int first_block;
+};
+
+struct Grid {
+    int nx, ny, nz;
+    int per;  // bit a: axis a periodic
+    int64_t N;
+    int64_t nxy;
+    int face_kind[6];
+    double face_u[6][3];
+    int wall_point;
+    int refill;
+    double rho0;
+};
+
+template <typename C>
+struct Phys {
+    C wp, wm;
+    C g[3];
+};
+
+// Geometry of one step for the fix-up pass.
+struct StepGeom {
+    int nbox;
+    int nblocks;     // blocks of all boxes
+    int nbodies;
+    int mode;        // 0: solidity from flags (static), 1: analytic (moving bodies)
+    int list_blocks; // blocks over the static boundary-node list (mode 0)
+    int pad[3];
+    Box box[kMaxBoxes];
+};
+
+enum { MODE_STEP = 0, MODE_GET = 1, MODE_COLLIDE_ONLY = 2, MODE_GET_RAW = 3 };
+
+// One kernel launch request; the per-storage translation units implement
+// launch<S>() for it, so the host code never instantiates kernels itself.
+enum { K_BULK = 0, K_FIXUP = 1, K_EQUILIBRIUM = 2, K_FROM_DOUBLE = 3 };
+
+struct LaunchReq {
+    int kind;
+    int mode;
+    int collision;
+    int forced;
+    Grid g;
+    Phys<double> p;
+    hipStream_t stream;
+    const void* fin;
+    void* fout;
+    double* gout;
+    double* rho_out;
+    double* u_out;
+    // fix-up
+    int grid_blocks;
+    const char* rec;
+    int nb;
+    uint8_t* flags;
+    const int32_t* list;
+    const int32_t* list_count;
+    double* partials;
+    int write_flags;
+    // start state
+    const double* src_rho;
+    const double* src_u;
+    const double* src_f;
+    double rho0;
+    double u0[3];
+};
+
+struct StoreF64;
+struct StoreF32;
+struct StoreF16S;
+struct StoreBF16S;
+
+template <class S>
+int launch(const LaunchReq& r);
+
+// storage-independent kernels (lbm_geom.hip)
+int launch_flags_full(uint8_t* flags, const DevBody* bodies, int nb, const Grid& g, hipStream_t st);
+int launch_bb_count(const uint8_t* flags, const Grid& g, int32_t* node_counts, int32_t* link_counts, hipStream_t st);
+int launch_scan2(int32_t* a, int32_t* b, int n, hipStream_t st);
+int launch_bb_fill(const uint8_t* flags, const Grid& g, const int32_t* node_off, const int32_t* link_off,
+                   int32_t* nodes, int32_t* link_cell, int8_t* link_dir, hipStream_t st);
+int launch_force_finalize(const double* partials, int nblocks, int n6, double* out, hipStream_t st);
+int launch_conservation(const double* rho, const double* u, const uint8_t* flags, int64_t N, double* partials,
+                        hipStream_t st);
+
+inline int64_t nblocks_of(int64_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace lbm

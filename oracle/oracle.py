@@ -1,0 +1,96 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle (liblbm_oracle.so).
+
+``OracleSolver(config, precision)`` exposes the same interface as
+``lbm3d_bb_amd.solver.Solver`` so a test runs one script on both.  Imported by
+tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg only.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+from lbm3d_bb_amd import _ffi
+from lbm3d_bb_amd.solver import Config, Solver
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liblbm_oracle.so")
+
+_P = ct.c_void_p
+_D = ct.POINTER(ct.c_double)
+_I64 = ct.POINTER(ct.c_int64)
+
+_SIGS = {
+    "destroy": [_P],
+    "set_bodies": [_P, ct.c_int32, ct.POINTER(_ffi.LbmBody)],
+    "set_motion": [_P, ct.c_int32, ct.POINTER(_ffi.LbmMotion)],
+    "set_flags": [_P, ct.POINTER(ct.c_uint8)],
+    "update_geometry": [_P],
+    "get_flags": [_P, ct.POINTER(ct.c_uint8)],
+    "get_links": [_P, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int8), ct.c_int64, _I64],
+    "init_equilibrium": [_P, _D, _D],
+    "step": [_P, ct.c_int32, _D],
+    "get_macros": [_P, _D, _D],
+    "get_populations": [_P, _D],
+    "set_populations": [_P, _D],
+    "conservation": [_P, _D, _D],
+}
+
+_LIB = None
+
+
+def build_oracle() -> str:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    return ORACLE_LIB
+
+
+def load_oracle() -> ct.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(ORACLE_LIB):
+            build_oracle()
+        lib = ct.CDLL(ORACLE_LIB)
+        lib.orc_create.argtypes = [ct.POINTER(_ffi.LbmConfig), ct.c_int, ct.POINTER(_P)]
+        lib.orc_create.restype = ct.c_int
+        for name, args in _SIGS.items():
+            fn = getattr(lib, "orc_" + name)
+            fn.argtypes = args
+            fn.restype = ct.c_int
+        lib.orc_body_at.argtypes = [ct.POINTER(_ffi.LbmMotion), ct.c_int64, ct.POINTER(_ffi.LbmBody)]
+        lib.orc_body_at.restype = ct.c_int
+        lib.orc_inside_test.argtypes = [ct.POINTER(_ffi.LbmBody), ct.c_float, ct.c_float, ct.c_float,
+                                        ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32)]
+        lib.orc_inside_test.restype = ct.c_int
+        lib.orc_num_threads.restype = ct.c_int
+        _LIB = lib
+    return _LIB
+
+
+class _OracleBackend:
+    prefix = "orc_"
+
+    def __init__(self, precision: str):
+        self.lib = load_oracle()
+        self.prec = 0 if precision == "fp64" else 1
+
+    def create(self, cfg, out) -> int:
+        return self.lib.orc_create(ct.byref(cfg), self.prec, ct.byref(out))
+
+    def fn(self, name: str):
+        return getattr(self.lib, self.prefix + name)
+
+
+class OracleSolver(Solver):
+    """The oracle behind the Solver interface; precision "fp64" or "fp32"."""
+
+    def __init__(self, config: Config, precision: str = "fp64"):
+        super().__init__(config, backend=_OracleBackend(precision))
+
+    def info(self) -> dict:  # not part of the oracle API
+        return {"ncells": self.N}
+
+
+def body_at(law, n: int) -> _ffi.LbmBody:
+    out = _ffi.LbmBody()
+    load_oracle().orc_body_at(ct.byref(law.to_c()), n, ct.byref(out))
+    return out

@@ -1,0 +1,141 @@
+/*
+ * oracle_geom.h -- TEST INFRASTRUCTURE ONLY (the oracle).  Shared,
+ * precision-independent parts of the CPU restatement: the D3Q19 tables, the FP32
+ * inside tests, the motion law and the brute-force boundary-link build.
+ *
+ * Nothing in the shipped library includes or links this file; tests and the
+ * benchmark's CPU-baseline leg use it as the checker.
+ */
+#ifndef ORACLE_GEOM_H
+#define ORACLE_GEOM_H
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/lbm_hip.h"
+
+/* D3Q19 in the reference order (src/Utilities.jl:950-952), 0-based. */
+static const int OCX[19] = {1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0};
+static const int OCY[19] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 0};
+static const int OCZ[19] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 0};
+/* bbOpp (src/Utilities.jl:960) minus one. */
+static const int OOPP[19] = {1, 0, 3, 2, 5, 4, 7, 6, 9, 8, 11, 10, 13, 12, 15, 14, 17, 16, 18};
+/* weight class: 0 -> 1/18, 1 -> 1/36, 2 -> 1/3 (src/Utilities.jl:954-958). */
+static const int OWCLS[19] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2};
+
+/* Per-body data prepared on the host once per pose (FP32 mask, FP64 kinematics). */
+typedef struct {
+    int shape;
+    float c[3];
+    float r2;          /* sphere R*R */
+    float ia2[3];      /* ellipsoid 1/(a*a), 1/(b*b), 1/(c*c) */
+    float rot[9];
+    float reach;       /* bounding half-extent */
+    double ch[3], v[3], om[3];
+} orc_body;
+
+static void orc_prepare_body(const lbm_body* b, orc_body* o) {
+    memset(o, 0, sizeof(*o));
+    o->shape = b->shape;
+    for (int k = 0; k < 3; ++k) {
+        o->c[k] = b->center[k];
+        o->ch[k] = b->center_half[k];
+        o->v[k] = b->vel[k];
+        o->om[k] = b->omega[k];
+    }
+    for (int k = 0; k < 9; ++k) o->rot[k] = b->rot[k];
+    if (b->shape == LBM_SPHERE) {
+        float R = b->semi[0];
+        o->r2 = R * R;
+        o->reach = R;
+    } else {
+        float m = 0.0f;
+        for (int k = 0; k < 3; ++k) {
+            float a = b->semi[k];
+            o->ia2[k] = 1.0f / (a * a);
+            if (a > m) m = a;
+        }
+        o->reach = m;
+    }
+}
+
+/* Minimum-image displacement along a periodic axis of length L (FP32). */
+static float orc_min_image(float d, int periodic, int L) {
+    if (periodic) {
+        float h = 0.5f * (float)L;
+        if (d > h) d -= (float)L;
+        else if (d < -h) d += (float)L;
+    }
+    return d;
+}
+
+/* Inside test in FP32 with the reference's tolerance R^2 + 1e-6 R^2
+ * (src/Geometry.jl:73), evaluated left to right with no contraction. */
+static int orc_inside(const orc_body* b, float px, float py, float pz, const int per[3],
+                      const int L[3]) {
+    float dx = orc_min_image(px - b->c[0], per[0], L[0]);
+    float dy = orc_min_image(py - b->c[1], per[1], L[1]);
+    float dz = orc_min_image(pz - b->c[2], per[2], L[2]);
+    if (b->shape == LBM_SPHERE) {
+        float s = dx * dx + dy * dy;
+        s = s + dz * dz;
+        return s <= b->r2 + 1e-6f * b->r2;
+    } else {
+        float xb = b->rot[0] * dx + b->rot[3] * dy;
+        xb = xb + b->rot[6] * dz;
+        float yb = b->rot[1] * dx + b->rot[4] * dy;
+        yb = yb + b->rot[7] * dz;
+        float zb = b->rot[2] * dx + b->rot[5] * dy;
+        zb = zb + b->rot[8] * dz;
+        float q = xb * xb * b->ia2[0] + yb * yb * b->ia2[1];
+        q = q + zb * zb * b->ia2[2];
+        return q <= 1.0f + 1e-6f;
+    }
+}
+
+/* Rodrigues rotation about unit axis k by angle th, composed as Rot * R0. */
+static void orc_rotation_at(const double om[3], double t, const double R0[9], double out[9]) {
+    double w = sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+    double Rr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (w > 0.0) {
+        double k[3] = {om[0] / w, om[1] / w, om[2] / w};
+        double th = w * t, s = sin(th), c1 = 1.0 - cos(th);
+        double K[9] = {0, -k[2], k[1], k[2], 0, -k[0], -k[1], k[0], 0};
+        double K2[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double a = 0.0;
+                for (int m = 0; m < 3; ++m) a += K[i * 3 + m] * K[m * 3 + j];
+                K2[i * 3 + j] = a;
+            }
+        for (int i = 0; i < 9; ++i) Rr[i] += s * K[i] + c1 * K2[i];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double a = 0.0;
+            for (int m = 0; m < 3; ++m) a += Rr[i * 3 + m] * R0[m * 3 + j];
+            out[i * 3 + j] = a;
+        }
+}
+
+/* motion_at for step n (1-based): mask pose at t = n-1, kinematics at t = n-1/2
+ * (src/Motion.jl:26-36 with dt = 1). */
+static void orc_body_from_law(const lbm_motion* m, int64_t n, lbm_body* b) {
+    memset(b, 0, sizeof(*b));
+    b->shape = m->shape;
+    double t0 = (double)(n - 1), th = (double)(n - 1) + 0.5;
+    double R[9];
+    orc_rotation_at(m->omega, t0, m->rot0, R);
+    for (int k = 0; k < 3; ++k) {
+        b->center[k] = (float)(m->center0[k] + m->vel[k] * t0);
+        b->semi[k] = (float)m->semi[k];
+        b->center_half[k] = m->center0[k] + m->vel[k] * th;
+        b->vel[k] = m->vel[k];
+        b->omega[k] = m->omega[k];
+    }
+    for (int k = 0; k < 9; ++k) b->rot[k] = (float)R[k];
+}
+
+#endif /* ORACLE_GEOM_H */

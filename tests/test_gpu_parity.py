@@ -1,0 +1,222 @@
+"""HIP path vs the oracle on the same inputs (GPU).
+
+Bars: solid flags and boundary links bit-exact; populations, moments and
+per-body force/torque within a normwise relative tolerance of 1e-12 (FP64
+storage vs the FP64 oracle) or 1e-5 (FP32 storage vs the FP32 oracle);
+normwise = max|gpu - oracle| / max|oracle| over the whole array.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from lbm3d_bb_amd.bodies import ConstantMotion, ellipsoid, rotation_matrix, sphere
+from lbm3d_bb_amd.solver import Solver
+from oracle.oracle import OracleSolver
+
+from cases import channel, periodic_box, reference_channel, rel_err, sphere_pack, trt_omegas
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+def _pair(cfg, prec):
+    return Solver(cfg), OracleSolver(cfg, prec)
+
+
+def _compare_state(g, o, tol, what=""):
+    rg, ug = g.macros()
+    ro, uo = o.macros()
+    assert rel_err(rg, ro) < tol, f"{what} rho {rel_err(rg, ro)}"
+    assert rel_err(ug, uo) < tol * 1e2 or np.abs(ug - uo).max() < tol, f"{what} u {rel_err(ug, uo)}"
+    fg, fo = g.populations(), o.populations()
+    assert rel_err(fg, fo) < tol, f"{what} f {rel_err(fg, fo)}"
+
+
+def test_flags_and_links_bit_exact_sphere():
+    cfg = periodic_box(n=32)
+    g, o = _pair(cfg, "fp64")
+    b = [sphere((15.7, 16.2, 16.9), 6.3), sphere((3.0, 3.5, 30.5), 4.0)]  # second one wraps
+    for s in (g, o):
+        s.set_bodies(b)
+        s.update_geometry()
+    assert np.array_equal(g.flags(), o.flags())
+    cg, dg = g.links()
+    co, do = o.links()
+    assert len(co) > 0
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+
+
+def test_flags_and_links_bit_exact_ellipsoid_channel():
+    cfg = channel(nx=48, ny=30, nz=28, storage="fp32")
+    g, o = _pair(cfg, "fp32")
+    R = rotation_matrix((0.3, -0.5, 0.8), 0.7)
+    b = [ellipsoid((20.3, 14.9, 13.2), (9.0, 4.5, 4.0), rot=R)]
+    for s in (g, o):
+        s.set_bodies(b)
+        s.update_geometry()
+    assert np.array_equal(g.flags(), o.flags())
+    cg, dg = g.links()
+    co, do = o.links()
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+
+
+def test_initial_state_matches():
+    cfg = periodic_box(n=16)
+    g, o = _pair(cfg, "fp64")
+    rng = np.random.default_rng(3)
+    rho = 1.0 + 0.01 * rng.standard_normal(cfg.ncells)
+    u = 0.02 * rng.standard_normal((3, cfg.ncells))
+    for s in (g, o):
+        s.init_equilibrium(rho, u)
+    _compare_state(g, o, 1e-14, "init")
+
+
+@pytest.mark.parametrize("collision", ["BGK", "TRT"])
+def test_periodic_sphere_fp64(collision):
+    """Config-1 shape at 24^3: fixed sphere, body force, halfway bounce-back, FP64."""
+    cfg = periodic_box(n=24, collision=collision)
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies([sphere((12.0, 12.0, 12.0), 4.0)])
+        s.init_equilibrium()
+    Fg = g.step(60)
+    Fo = o.step(60)
+    assert np.abs(Fo).max() > 0
+    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    _compare_state(g, o, TOL["fp64"], collision)
+
+
+def test_multi_body_static_fp64():
+    cfg = periodic_box(n=28, collision="TRT", g=(2e-5, 1e-5, 0.0))
+    bodies = sphere_pack(4, 28, 2.5, 4.0, seed=0)
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies(bodies)
+        s.init_equilibrium()
+    Fg = g.step(40)
+    Fo = o.step(40)
+    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    _compare_state(g, o, TOL["fp64"], "multi")
+
+
+def test_channel_translating_sphere_fp64():
+    """Config-2 shape: inlet, zero-gradient outlet, no-slip walls, moving sphere, refill."""
+    cfg = channel(nx=40, ny=20, nz=20, storage="fp64")
+    g, o = _pair(cfg, "fp64")
+    law = ConstantMotion("sphere", (12.3, 10.1, 9.8), (4.0,), vel=(0.05, 0.0, 0.0))
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg = g.step(50)
+    Fo = o.step(50)
+    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    _compare_state(g, o, TOL["fp64"], "channel")
+    assert np.array_equal(g.flags(), o.flags())
+
+
+def test_channel_translating_sphere_fp32():
+    cfg = channel(nx=40, ny=20, nz=20, storage="fp32")
+    g, o = _pair(cfg, "fp32")
+    law = ConstantMotion("sphere", (12.3, 10.1, 9.8), (4.0,), vel=(0.01, 0.0, 0.0))
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg = g.step(30)
+    Fo = o.step(30)
+    assert rel_err(Fg, Fo) < TOL["fp32"] * 10, rel_err(Fg, Fo)
+    rg, ug = g.macros()
+    ro, uo = o.macros()
+    assert rel_err(rg, ro) < TOL["fp32"]
+    assert rel_err(ug, uo) < TOL["fp32"] * 10
+
+
+def test_reference_face_set_fp64():
+    """The reference's faces: velocity inlet and y-walls at U_inf, ABB outlet, z periodic,
+    with the reference's wall-velocity sampling point and a sphere moving in y."""
+    cfg = reference_channel(nx=40, ny=24, nz=8, storage="fp64")
+    g, o = _pair(cfg, "fp64")
+    law = ConstantMotion("sphere", (14.0, 12.0, 4.0), (3.5,), vel=(0.0, 0.02, 0.0), omega=(0.0, 0.0, 0.01))
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg = g.step(40)
+    Fo = o.step(40)
+    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    _compare_state(g, o, TOL["fp64"], "refchannel")
+
+
+def test_rotating_ellipsoid_refill_fp64():
+    cfg = channel(nx=36, ny=28, nz=28, storage="fp64", u_in=0.02)
+    g, o = _pair(cfg, "fp64")
+    law = ConstantMotion("ellipsoid", (17.0, 14.0, 14.0), (8.0, 4.0, 4.0), omega=(0.0, 0.02, 0.03),
+                         rot0=rotation_matrix((1, 1, 0), 0.4))
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg = g.step(30)
+    Fo = o.step(30)
+    assert np.abs(Fo[:, :, 3:]).max() > 0
+    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    _compare_state(g, o, TOL["fp64"], "ellipsoid")
+    assert np.array_equal(g.flags(), o.flags())
+
+
+@pytest.mark.parametrize("storage,tol", [("fp16s", 5e-3), ("bf16s", 5e-2)])
+def test_shifted_storage_drag_vs_fp64(storage, tol):
+    cfg64 = periodic_box(n=24, collision="TRT", storage="fp64", g=(1e-5, 0, 0))
+    cfgs = periodic_box(n=24, collision="TRT", storage=storage, g=(1e-5, 0, 0))
+    o = OracleSolver(cfg64, "fp64")
+    g = Solver(cfgs)
+    for s in (g, o):
+        s.set_bodies([sphere((12.0, 12.0, 12.0), 4.0)])
+        s.init_equilibrium()
+    Fg = g.step(200)
+    Fo = o.step(200)
+    err = abs(Fg[-1, 0, 0] - Fo[-1, 0, 0]) / abs(Fo[-1, 0, 0])
+    assert err < tol, err
+
+
+def test_set_populations_roundtrip_and_step():
+    cfg = periodic_box(n=16, collision="TRT")
+    g, o = _pair(cfg, "fp64")
+    rng = np.random.default_rng(7)
+    from lbm3d_bb_amd.lattice import equilibrium
+
+    rho = 1.0 + 0.01 * rng.standard_normal(cfg.ncells)
+    u = 0.03 * rng.standard_normal((3, cfg.ncells))
+    f = equilibrium(rho, *u) + 1e-4 * rng.standard_normal((19, cfg.ncells))
+    for s in (g, o):
+        s.set_bodies([sphere((8.0, 8.0, 8.0), 3.0)])
+        s.set_populations(f)
+    assert rel_err(g.populations(), f) < 1e-15
+    g.step(10)
+    o.step(10)
+    _compare_state(g, o, TOL["fp64"], "setpop")
+
+
+def test_forces_deterministic():
+    cfg = periodic_box(n=28, collision="TRT")
+    bodies = sphere_pack(5, 28, 2.5, 3.5, seed=1)
+    runs = []
+    for _ in range(2):
+        g = Solver(cfg)
+        g.set_bodies(bodies)
+        g.init_equilibrium()
+        runs.append(g.step(20))
+        g.close()
+    assert np.array_equal(runs[0], runs[1])
+
+
+def test_mass_conserved_periodic_no_body():
+    cfg = periodic_box(n=20, collision="TRT", storage="fp64", g=(1e-4, 0, 0))
+    g = Solver(cfg)
+    rng = np.random.default_rng(0)
+    g.init_equilibrium(1.0 + 0.01 * rng.standard_normal(cfg.ncells), 0.01 * rng.standard_normal((3, cfg.ncells)))
+    m0, p0 = g.conservation()
+    g.step(100)
+    m1, p1 = g.conservation()
+    assert abs(m1 - m0) / m0 < 1e-13
+    # momentum grows by exactly sum(F) = M g per step
+    assert abs((p1[0] - p0[0]) - 100 * m0 * 1e-4) < 1e-10 * m0

@@ -1,0 +1,58 @@
+"""params.txt reader and LBMParams derivation (src/Parameters.jl:49-164)."""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from lbm3d_bb_amd.params import read_params_input, trt_rates
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_read_case_file():
+    p = read_params_input(os.path.join(HERE, "golden", "params_sphere_case.txt"))
+    assert (p.Nx, p.Ny, p.Nz, p.q, p.n_max, p.n_save) == (256, 128, 128, 19, 20000, 1000)
+    assert p.bc == "SBB" and p.coll == "TRT" and p.shape == "CYL" and p.naca_code == "2412"
+    assert p.chord == 60.0 and p.R == 12.0
+    # The reference takes the first number on each line (src/Parameters.jl:124), which on
+    # "Theta0: | 15" is the 0 of the key itself: theta0 reads as 0 whatever the value.
+    assert p.theta0 == 0.0
+    # nu = U (2R) / Re = 0.05 * 24 / 60 = 0.02; tau+ = 0.56; tau- = 0.5 + 0.1875/0.06
+    assert abs(p.nu - 0.02) < 1e-8
+    assert abs(p.omega_p - 1 / 0.56) < 1e-6
+    assert abs(p.omega_m - 1 / (0.5 + 0.1875 / 0.06)) < 1e-6
+
+
+def test_trt_magic_product():
+    # Lambda = (1/omega+ - 1/2)(1/omega- - 1/2)
+    for nu in (0.01, 0.05, 0.2):
+        wp, wm = trt_rates(nu, 0.1875, dtype=np.float64)
+        assert abs((1 / wp - 0.5) * (1 / wm - 0.5) - 0.1875) < 1e-12
+
+
+def test_parse_errors(tmp_path):
+    f = tmp_path / "bad.txt"
+    f.write_text("Nx: | 10\nBC: | SBB\n")
+    with pytest.raises(ValueError, match="Expected 19"):
+        read_params_input(str(f))
+    lines = [f"v{i}: | {i + 1}" for i in range(19)]
+    f.write_text("\n".join(lines) + "\n")
+    with pytest.raises(ValueError, match="Missing BC"):
+        read_params_input(str(f))
+    f.write_text("\n".join(lines) + "\nBC: | SBB\nShape: | BOX\n")
+    with pytest.raises(ValueError, match="CYL or AIR"):
+        read_params_input(str(f))
+
+
+def test_airfoil_defaults_naca():
+    import tempfile
+
+    lines = [f"v{i}: | {i + 1}" for i in range(19)] + ["BC: | IBB", "Shape: | AIR"]
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as fh:
+        fh.write("\n".join(lines) + "\n")
+    p = read_params_input(fh.name)
+    os.unlink(fh.name)
+    assert p.naca_code == "0012" and p.coll == "BGK" and p.shape == "AIR"
