@@ -90,6 +90,7 @@ def main() -> int:
     ap.add_argument("--workload", default="channel256", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-launch HIP events (no roofline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): this build has no multi-GPU path; "
@@ -106,7 +107,7 @@ def main() -> int:
     if args.warmup:
         s.step(args.warmup)
     s.sync()
-    s.profile(True)
+    s.profile(not args.no_profile)
     t0 = time.perf_counter()
     forces = s.step(args.steps)
     s.sync()
@@ -123,7 +124,7 @@ def main() -> int:
     bpc = bytes_per_cell(w["storage"])
     bulk_avg_s = prof["bulk_ms"] / max(prof["bulk_launches"], 1) / 1e3
     fix_avg_s = prof["link_ms"] / max(prof["link_launches"], 1) / 1e3
-    achieved = bpc * interior / bulk_avg_s / 1e9
+    achieved = bpc * interior / bulk_avg_s / 1e9 if bulk_avg_s > 0 else None
     traffic = None
     tfile = os.path.join(REPO, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tfile):
@@ -158,10 +159,10 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": traffic,
         },
     }

@@ -58,6 +58,8 @@ struct lbm_ctx {
     std::vector<DevBody> mask_bodies;  // pose that flags reflect
     std::vector<DevBody> step_bodies;  // pose + kinematics of the current step
     std::vector<float> reach;          // bounding half-extent per body
+    bool have_flags_box = false;       // lbm_set_flags: bounding box of the solid cells
+    float flags_lo[3] = {0, 0, 0}, flags_hi[3] = {0, 0, 0};
     bool geom_valid = false;
 
     // per-step records: [kChunk] x (StepGeom | DevBody new[nb] | DevBody old[nb])
@@ -77,6 +79,7 @@ struct lbm_ctx {
     // last step (for getters)
     const char* last_rec = nullptr;
     int last_grid = 0;
+    int last_face_grid = 0;
     bool last_rec_valid = false;
 
     int64_t step = 0;
@@ -181,6 +184,7 @@ DevBody to_dev(const lbm_body& b) {
         d.om[k] = b.omega[k];
     }
     for (int k = 0; k < 9; ++k) d.rot[k] = b.rot[k];
+    d.reach = b.shape == LBM_SPHERE ? b.semi[0] : std::max(b.semi[0], std::max(b.semi[1], b.semi[2]));
     if (b.shape == LBM_SPHERE) {
         const float R = b.semi[0];
         d.r2 = R * R;
@@ -205,32 +209,6 @@ bool same_mask_pose(const DevBody& a, const DevBody& b) {
 }
 
 
-// Face slabs of the non-periodic axes, disjoint by construction: axis a takes
-// the full extent of later axes and the interior of earlier non-periodic ones.
-void add_face_slabs(const lbm_ctx* c, StepGeom& sg) {
-    const int L[3] = {c->g.nx, c->g.ny, c->g.nz};
-    for (int a = 0; a < 3; ++a) {
-        if ((c->g.per >> a) & 1) continue;
-        for (int side = 0; side < 2; ++side) {
-            Box b{};
-            for (int o = 0; o < 3; ++o) {
-                if (o == a) {
-                    b.lo[o] = side ? L[o] - 1 : 0;
-                    b.ext[o] = 1;
-                } else if (o < a && !((c->g.per >> o) & 1)) {
-                    b.lo[o] = 1;
-                    b.ext[o] = L[o] - 2;
-                } else {
-                    b.lo[o] = 0;
-                    b.ext[o] = L[o];
-                }
-            }
-            b.kind = 0;
-            sg.box[sg.nbox++] = b;
-        }
-    }
-}
-
 void index_box(const lbm_ctx* c, float lo_f, float hi_f, int axis, int& lo, int& ext) {
     const int L = axis == 0 ? c->g.nx : (axis == 1 ? c->g.ny : c->g.nz);
     int l = (int)std::floor(lo_f) - 2, h = (int)std::ceil(hi_f) + 1;
@@ -252,14 +230,47 @@ void index_box(const lbm_ctx* c, float lo_f, float hi_f, int axis, int& lo, int&
     ext = std::max(h - l + 1, 0);
 }
 
+// Domain edges: the lines where two non-periodic faces meet (corner cells are
+// shared by several lines; the first one owns them).
+void add_edge_boxes(const lbm_ctx* c, StepGeom& sg) {
+    const int L[3] = {c->g.nx, c->g.ny, c->g.nz};
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b) {
+            if (((c->g.per >> a) & 1) || ((c->g.per >> b) & 1)) continue;
+            const int o = 3 - a - b;
+            for (int sa = 0; sa < 2; ++sa)
+                for (int sb = 0; sb < 2; ++sb) {
+                    Box e{};
+                    e.lo[a] = sa ? L[a] - 1 : 0;
+                    e.ext[a] = 1;
+                    e.lo[b] = sb ? L[b] - 1 : 0;
+                    e.ext[b] = 1;
+                    e.lo[o] = 0;
+                    e.ext[o] = L[o];
+                    e.kind = 2;
+                    e.face = -1;
+                    sg.box[sg.nbox++] = e;
+                }
+        }
+}
+
+// Block numbering: face slabs (k_faces) and edge/body boxes (k_fixup) each from 0.
 void finish_boxes(StepGeom& sg) {
-    int blocks = 0;
+    int fblocks = 0, xblocks = 0;
+    sg.first_fix_box = sg.nbox;
     for (int i = 0; i < sg.nbox; ++i) {
-        sg.box[i].first_block = blocks;
         const int64_t vol = (int64_t)sg.box[i].ext[0] * sg.box[i].ext[1] * sg.box[i].ext[2];
-        blocks += (int)nblocks_of(vol);
+        if (sg.box[i].kind == 0) {
+            sg.box[i].first_block = fblocks;
+            fblocks += (int)nblocks_of(vol);
+        } else {
+            if (sg.first_fix_box == sg.nbox) sg.first_fix_box = i;
+            sg.box[i].first_block = xblocks;
+            xblocks += (int)nblocks_of(vol);
+        }
     }
-    sg.nblocks = blocks;
+    sg.nface_blocks = fblocks;
+    sg.nfix_blocks = xblocks;
 }
 
 LaunchReq base_req(const lbm_ctx* c, int kind, int mode) {
@@ -293,8 +304,10 @@ int do_launch(lbm_ctx* c, const LaunchReq& r) {
     return LBM_OK;
 }
 
-int launch_bulk(lbm_ctx* c, int mode, const void* fin, void* fout, double* gout, double* rho, double* u) {
+int launch_bulk(lbm_ctx* c, int mode, const void* fin, void* fout, double* gout, double* rho, double* u,
+                const char* rec) {
     LaunchReq r = base_req(c, K_BULK, mode);
+    r.rec = rec;
     r.fin = fin;
     r.fout = fout;
     r.gout = gout;
@@ -304,7 +317,7 @@ int launch_bulk(lbm_ctx* c, int mode, const void* fin, void* fout, double* gout,
 }
 
 int launch_fixup(lbm_ctx* c, int mode, int grid_blocks, const void* fin, void* fout, double* gout, double* rho,
-                 double* u, const char* rec, int write_flags) {
+                 double* u, const char* rec, int write_flags, double* partials) {
     if (grid_blocks <= 0) return LBM_OK;
     LaunchReq r = base_req(c, K_FIXUP, mode);
     r.fin = fin;
@@ -315,6 +328,7 @@ int launch_fixup(lbm_ctx* c, int mode, int grid_blocks, const void* fin, void* f
     r.grid_blocks = grid_blocks;
     r.rec = rec;
     r.write_flags = write_flags;
+    r.partials = partials;
     return do_launch(c, r);
 }
 
@@ -355,7 +369,7 @@ int ensure_records(lbm_ctx* c) {
 }
 
 int ensure_partials(lbm_ctx* c, int64_t blocks) {
-    const int64_t need = blocks * std::max(c->nb, 1) * 6;
+    const int64_t need = blocks * std::max(c->nb, 1) * 6;  // blocks = slots over all steps of a chunk
     if (need <= c->partials_cap) return LBM_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dev_free(c, c->d_partials, (size_t)c->partials_cap * sizeof(double));
@@ -444,10 +458,18 @@ constexpr int kStopChunk = 1;
 // flags, or the static list after a motion phase) returns kStopChunk unless
 // allow_sync, so that the rebuild sees the flags of every earlier step.
 // Returns the fix-up grid size via *grid.
-int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, bool allow_sync) {
+void add_body_box(const lbm_ctx* c, StepGeom& sg, const float lo_f[3], const float hi_f[3]) {
+    Box b{};
+    for (int a = 0; a < 3; ++a) index_box(c, lo_f[a], hi_f[a], a, b.lo[a], b.ext[a]);
+    b.kind = 1;
+    b.face = -1;
+    if ((int64_t)b.ext[0] * b.ext[1] * b.ext[2] > 0) sg.box[sg.nbox++] = b;
+}
+
+int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bool allow_sync) {
     StepGeom sg;
     std::memset(&sg, 0, sizeof(sg));
-    add_face_slabs(c, sg);
+    add_edge_boxes(c, sg);
     sg.nbodies = c->nb;
     std::vector<DevBody> bnew;
     if (!c->flags_only) bodies_for_step(c, n, bnew);
@@ -470,15 +492,13 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, bool allow_sync) {
         bold = c->mask_bodies;
         // body regions: union of the old and new bounding boxes, dilated
         for (int i = 0; i < c->nb; ++i) {
-            Box b{};
+            float lo_f[3], hi_f[3];
             const float rr = c->reach[i];
             for (int a = 0; a < 3; ++a) {
-                const float lo_f = std::min(bnew[i].c[a], bold[i].c[a]) - rr;
-                const float hi_f = std::max(bnew[i].c[a], bold[i].c[a]) + rr;
-                index_box(c, lo_f, hi_f, a, b.lo[a], b.ext[a]);
+                lo_f[a] = std::min(bnew[i].c[a], bold[i].c[a]) - rr;
+                hi_f[a] = std::max(bnew[i].c[a], bold[i].c[a]) + rr;
             }
-            b.kind = 1;
-            if ((int64_t)b.ext[0] * b.ext[1] * b.ext[2] > 0) sg.box[sg.nbox++] = b;
+            add_body_box(c, sg, lo_f, hi_f);
         }
         finish_boxes(sg);
         sg.list_blocks = 0;
@@ -491,6 +511,19 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, bool allow_sync) {
             if (rc) return rc;
             c->list_valid = true;
         }
+        // body regions (their face cells go to the general path)
+        if (c->flags_only) {
+            if (c->have_flags_box) add_body_box(c, sg, c->flags_lo, c->flags_hi);
+        } else {
+            for (int i = 0; i < c->nb; ++i) {
+                float lo_f[3], hi_f[3];
+                for (int a = 0; a < 3; ++a) {
+                    lo_f[a] = bnew[i].c[a] - c->reach[i];
+                    hi_f[a] = bnew[i].c[a] + c->reach[i];
+                }
+                add_body_box(c, sg, lo_f, hi_f);
+            }
+        }
         finish_boxes(sg);
         sg.list_blocks = c->nb > 0 ? (int)nblocks_of(c->list_n) : 0;
         bold = bnew;
@@ -502,7 +535,8 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, bool allow_sync) {
         bn[i] = bnew[i];
         bn[c->nb + i] = bold[i];
     }
-    *grid = sg.nblocks + sg.list_blocks;
+    *grid = sg.nfix_blocks + sg.list_blocks;
+    *face_grid = sg.nface_blocks;
     return LBM_OK;
 }
 
@@ -519,10 +553,11 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
     // the pinned buffers of this parity are free once their last copy finished
     const int par = c->parity;
     if (c->rec_ev_used[par]) HIPCHK(c, hipEventSynchronize(c->rec_ev[par]));
-    std::vector<int> grids(nsteps);
+    std::vector<int> grids(nsteps), fgrids(nsteps);
     int nprep = 0;
     for (int s = 0; s < nsteps; ++s) {
-        int rc = prepare_step(c, c->step + 1 + s, c->h_rec[par] + (size_t)s * c->rec_bytes, &grids[s], s == 0);
+        int rc = prepare_step(c, c->step + 1 + s, c->h_rec[par] + (size_t)s * c->rec_bytes, &grids[s], &fgrids[s],
+                              s == 0);
         if (rc == kStopChunk) break;
         if (rc) return rc;
         ++nprep;
@@ -530,7 +565,9 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
     HIPCHK(c, hipMemcpyAsync(c->d_rec, c->h_rec[par], (size_t)nprep * c->rec_bytes, hipMemcpyHostToDevice, c->stream));
     int maxgrid = 0;
     for (int s = 0; s < nprep; ++s) maxgrid = std::max(maxgrid, grids[s]);
-    int rc = ensure_partials(c, std::max(maxgrid, 1));
+    // block partials of every step of the chunk, summed once per chunk
+    const int64_t stride = (int64_t)std::max(maxgrid, 1) * std::max(c->nb, 1) * 6;
+    int rc = ensure_partials(c, (int64_t)std::max(maxgrid, 1) * nprep);
     if (rc) return rc;
     const int n6 = c->nb * 6;
     for (int s = 0; s < nprep; ++s) {
@@ -544,26 +581,23 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
             e2 = next_event(c);
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        int lr = launch_bulk(c, MODE_STEP, fin, fout, nullptr, nullptr, nullptr);
+        int lr = launch_bulk(c, MODE_STEP, fin, fout, nullptr, nullptr, nullptr, rec);
         if (lr) return lr;
         if (c->profiling) HIPCHK(c, hipEventRecord(e1, c->stream));
-        lr = launch_fixup(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, 1);
+        lr = launch_fixup(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, 1,
+                          c->d_partials + (size_t)s * stride);
         if (lr) return lr;
         if (c->profiling) HIPCHK(c, hipEventRecord(e2, c->stream));
-        if (n6 > 0) {
-            if (grids[s] > 0) {
-                GEOCHK(c, launch_force_finalize(c->d_partials, grids[s], n6, c->d_forces + (size_t)s * n6, c->stream));
-            } else {
-                HIPCHK(c, hipMemsetAsync(c->d_forces + (size_t)s * n6, 0, (size_t)n6 * sizeof(double), c->stream));
-            }
-        }
-        HIPCHK(c, hipGetLastError());
         c->cur = 1 - c->cur;
         c->last_rec = rec;
         c->last_grid = grids[s];
+        c->last_face_grid = fgrids[s];
         c->last_rec_valid = true;
         c->raw_state = false;
     }
+    if (n6 > 0 && nprep > 0)
+        GEOCHK(c, launch_force_finalize(c->d_partials, stride, c->d_rec, c->rec_bytes, nprep, n6, c->d_forces,
+                                        c->stream));
     if (force_out && n6 > 0) {
         HIPCHK(c, hipMemcpyAsync(c->h_forces[par], c->d_forces, (size_t)nprep * n6 * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
@@ -578,17 +612,18 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
 }
 
 int collide_only(lbm_ctx* c) {
-    return launch_bulk(c, MODE_COLLIDE_ONLY, c->fbuf[1 - c->cur], c->fbuf[c->cur], nullptr, nullptr, nullptr);
+    return launch_bulk(c, MODE_COLLIDE_ONLY, c->fbuf[1 - c->cur], c->fbuf[c->cur], nullptr, nullptr, nullptr, nullptr);
 }
 
 // Pulled state f_old and moments of the last completed step.
 int get_state(lbm_ctx* c, double* d_f, double* d_rho, double* d_u) {
     if (!c->initialised) return fail(c, LBM_ERR_STATE, "not initialised");
     const void* prev = c->fbuf[1 - c->cur];
-    if (c->raw_state) return launch_bulk(c, MODE_GET_RAW, prev, nullptr, d_f, d_rho, d_u);
-    int rc = launch_bulk(c, MODE_GET, prev, nullptr, d_f, d_rho, d_u);
-    if (rc == LBM_OK && c->last_rec_valid)
-        rc = launch_fixup(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0);
+    if (c->raw_state) return launch_bulk(c, MODE_GET_RAW, prev, nullptr, d_f, d_rho, d_u, nullptr);
+    if (!c->last_rec_valid) return fail(c, LBM_ERR_STATE, "no completed step to read");
+    int rc = launch_bulk(c, MODE_GET, prev, nullptr, d_f, d_rho, d_u, c->last_rec);
+    if (rc == LBM_OK)
+        rc = launch_fixup(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0, nullptr);
     return rc;
 }
 
@@ -640,7 +675,12 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         c->pf.g[k] = (float)cfg->g[k];
     }
     const int bytes = cfg->storage == LBM_FP64 ? 8 : (cfg->storage == LBM_FP32 ? 4 : 2);
-    c->fbytes = (size_t)19 * N * bytes;
+    // Plane stride: N rounded up to 256 B plus 2304 B, so consecutive direction
+    // planes do not start at the same offset modulo the HBM channel interleave
+    // (N * 4 B is exactly 2^24 B for a 256x128x128 lattice).
+    const int64_t align = 256 / bytes;
+    c->g.P = (N + align - 1) / align * align + 2304 / bytes;
+    c->fbytes = (size_t)19 * c->g.P * bytes;
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return LBM_ERR_HIP;
@@ -757,8 +797,22 @@ int lbm_set_motion(lbm_ctx* c, int32_t n, const lbm_motion* laws) {
 int lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
     if (!c || !flags) return LBM_ERR_ARG;
     int maxf = 0;
-    for (int64_t i = 0; i < c->g.N; ++i) maxf = std::max<int>(maxf, flags[i]);
+    int lo[3] = {c->g.nx, c->g.ny, c->g.nz}, hi[3] = {-1, -1, -1};
+    for (int64_t i = 0; i < c->g.N; ++i) {
+        if (!flags[i]) continue;
+        maxf = std::max<int>(maxf, flags[i]);
+        const int q[3] = {(int)(i % c->g.nx), (int)((i / c->g.nx) % c->g.ny), (int)(i / c->g.nxy)};
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], q[a]);
+            hi[a] = std::max(hi[a], q[a]);
+        }
+    }
     if (maxf > LBM_MAX_BODIES) return fail(c, LBM_ERR_ARG, "flag value above the body limit");
+    c->have_flags_box = maxf > 0;
+    for (int a = 0; a < 3; ++a) {
+        c->flags_lo[a] = (float)lo[a] + 0.5f;
+        c->flags_hi[a] = (float)hi[a] + 0.5f;
+    }
     int rc = set_body_count(c, maxf);
     if (rc) return rc;
     c->have_laws = false;
@@ -768,7 +822,9 @@ int lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
     c->flags_only = true;
     // bodies without kinematics: at rest, centre at the origin
     c->reach.assign(maxf, 0.0f);
-    c->step_bodies.assign(maxf, DevBody{});
+    DevBody unknown{};
+    unknown.reach = -1.0f;  // no analytic shape: every face cell checks its neighbours' flags
+    c->step_bodies.assign(maxf, unknown);
     c->mask_bodies = c->step_bodies;
     HIPCHK(c, hipMemcpyAsync(c->flags, flags, (size_t)c->g.N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

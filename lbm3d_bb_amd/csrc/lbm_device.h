@@ -197,7 +197,7 @@ __device__ inline void macros_of(const C f[Q], const Phys<C>& p, C& rho, C u[3])
 // Writes the finished cell: f* in storage format (STEP, COLLIDE_ONLY) or the
 // pulled f_old plus moments in FP64 (GET, GET_RAW).
 template <class S, int COLL, bool FORCED, int MODE>
-__device__ inline void finish_cell(typename S::C f[Q], const Phys<typename S::C>& p, int64_t cell, int64_t N,
+__device__ inline void finish_cell(typename S::C f[Q], const Phys<typename S::C>& p, int64_t cell, int64_t N, int64_t P,
                                    typename S::T* __restrict__ fout, double* __restrict__ gout,
                                    double* __restrict__ rho_out, double* __restrict__ u_out) {
     using C = typename S::C;
@@ -205,7 +205,7 @@ __device__ inline void finish_cell(typename S::C f[Q], const Phys<typename S::C>
         C rho, u[3];
         collide<C, COLL, FORCED>(f, p, rho, u);
 #pragma unroll
-        for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * N + cell, d, f[d]);
+        for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * P + cell, d, f[d]);
     } else {
         if (gout) {
 #pragma unroll
@@ -223,52 +223,253 @@ __device__ inline void finish_cell(typename S::C f[Q], const Phys<typename S::C>
 }
 
 // ---------------------------------------------------------------------------
-// k_bulk: one thread per cell, cells on non-periodic faces are left to k_fixup.
+// Geometry views shared by the kernels below.
+// ---------------------------------------------------------------------------
+struct SolidView {
+    // mode 0: flags; mode 1: analytic (new and old poses)
+    int mode;
+    int nb;
+    const uint8_t* flags;
+    const DevBody* bnew;
+    const DevBody* bold;
+};
+
+__device__ inline int solid_at(const SolidView& sv, int x, int y, int z, const Grid& g) {
+    if (sv.mode == 0) return sv.flags[(int64_t)x + (int64_t)g.nx * y + g.nxy * z];
+    return flag_of(sv.bnew, sv.nb, x, y, z, g);
+}
+
+__device__ inline bool fresh_at(const SolidView& sv, int x, int y, int z, const Grid& g) {
+    if (sv.mode == 0) return false;
+    return flag_of(sv.bnew, sv.nb, x, y, z, g) == 0 && flag_of(sv.bold, sv.nb, x, y, z, g) != 0;
+}
+
+__device__ inline bool in_box(const Box& b, int x, int y, int z, const Grid& g) {
+    const int q[3] = {x, y, z};
+    const int L[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; ++a) {
+        int r = q[a] - b.lo[a];
+        if ((g.per >> a) & 1) {
+            r %= L[a];
+            if (r < 0) r += L[a];
+        }
+        if (r < 0 || r >= b.ext[a]) return false;
+    }
+    return true;
+}
+
+// Body regions (box kind 1) cover every cell that can be a boundary node, a
+// fresh node or a face cell next to a body, in both geometry modes.
+__device__ inline bool in_body_box(const StepGeom& sg, int x, int y, int z, const Grid& g) {
+    for (int m = 0; m < sg.nbox; ++m)
+        if (sg.box[m].kind == 1 && in_box(sg.box[m], x, y, z, g)) return true;
+    return false;
+}
+
+// number of non-periodic faces the cell lies on
+__device__ inline int open_face_count(int x, int y, int z, const Grid& g) {
+    return (!(g.per & 1) && (x == 0 || x == g.nx - 1)) + (!(g.per & 2) && (y == 0 || y == g.ny - 1)) +
+           (!(g.per & 4) && (z == 0 || z == g.nz - 1));
+}
+
+// Runtime-indexed reads of the face table through selects: indexing the
+// by-value kernel argument directly would copy it to private memory.
+__device__ inline int face_kind_at(const Grid& g, int i) {
+    int k = g.face_kind[0];
+    k = i == 1 ? g.face_kind[1] : k;
+    k = i == 2 ? g.face_kind[2] : k;
+    k = i == 3 ? g.face_kind[3] : k;
+    k = i == 4 ? g.face_kind[4] : k;
+    k = i == 5 ? g.face_kind[5] : k;
+    return k;
+}
+
+__device__ inline double face_u_at(const Grid& g, int i, int c) {
+    double v = g.face_u[0][c];
+    v = i == 1 ? g.face_u[1][c] : v;
+    v = i == 2 ? g.face_u[2][c] : v;
+    v = i == 3 ? g.face_u[3][c] : v;
+    v = i == 4 ? g.face_u[4][c] : v;
+    v = i == 5 ? g.face_u[5][c] : v;
+    return v;
+}
+
+// Face rules for direction d whose source lies outside face fidx:
+// VELOCITY src/OtherBC.jl:143-148, PRESSURE_ABB src/OtherBC.jl:156-165.
+template <typename C>
+__device__ inline C face_value(int kind, int fidx, int d, const Grid& g, C own_opp, C rho_prev, const C up[3],
+                               C uu2) {
+    const C w = weight<C>(d);
+    if (kind == LBM_FACE_VELOCITY) {
+        const C cu = C(kCx[d]) * (-C(face_u_at(g, fidx, 0))) + C(kCy[d]) * (-C(face_u_at(g, fidx, 1))) +
+                     C(kCz[d]) * (-C(face_u_at(g, fidx, 2)));
+        return own_opp - C(2) * w * rho_prev * cu * C(3);
+    }
+    const C cu = C(kCx[d]) * up[0] + C(kCy[d]) * up[1] + C(kCz[d]) * up[2];
+    return -own_opp + C(2) * w * rho_prev * (C(1) + C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3));
+}
+
+template <class S, bool FORCED>
+__device__ inline void own_moments(const typename S::C own[Q], const Phys<typename S::C>& p, typename S::C& rho,
+                                   typename S::C up[3], typename S::C& uu2) {
+    using C = typename S::C;
+    C jx, jy, jz;
+    moments<C>(own, rho, jx, jy, jz);
+    up[0] = jx / rho;
+    up[1] = jy / rho;
+    up[2] = jz / rho;
+    if (FORCED) {
+        up[0] -= C(0.5) * p.g[0];
+        up[1] -= C(0.5) * p.g[1];
+        up[2] -= C(0.5) * p.g[2];
+    }
+    uu2 = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
+}
+
+// ---------------------------------------------------------------------------
+// Face cell on exactly one non-periodic y or z face (AXIS, SIDE): directions
+// whose source lies outside the face take the face rule (or zero gradient);
+// all others are plain pulls with periodic wrap on the remaining axes.
+// ---------------------------------------------------------------------------
+template <class S, int COLL, bool FORCED, int MODE, int AXIS, int SIDE>
+__device__ __forceinline__ void face_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                          double* __restrict__ gout, double* __restrict__ rho_out, double* __restrict__ u_out,
+                          const Grid& g, const Phys<typename S::C>& p, int x, int y, int z) {
+    using C = typename S::C;
+    const int64_t N = g.N;
+    const int nx = g.nx, nxy = (int)g.nxy;
+    const int cell = x + nx * y + nxy * z;
+    const int fidx = 2 * AXIS + SIDE;
+    const int kind = g.face_kind[fidx];
+    const int wxm = ((g.per & 1) && x == 0) ? nx : 0, wxp = ((g.per & 1) && x == nx - 1) ? -nx : 0;
+    const int wym = (AXIS != 1 && y == 0) ? nxy : 0, wyp = (AXIS != 1 && y == g.ny - 1) ? -nxy : 0;
+    const int wzm = (AXIS != 2 && z == 0) ? (int)N : 0, wzp = (AXIS != 2 && z == g.nz - 1) ? -(int)N : 0;
+    const int axis_stride = AXIS == 1 ? nx : nxy;
+    C own[Q], f[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        const int ca = AXIS == 1 ? kCy[d] : kCz[d];
+        const bool out = SIDE == 0 ? (ca > 0) : (ca < 0);
+        int src = cell - (kCx[d] + kCy[d] * nx + kCz[d] * nxy);
+        if (kCx[d] > 0) src += wxm;
+        if (kCx[d] < 0) src += wxp;
+        if (kCy[d] > 0) src += wym;
+        if (kCy[d] < 0) src += wyp;
+        if (kCz[d] > 0) src += wzm;
+        if (kCz[d] < 0) src += wzp;
+        // zero gradient: the source clamped back onto the face plane
+        if (out) src = kind == LBM_FACE_ZERO_GRADIENT ? src + ca * axis_stride : cell;
+        f[d] = S::load(fin + (int64_t)d * g.P + src, d);
+    }
+    if (kind != LBM_FACE_ZERO_GRADIENT) {
+        C rho_prev, up[3], uu2;
+        own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            const int ca = AXIS == 1 ? kCy[d] : kCz[d];
+            const bool out = SIDE == 0 ? (ca > 0) : (ca < 0);
+            if (out) f[d] = face_value<C>(kind, fidx, d, g, own[kOpp[d]], rho_prev, up, uu2);
+        }
+    }
+    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+}
+
+// ---------------------------------------------------------------------------
+// k_bulk: a block is a run of blockDim.x consecutive x cells (up to 256, a
+// whole row when Nx <= 256) in blockDim.y rows, so every wave streams 64
+// consecutive cells of one row in each direction plane, the +-1 shifted loads
+// of the x neighbours stay inside the block's own cache lines, and no thread
+// divides.  It finishes every cell except
+//   * domain edges (cells on two or more non-periodic faces), and
+//   * face cells inside a body region,
+// which k_fixup does.  A row on a y or z face takes the face path as a whole
+// wave; a cell on an x face takes the face rule for its five outgoing
+// directions inside the row's own coalesced loads.
 // ---------------------------------------------------------------------------
 template <class S, int COLL, bool FORCED, int MODE>
 __global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict__ fin,
                                                  typename S::T* __restrict__ fout, double* __restrict__ gout,
                                                  double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
-                                                 Phys<typename S::C> p) {
+                                                 Phys<typename S::C> p, const StepGeom* __restrict__ sgp) {
     using C = typename S::C;
-    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny) return;
+    const int nx = g.nx, nxy = (int)g.nxy;
     const int64_t N = g.N;
-    if (cell >= N) return;
-    int x, y, z;
-    cell_xyz(cell, g, x, y, z);
+    const int cell = x + nx * y + nxy * z;
     C f[Q];
     if (MODE == MODE_COLLIDE_ONLY || MODE == MODE_GET_RAW) {
 #pragma unroll
-        for (int d = 0; d < Q; ++d) f[d] = S::load(fin + (int64_t)d * N + cell, d);
-    } else {
-        if (on_open_face(x, y, z, g)) return;
-        // periodic wraps (zero on non-periodic axes: those faces returned above)
-        const int64_t wxm = (x == 0) ? (int64_t)g.nx : 0, wxp = (x == g.nx - 1) ? -(int64_t)g.nx : 0;
-        const int64_t wym = (y == 0) ? g.nxy : 0, wyp = (y == g.ny - 1) ? -g.nxy : 0;
-        const int64_t wzm = (z == 0) ? N : 0, wzp = (z == g.nz - 1) ? -N : 0;
+        for (int d = 0; d < Q; ++d) f[d] = S::load(fin + (int64_t)d * g.P + cell, d);
+        finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+        return;
+    }
+    const bool px = g.per & 1;
+    const bool xlo = !px && x == 0, xhi = !px && x == nx - 1;
+    const bool ylo = !(g.per & 2) && y == 0, yhi = !(g.per & 2) && y == g.ny - 1;
+    const bool zlo = !(g.per & 4) && z == 0, zhi = !(g.per & 4) && z == g.nz - 1;
+    const int nyz = (ylo || yhi) + (zlo || zhi);
+    const bool xf = xlo || xhi;
+    if (nyz + xf >= 2) return;  // domain edge
+    if ((nyz + xf) == 1 && sgp && in_body_box(*sgp, x, y, z, g)) return;
+    if (nyz == 1) {
+        if (ylo) face_cell<S, COLL, FORCED, MODE, 1, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+        else if (yhi) face_cell<S, COLL, FORCED, MODE, 1, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+        else if (zlo) face_cell<S, COLL, FORCED, MODE, 2, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+        else face_cell<S, COLL, FORCED, MODE, 2, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+        return;
+    }
+    // periodic wraps (zero on non-periodic axes)
+    const int wxm = (px && x == 0) ? nx : 0, wxp = (px && x == nx - 1) ? -nx : 0;
+    const int wym = (y == 0) ? nxy : 0, wyp = (y == g.ny - 1) ? -nxy : 0;
+    const int wzm = (z == 0) ? (int)N : 0, wzp = (z == g.nz - 1) ? -(int)N : 0;
+    const int xkind = xlo ? g.face_kind[0] : (xhi ? g.face_kind[1] : -1);
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        int src = cell - (kCx[d] + kCy[d] * nx + kCz[d] * nxy);
+        if (kCx[d] > 0) src += wxm;
+        if (kCx[d] < 0) src += wxp;
+        if (kCy[d] > 0) src += wym;
+        if (kCy[d] < 0) src += wyp;
+        if (kCz[d] > 0) src += wzm;
+        if (kCz[d] < 0) src += wzp;
+        if (kCx[d] != 0) {
+            // x-face lane, outgoing direction: zero gradient clamps onto the face plane,
+            // the other rules read the cell itself
+            const bool out = kCx[d] > 0 ? xlo : xhi;
+            if (out) src = xkind == LBM_FACE_ZERO_GRADIENT ? src + kCx[d] : cell;
+        }
+        f[d] = S::load(fin + (int64_t)d * g.P + src, d);
+    }
+    if (xf && xkind != LBM_FACE_ZERO_GRADIENT) {
+        C own[Q];
+#pragma unroll
+        for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
+        C rho_prev, up[3], uu2;
+        own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
+        const int fidx = xlo ? 0 : 1;
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
-            int64_t src = cell - (kCx[d] + (int64_t)kCy[d] * g.nx + (int64_t)kCz[d] * g.nxy);
-            if (kCx[d] > 0) src += wxm;
-            if (kCx[d] < 0) src += wxp;
-            if (kCy[d] > 0) src += wym;
-            if (kCy[d] < 0) src += wyp;
-            if (kCz[d] > 0) src += wzm;
-            if (kCz[d] < 0) src += wzp;
-            f[d] = S::load(fin + (int64_t)d * N + src, d);
+            if (kCx[d] == 0) continue;
+            const bool out = kCx[d] > 0 ? xlo : xhi;
+            if (out) f[d] = face_value<C>(xkind, fidx, d, g, own[kOpp[d]], rho_prev, up, uu2);
         }
     }
-    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, fout, gout, rho_out, u_out);
+    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
 }
 
 // ---------------------------------------------------------------------------
-// Force accumulation: per thread up to two bodies in registers (a fluid node
-// next to a third body is accumulated through an LDS atomic), per wave a
+// Force accumulation: per thread one body in registers (a fluid node next to a
+// second body accumulates that one through LDS atomics), per wave a
 // deterministic segmented shuffle reduction, per block a fixed-order sum.
 // ---------------------------------------------------------------------------
 struct ForceAcc {
-    int b0, b1;
-    double a0[6], a1[6];
+    int b0;
+    double a0[6];
 };
 
 __device__ inline double wave_sum(double v) {
@@ -295,29 +496,14 @@ __device__ inline void wave_reduce_to_lds(int key, const double vals[6], double*
 }
 
 // ---------------------------------------------------------------------------
-// The general cell: faces, bounce-back links, refill, momentum exchange.
+// The general cell: faces (any number, with precedence), bounce-back links,
+// refill, momentum exchange.  Every direction is classified first, then all
+// loads are issued together, so the cell costs one memory round trip.
 // ---------------------------------------------------------------------------
-struct SolidView {
-    // mode 0: flags; mode 1: analytic (new and old poses)
-    int mode;
-    int nb;
-    const uint8_t* flags;
-    const DevBody* bnew;
-    const DevBody* bold;
-};
-
-__device__ inline int solid_at(const SolidView& sv, int x, int y, int z, const Grid& g) {
-    if (sv.mode == 0) return sv.flags[(int64_t)x + (int64_t)g.nx * y + g.nxy * z];
-    return flag_of(sv.bnew, sv.nb, x, y, z, g);
-}
-
-__device__ inline bool fresh_at(const SolidView& sv, int x, int y, int z, const Grid& g) {
-    if (sv.mode == 0) return false;
-    return flag_of(sv.bnew, sv.nb, x, y, z, g) == 0 && flag_of(sv.bold, sv.nb, x, y, z, g) != 0;
-}
+enum { DIR_PULL = 0, DIR_FACE = 1, DIR_LINK = 2 };
 
 template <class S, int COLL, bool FORCED, int MODE>
-__device__ void general_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+__device__ __forceinline__ void general_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                              double* __restrict__ gout, double* __restrict__ rho_out, double* __restrict__ u_out,
                              const Grid& g, const Phys<typename S::C>& p, const SolidView& sv, int x, int y, int z,
                              ForceAcc& acc, double* lds_overflow) {
@@ -325,22 +511,14 @@ __device__ void general_cell(const typename S::T* __restrict__ fin, typename S::
     const int64_t N = g.N;
     const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
     const int L[3] = {g.nx, g.ny, g.nz};
+    const int q0[3] = {x, y, z};
+    // solid cells are streamed like fluid ones (no links), as in the reference
+    const bool self_solid = sv.nb > 0 && solid_at(sv, x, y, z, g) != 0;
+    const bool fresh = sv.nb > 0 && !self_solid && g.refill && fresh_at(sv, x, y, z, g);
     C own[Q];
 #pragma unroll
-    for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * N + cell, d);
-    // previous-step moments of this node from its own f*
-    C rho_prev, jx, jy, jz;
-    moments<C>(own, rho_prev, jx, jy, jz);
-    C up[3] = {jx / rho_prev, jy / rho_prev, jz / rho_prev};
-    if (FORCED) {
-        up[0] -= C(0.5) * p.g[0];
-        up[1] -= C(0.5) * p.g[1];
-        up[2] -= C(0.5) * p.g[2];
-    }
+    for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
     C f[Q];
-    // solid cells are streamed like fluid ones (no links), as in the reference
-    const bool self_solid = solid_at(sv, x, y, z, g) != 0;
-    const bool fresh = !self_solid && g.refill && fresh_at(sv, x, y, z, g);
     if (fresh) {
         // equilibrium of the mean previous-step rho, u of fluid neighbours
         C sr = C(0), su = C(0), sv_ = C(0), sw = C(0);
@@ -353,7 +531,8 @@ __device__ void general_cell(const typename S::T* __restrict__ fin, typename S::
             if (solid_at(sv, q[0], q[1], q[2], g) != 0 || fresh_at(sv, q[0], q[1], q[2], g)) continue;
             const int64_t nc = (int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2];
             C nf[Q];
-            for (int e = 0; e < Q; ++e) nf[e] = S::load(fin + (int64_t)e * N + nc, e);
+#pragma unroll
+            for (int e = 0; e < Q; ++e) nf[e] = S::load(fin + (int64_t)e * g.P + nc, e);
             C r, mx, my, mz;
             moments<C>(nf, r, mx, my, mz);
             C ux = mx / r, uy = my / r, uz = mz / r;
@@ -382,69 +561,78 @@ __device__ void general_cell(const typename S::T* __restrict__ fin, typename S::
             const C cu = C(kCx[d]) * ux + C(kCy[d]) * uy + C(kCz[d]) * uz;
             f[d] = weight<C>(d) * r * (C(1) + cu * C(3) + C(0.5) * cu * cu * C(9) - C(0.5) * u2 * C(3));
         }
-    } else {
-        const int q0[3] = {x, y, z};
+        finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+        return;
+    }
+    // 1) classify every direction and issue its load right away (the loaded
+    //    values are only consumed after the loop, so the loads overlap)
+    int code[Q];  // DIR_* | (face index or body index) << 2
 #pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            const int cd[3] = {kCx[d], kCy[d], kCz[d]};
-            int s[3];
-            int face_axis_side[3] = {-1, -1, -1};
-            bool out = false;
-            for (int a = 0; a < 3; ++a) {
-                s[a] = q0[a] - cd[a];
-                if (s[a] < 0 || s[a] >= L[a]) {
-                    if ((g.per >> a) & 1) {
-                        s[a] = s[a] < 0 ? s[a] + L[a] : s[a] - L[a];
-                    } else {
-                        face_axis_side[a] = 2 * a + (s[a] >= L[a] ? 1 : 0);
-                        out = true;
-                    }
-                }
-            }
-            const C w = weight<C>(d);
-            if (out) {
-                // edge precedence: VELOCITY > PRESSURE_ABB > ZERO_GRADIENT, lower axis first
-                int chosen = -1, kind = -1;
-                const int order[3] = {LBM_FACE_VELOCITY, LBM_FACE_PRESSURE_ABB, LBM_FACE_ZERO_GRADIENT};
-                for (int k = 0; k < 3 && chosen < 0; ++k)
-                    for (int a = 0; a < 3; ++a)
-                        if (face_axis_side[a] >= 0 && g.face_kind[face_axis_side[a]] == order[k]) {
-                            chosen = face_axis_side[a];
-                            kind = order[k];
-                            break;
-                        }
-                if (kind == LBM_FACE_VELOCITY) {
-                    // src/OtherBC.jl:143-148
-                    const C cu = C(cd[0]) * (-C(g.face_u[chosen][0])) + C(cd[1]) * (-C(g.face_u[chosen][1])) +
-                                 C(cd[2]) * (-C(g.face_u[chosen][2]));
-                    f[d] = own[kOpp[d]] - C(2) * w * rho_prev * cu * C(3);
-                } else if (kind == LBM_FACE_PRESSURE_ABB) {
-                    // src/OtherBC.jl:156-165
-                    const C cu = C(cd[0]) * up[0] + C(cd[1]) * up[1] + C(cd[2]) * up[2];
-                    const C uu2 = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
-                    f[d] = -own[kOpp[d]] + C(2) * w * rho_prev * (C(1) + C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3));
+    for (int d = 0; d < Q; ++d) {
+        const int cd[3] = {kCx[d], kCy[d], kCz[d]};
+        int s[3];
+        int fas[3] = {-1, -1, -1};
+        bool out = false;
+        for (int a = 0; a < 3; ++a) {
+            s[a] = q0[a] - cd[a];
+            if (s[a] < 0 || s[a] >= L[a]) {
+                if ((g.per >> a) & 1) {
+                    s[a] = s[a] < 0 ? s[a] + L[a] : s[a] - L[a];
                 } else {
-                    for (int a = 0; a < 3; ++a)
-                        if (face_axis_side[a] >= 0) s[a] = q0[a];
-                    const int64_t src = (int64_t)s[0] + (int64_t)g.nx * s[1] + g.nxy * s[2];
-                    f[d] = S::load(fin + (int64_t)d * N + src, d);
+                    fas[a] = 2 * a + (s[a] >= L[a] ? 1 : 0);
+                    out = true;
                 }
-                continue;
             }
-            const int sb = self_solid ? 0 : solid_at(sv, s[0], s[1], s[2], g);
-            if (sb == 0) {
-                const int64_t src = (int64_t)s[0] + (int64_t)g.nx * s[1] + g.nxy * s[2];
-                f[d] = S::load(fin + (int64_t)d * N + src, d);
-                continue;
+        }
+        code[d] = DIR_PULL;
+        if (out) {
+            // edge precedence: VELOCITY > PRESSURE_ABB > ZERO_GRADIENT, lower axis first
+            int chosen = -1, kind = LBM_FACE_ZERO_GRADIENT;
+            const int order[2] = {LBM_FACE_VELOCITY, LBM_FACE_PRESSURE_ABB};
+            for (int k = 0; k < 2 && chosen < 0; ++k)
+                for (int a = 0; a < 3 && chosen < 0; ++a)
+                    if (fas[a] >= 0 && face_kind_at(g, fas[a]) == order[k]) {
+                        chosen = fas[a];
+                        kind = order[k];
+                    }
+            if (kind == LBM_FACE_ZERO_GRADIENT) {
+                for (int a = 0; a < 3; ++a)
+                    if (fas[a] >= 0) s[a] = q0[a];
+            } else {
+                code[d] = DIR_FACE | (chosen << 2);
+                s[0] = x;
+                s[1] = y;
+                s[2] = z;
             }
-            // boundary link (x, dl) with dl = opp(d): the solid node is x + c_dl = s.
+        } else if (!self_solid && sv.nb > 0) {
+            const int sb = solid_at(sv, s[0], s[1], s[2], g);
+            if (sb != 0) {
+                code[d] = DIR_LINK | ((sb - 1) << 2);
+                s[0] = x;
+                s[1] = y;
+                s[2] = z;
+            }
+        }
+        f[d] = S::load(fin + (int64_t)d * g.P + (s[0] + g.nx * s[1] + (int)g.nxy * s[2]), d);
+    }
+    // 2) boundary values
+    C rho_prev, up[3], uu2;
+    own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        const int kd = code[d] & 3, aux = code[d] >> 2;
+        if (kd == DIR_FACE) {
+            f[d] = face_value<C>(face_kind_at(g, aux), aux, d, g, own[kOpp[d]], rho_prev, up, uu2);
+        } else if (kd == DIR_LINK) {
+            // boundary link (x, dl), dl = opp(d): the solid node is x + c_dl
             const int dl = kOpp[d];
-            const DevBody& B = sv.bnew[sb - 1];
+            const DevBody& B = sv.bnew[aux];
             // wall point: physical x + c_dl/2, or the reference's x - c_dl/2
             const double hs = g.wall_point == LBM_WALL_REFERENCE ? -0.5 : 0.5;
             double r[3];
             for (int a = 0; a < 3; ++a) {
-                r[a] = (double)q0[a] + 0.5 + hs * (double)(-cd[a]) - B.ch[a];
+                const int cdl = a == 0 ? kCx[dl] : (a == 1 ? kCy[dl] : kCz[dl]);
+                r[a] = (double)q0[a] + 0.5 + hs * (double)cdl - B.ch[a];
                 if ((g.per >> a) & 1) {
                     const double h = 0.5 * L[a];
                     if (r[a] > h) r[a] -= L[a];
@@ -459,12 +647,13 @@ __device__ void general_cell(const typename S::T* __restrict__ fin, typename S::
             const C val = fin_dl - C(2) * weight<C>(dl) * rho_prev * cuw * C(3);
             f[d] = val;
             if (MODE == MODE_STEP) {
-                // momentum exchange of this link (src/MEA.jl:102-107), FP64
+                // momentum exchange of this link (src/MEA.jl:102-107), FP64, arm to x + c_dl/2
                 const double t = (double)(fin_dl + val);
                 const double P[3] = {t * kCx[dl], t * kCy[dl], t * kCz[dl]};
                 double ra[3];
                 for (int a = 0; a < 3; ++a) {
-                    ra[a] = (double)q0[a] + 0.5 + 0.5 * (double)(-cd[a]) - B.ch[a];
+                    const int cdl = a == 0 ? kCx[dl] : (a == 1 ? kCy[dl] : kCz[dl]);
+                    ra[a] = (double)q0[a] + 0.5 + 0.5 * (double)cdl - B.ch[a];
                     if ((g.per >> a) & 1) {
                         const double h = 0.5 * L[a];
                         if (ra[a] > h) ra[a] -= L[a];
@@ -473,43 +662,30 @@ __device__ void general_cell(const typename S::T* __restrict__ fin, typename S::
                 }
                 const double v6[6] = {P[0], P[1], P[2], ra[1] * P[2] - ra[2] * P[1], ra[2] * P[0] - ra[0] * P[2],
                                       ra[0] * P[1] - ra[1] * P[0]};
-                const int bi = sb - 1;
+                const int bi = aux;
                 if (acc.b0 < 0 || acc.b0 == bi) {
                     acc.b0 = bi;
                     for (int c = 0; c < 6; ++c) acc.a0[c] += v6[c];
-                } else if (acc.b1 < 0 || acc.b1 == bi) {
-                    acc.b1 = bi;
-                    for (int c = 0; c < 6; ++c) acc.a1[c] += v6[c];
                 } else {
                     for (int c = 0; c < 6; ++c) atomicAdd(&lds_overflow[bi * 6 + c], v6[c]);
                 }
             }
         }
     }
-    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, fout, gout, rho_out, u_out);
+    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
 }
 
 // ---------------------------------------------------------------------------
-// k_fixup: blocks [0, sg.nblocks) walk the boxes (face slabs, then body
-// regions); in static mode blocks [sg.nblocks, sg.nblocks + sg.list_blocks)
-// walk the compacted boundary-node list.  In moving mode a body-region thread
-// also rewrites the flag of its cell.  Every block writes nbodies*6 partial
-// force/torque sums (fixed order) to partials[blockIdx].
+// k_fixup: the general path.  Blocks [0, sg.nfix_blocks) walk the boxes of
+// kind 2 (domain edges: every cell) and kind 1 (body regions); blocks
+// [sg.nfix_blocks, + sg.list_blocks) walk the static boundary-node list.
+//  * body region, moving bodies (mode 1): the thread rewrites its cell's flag,
+//    then takes face cells, boundary nodes and fresh nodes;
+//  * body region, static bodies (mode 0): face cells only;
+//  * list: boundary nodes not on a face.
+// Cells in several regions go to the first.  Every block writes nbodies*6
+// partial force/torque sums (fixed order) to partials[blockIdx].
 // ---------------------------------------------------------------------------
-__device__ inline bool in_box(const Box& b, int x, int y, int z, const Grid& g) {
-    const int q[3] = {x, y, z};
-    const int L[3] = {g.nx, g.ny, g.nz};
-    for (int a = 0; a < 3; ++a) {
-        int r = q[a] - b.lo[a];
-        if ((g.per >> a) & 1) {
-            r %= L[a];
-            if (r < 0) r += L[a];
-        }
-        if (r < 0 || r >= b.ext[a]) return false;
-    }
-    return true;
-}
-
 template <class S, int COLL, bool FORCED, int MODE>
 __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restrict__ fin,
                                                   typename S::T* __restrict__ fout, double* __restrict__ gout,
@@ -538,14 +714,14 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restric
     sv.bold = bold;
 
     ForceAcc acc;
-    acc.b0 = acc.b1 = -1;
-    for (int c = 0; c < 6; ++c) acc.a0[c] = acc.a1[c] = 0.0;
+    acc.b0 = -1;
+    for (int c = 0; c < 6; ++c) acc.a0[c] = 0.0;
 
     const int blk = blockIdx.x;
     bool work = false;
     int x = 0, y = 0, z = 0;
-    if (blk < sg.nblocks) {
-        int bi = 0;
+    if (blk < sg.nfix_blocks) {
+        int bi = sg.first_fix_box;
         while (bi + 1 < sg.nbox && sg.box[bi + 1].first_block <= blk) ++bi;
         const Box b = sg.box[bi];
         const int64_t t = (int64_t)(blk - b.first_block) * kBlock + tid;
@@ -564,19 +740,22 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restric
             if (y >= g.ny) y -= g.ny;
             if (z < 0) z += g.nz;
             if (z >= g.nz) z -= g.nz;
-            if (b.kind == 0) {
-                work = true;  // face slab: every cell
-            } else {
-                bool dup = false;
-                for (int m = 0; m < bi && !dup; ++m)
-                    if (sg.box[m].kind == 1 && in_box(sg.box[m], x, y, z, g)) dup = true;
-                if (!dup) {
+            bool dup = false;
+            for (int m = sg.first_fix_box; m < bi && !dup; ++m) dup = in_box(sg.box[m], x, y, z, g);
+            if (!dup && sv.mode == 1 && write_flags) {
+                const int fl = flag_of(bnew, nb, x, y, z, g);
+                const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
+                if (flags[cell] != (uint8_t)fl) flags[cell] = (uint8_t)fl;
+            }
+            if (b.kind == 2) {
+                work = !dup;  // domain edge: every cell
+            } else if (!dup) {
+                const bool face = open_face_count(x, y, z, g) == 1;  // edges went to the edge boxes
+                if (sv.mode == 1) {
                     const int fl = flag_of(bnew, nb, x, y, z, g);
-                    if (write_flags) {
-                        const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
-                        if (flags[cell] != (uint8_t)fl) flags[cell] = (uint8_t)fl;
-                    }
-                    if (fl == 0 && !on_open_face(x, y, z, g)) {
+                    if (face) {
+                        work = true;
+                    } else if (fl == 0) {
                         // fluid: work if any neighbour is solid or the node is fresh
                         bool nearb = g.refill && fresh_at(sv, x, y, z, g);
                         for (int d = 0; d < 18 && !nearb; ++d) {
@@ -588,29 +767,29 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restric
                         }
                         work = nearb;
                     }
+                } else {
+                    work = face;
                 }
             }
         }
     } else {
-        const int64_t t = (int64_t)(blk - sg.nblocks) * kBlock + tid;
+        const int64_t t = (int64_t)(blk - sg.nfix_blocks) * kBlock + tid;
         if (t < (int64_t)(*list_count)) {
             const int64_t cell = list[t];
             int64_t r = cell / g.nx;
             x = (int)(cell - r * g.nx);
             z = (int)(r / g.ny);
             y = (int)(r - (int64_t)z * g.ny);
-            work = !on_open_face(x, y, z, g);  // face cells belong to the face slabs
+            work = open_face_count(x, y, z, g) == 0;  // face cells belong to the regions
         }
     }
     if (work) {
-        if (MODE == MODE_STEP || MODE == MODE_GET) {
+        if (MODE == MODE_STEP || MODE == MODE_GET)
             general_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sv, x, y, z, acc, s_over);
-        }
     }
     if (MODE == MODE_STEP && nb > 0) {
         const int wave = tid >> 6;
         wave_reduce_to_lds(acc.b0, acc.a0, s_acc[wave]);
-        wave_reduce_to_lds(acc.b1, acc.a1, s_acc[wave]);
         __syncthreads();
         for (int i = tid; i < nb * 6; i += kBlock) {
             double s = s_over[i];
@@ -628,7 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restric
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_equilibrium(typename S::T* __restrict__ f, const double* __restrict__ rho,
                                                         const double* __restrict__ u, double rho0, double u0x,
-                                                        double u0y, double u0z, int64_t N) {
+                                                        double u0y, double u0z, int64_t N, int64_t P) {
     using C = typename S::C;
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (cell >= N) return;
@@ -641,17 +820,17 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(typename S::T* __restric
     for (int d = 0; d < Q; ++d) {
         const C cu = C(kCx[d]) * ux + C(kCy[d]) * uy + C(kCz[d]) * uz;
         const C feq = weight<C>(d) * r * (C(1) + cu * C(3) + C(0.5) * cu * cu * C(9) - C(0.5) * u2 * C(3));
-        S::store(f + (int64_t)d * N + cell, d, feq);
+        S::store(f + (int64_t)d * P + cell, d, feq);
     }
 }
 
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_from_double(typename S::T* __restrict__ f, const double* __restrict__ src,
-                                                        int64_t N) {
+                                                        int64_t N, int64_t P) {
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (cell >= N) return;
 #pragma unroll
-    for (int d = 0; d < Q; ++d) S::store(f + (int64_t)d * N + cell, d, (typename S::C)src[(int64_t)d * N + cell]);
+    for (int d = 0; d < Q; ++d) S::store(f + (int64_t)d * P + cell, d, (typename S::C)src[(int64_t)d * N + cell]);
 }
 
 }  // namespace lbm

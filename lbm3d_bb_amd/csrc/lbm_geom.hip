@@ -5,14 +5,28 @@
 
 namespace lbm {
 
-// Sums the block partials in block order: out[nb*6].
-__global__ __launch_bounds__(kBlock) void k_force_finalize(const double* __restrict__ partials, int nblocks, int n6,
-                                                           double* __restrict__ out) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n6) return;
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += partials[(int64_t)b * n6 + i];
-    out[i] = s;
+// Sums the block partials of each step of a chunk: one block per (component,
+// step), a strided pass over the blocks then a fixed-shape tree, so the result
+// is deterministic.  out[s][i] = sum_{b < grid(s)} partials[s * step_stride + b * n6 + i],
+// with grid(s) read from the step's record.
+__global__ __launch_bounds__(kBlock) void k_force_finalize(const double* __restrict__ partials, int64_t step_stride,
+                                                           const char* __restrict__ recs, size_t rec_bytes,
+                                                           int n6, double* __restrict__ out) {
+    __shared__ double s_sum[kBlock];
+    const int i = blockIdx.x;
+    const int s = blockIdx.y;
+    const StepGeom* sg = (const StepGeom*)(recs + (size_t)s * rec_bytes);
+    const int nblocks = sg->nfix_blocks + sg->list_blocks;
+    const double* p = partials + (int64_t)s * step_stride;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) acc += p[(int64_t)b * n6 + i];
+    s_sum[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s_sum[threadIdx.x] += s_sum[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(int64_t)s * n6 + i] = s_sum[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -230,9 +244,10 @@ int launch_bb_fill(const uint8_t* flags, const Grid& g, const int32_t* node_off,
     LAUNCH_CHECK();
 }
 
-int launch_force_finalize(const double* partials, int nblocks, int n6, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_force_finalize, dim3((unsigned)nblocks_of(n6)), dim3(kBlock), 0, st, partials, nblocks, n6,
-                       out);
+int launch_force_finalize(const double* partials, int64_t step_stride, const char* recs, size_t rec_bytes,
+                          int nsteps, int n6, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_force_finalize, dim3((unsigned)n6, (unsigned)nsteps), dim3(kBlock), 0, st,
+                       partials, step_stride, recs, rec_bytes, n6, out);
     LAUNCH_CHECK();
 }
 

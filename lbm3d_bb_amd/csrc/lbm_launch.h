@@ -9,8 +9,11 @@ namespace lbm {
 template <class S, int MODE, int COLL, bool FORCED>
 static void launch_bulk_t(const LaunchReq& r, const Phys<typename S::C>& p) {
     using T = typename S::T;
-    hipLaunchKernelGGL((k_bulk<S, COLL, FORCED, MODE>), dim3((unsigned)nblocks_of(r.g.N)), dim3(kBlock), 0, r.stream,
-                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p);
+    const int bx = r.g.nx >= 256 ? 256 : (r.g.nx >= 128 ? 128 : 64);
+    const int by = kBlock / bx;
+    const dim3 grid((unsigned)((r.g.nx + bx - 1) / bx), (unsigned)((r.g.ny + by - 1) / by), (unsigned)r.g.nz);
+    hipLaunchKernelGGL((k_bulk<S, COLL, FORCED, MODE>), grid, dim3(bx, by), 0, r.stream,
+                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, (const StepGeom*)r.rec);
 }
 
 template <class S, int MODE, int COLL, bool FORCED>
@@ -59,10 +62,10 @@ int launch(const LaunchReq& r) {
             break;
         case K_EQUILIBRIUM:
             hipLaunchKernelGGL(k_equilibrium<S>, grid, block, 0, r.stream, (T*)r.fout, r.src_rho, r.src_u, r.rho0,
-                               r.u0[0], r.u0[1], r.u0[2], r.g.N);
+                               r.u0[0], r.u0[1], r.u0[2], r.g.N, r.g.P);
             break;
         case K_FROM_DOUBLE:
-            hipLaunchKernelGGL(k_from_double<S>, grid, block, 0, r.stream, (T*)r.fout, r.src_f, r.g.N);
+            hipLaunchKernelGGL(k_from_double<S>, grid, block, 0, r.stream, (T*)r.fout, r.src_f, r.g.N, r.g.P);
             break;
         default: return (int)hipErrorInvalidValue;
     }

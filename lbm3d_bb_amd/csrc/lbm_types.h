@@ -10,7 +10,7 @@ namespace lbm {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxBoxes = 6 + LBM_MAX_BODIES;
+constexpr int kMaxBoxes = 6 + 12 + LBM_MAX_BODIES;
 
 // Body as the kernels see it: FP32 mask pose (inside test) and FP64 half-step
 // kinematics (moving-wall velocity and torque arm).
@@ -20,7 +20,7 @@ struct DevBody {
     float r2;
     float ia2[3];
     float rot[9];
-    float pad;
+    float reach;  // bounding half-extent of the mask (< 0: unknown, flags only)
     double ch[3];
     double v[3];
     double om[3];
@@ -33,12 +33,15 @@ struct Box {
     int ext[3];
     int kind;
     int first_block;
+    int face;  // face slab: 2 * axis + side (0..5); body region: -1
+    int pad;
 };
 
 struct Grid {
     int nx, ny, nz;
     int per;  // bit a: axis a periodic
     int64_t N;
+    int64_t P;  // population plane stride (elements): N plus a pad that staggers the planes' channel mapping
     int64_t nxy;
     int face_kind[6];
     double face_u[6][3];
@@ -54,13 +57,17 @@ struct Phys {
 };
 
 // Geometry of one step for the fix-up pass.
+// Boxes: face slabs (kind 0) first, walked by k_faces; then domain edges
+// (kind 2) and body regions (kind 1), walked by k_fixup.
 struct StepGeom {
     int nbox;
-    int nblocks;     // blocks of all boxes
+    int nface_blocks;  // k_faces grid (blocks over the face slabs)
+    int nfix_blocks;   // k_fixup blocks over edges and body regions
+    int first_fix_box; // index of the first edge/body box
     int nbodies;
-    int mode;        // 0: solidity from flags (static), 1: analytic (moving bodies)
-    int list_blocks; // blocks over the static boundary-node list (mode 0)
-    int pad[3];
+    int mode;          // 0: solidity from flags (static), 1: analytic (moving bodies)
+    int list_blocks;   // k_fixup blocks over the static boundary-node list (mode 0)
+    int pad;
     Box box[kMaxBoxes];
 };
 
@@ -68,7 +75,7 @@ enum { MODE_STEP = 0, MODE_GET = 1, MODE_COLLIDE_ONLY = 2, MODE_GET_RAW = 3 };
 
 // One kernel launch request; the per-storage translation units implement
 // launch<S>() for it, so the host code never instantiates kernels itself.
-enum { K_BULK = 0, K_FIXUP = 1, K_EQUILIBRIUM = 2, K_FROM_DOUBLE = 3 };
+enum { K_BULK = 0, K_FIXUP = 1, K_EQUILIBRIUM = 2, K_FROM_DOUBLE = 3, K_FACES = 4 };
 
 struct LaunchReq {
     int kind;
@@ -114,7 +121,8 @@ int launch_bb_count(const uint8_t* flags, const Grid& g, int32_t* node_counts, i
 int launch_scan2(int32_t* a, int32_t* b, int n, hipStream_t st);
 int launch_bb_fill(const uint8_t* flags, const Grid& g, const int32_t* node_off, const int32_t* link_off,
                    int32_t* nodes, int32_t* link_cell, int8_t* link_dir, hipStream_t st);
-int launch_force_finalize(const double* partials, int nblocks, int n6, double* out, hipStream_t st);
+int launch_force_finalize(const double* partials, int64_t step_stride, const char* recs, size_t rec_bytes,
+                          int nsteps, int n6, double* out, hipStream_t st);
 int launch_conservation(const double* rho, const double* u, const uint8_t* flags, int64_t N, double* partials,
                         hipStream_t st);
 

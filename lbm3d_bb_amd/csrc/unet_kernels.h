@@ -1,0 +1,436 @@
+// unet_kernels.h -- gfx950 kernels of the warm-start U-Net.
+//
+// Activations are BF16, channels last: element (voxel v, channel c) of a tensor
+// with channel stride cs and channel offset co sits at p[v * cs + co + c];
+// v = ((n * D + z) * H + y) * W + x.  Convolutions are implicit GEMMs on
+// v_mfma_f32_16x16x32_bf16: rows = voxels, columns = output channels, the
+// reduction runs over (tap, input channel) in groups of 8 channels so that one
+// lane loads 16 contiguous bytes of one voxel per group.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace unet {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+
+// Geometry of a convolution: which input voxel feeds output row r through tap t.
+enum { G_CONV3 = 0, G_DOWN2 = 1, G_UP2 = 2, G_PW = 3 };
+
+struct Dims {
+    int n, d, h, w;
+};
+
+__device__ inline int64_t vox(const Dims& s, int n, int z, int y, int x) {
+    return (((int64_t)n * s.d + z) * s.h + y) * s.w + x;
+}
+
+__device__ inline void unvox(const Dims& s, int64_t v, int& n, int& z, int& y, int& x) {
+    x = (int)(v % s.w);
+    int64_t t = v / s.w;
+    y = (int)(t % s.h);
+    t /= s.h;
+    z = (int)(t % s.d);
+    n = (int)(t / s.d);
+}
+
+struct ConvArgs {
+    const bf16* x;
+    int xcs, xco, cin;      // input: channel stride, offset, channels (multiple of 8)
+    const bf16* wt;         // [npar][coutp][kpad]
+    int coutp, kpad, ntaps;
+    const float* bias;      // [coutp] or null
+    bf16* y;                // BF16 output or null
+    int ycs, yco, cstore;   // output channel stride, offset, channels stored
+    float* yf;              // FP32 output (channel stride cstore) or null
+    const bf16* aux;        // if set: output *= (aux > 0) (ReLU backward)
+    int acs, aco;
+    int relu;
+    Dims rows;              // dims the rows enumerate (output; coarse input for UP2)
+    Dims in;                // input dims
+    Dims out;               // output dims
+    int64_t m;              // number of rows
+};
+
+// Input voxel of row (n,z,y,x) through tap t; false for zero padding.
+template <int GEOM>
+__device__ inline bool in_voxel(const ConvArgs& a, int n, int z, int y, int x, int t, int64_t& v) {
+    if (GEOM == G_CONV3) {
+        const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
+        if (zz < 0 || zz >= a.in.d || yy < 0 || yy >= a.in.h || xx < 0 || xx >= a.in.w) return false;
+        v = vox(a.in, n, zz, yy, xx);
+    } else if (GEOM == G_DOWN2) {
+        v = vox(a.in, n, 2 * z + (t >> 2), 2 * y + ((t >> 1) & 1), 2 * x + (t & 1));
+    } else {
+        v = vox(a.in, n, z, y, x);
+    }
+    return true;
+}
+
+__device__ inline bf16x8 load8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+__device__ inline bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) z[j] = (bf16)0.0f;
+    return z;
+}
+
+// One wave computes 64 rows x 16*NT output channels; a block is 4 waves
+// (256 rows).  grid = (ceil(m/256), coutp/(16*NT), npar).
+template <int GEOM, int NT>
+__global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int par = blockIdx.z;
+    const int co_base = blockIdx.y * 16 * NT;
+    const int64_t row0 = (int64_t)blockIdx.x * kThreads + wave * 64;
+    const bf16* wt = a.wt + (int64_t)par * a.coutp * a.kpad;
+
+    // rows of this lane in each of the 4 row tiles
+    int rn[4], rz[4], ry[4], rx[4];
+    bool rok[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int64_t r = row0 + mt * 16 + r16;
+        rok[mt] = r < a.m;
+        unvox(a.rows, rok[mt] ? r : 0, rn[mt], rz[mt], ry[mt], rx[mt]);
+    }
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int groups_per_tap = a.cin >> 3;
+    const int ngroups = a.ntaps * groups_per_tap;
+    for (int k0 = 0; k0 < a.kpad; k0 += 32) {
+        const int G = (k0 >> 3) + g;  // this lane's 8-channel group
+        bf16x8 A[4];
+        if (G < ngroups) {
+            const int t = G / groups_per_tap, c0 = (G - t * groups_per_tap) * 8;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                int64_t v;
+                if (rok[mt] && in_voxel<GEOM>(a, rn[mt], rz[mt], ry[mt], rx[mt], t, v))
+                    A[mt] = load8(a.x + v * a.xcs + a.xco + c0);
+                else
+                    A[mt] = zero8();
+            }
+        } else {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) A[mt] = zero8();
+        }
+        bf16x8 B[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) B[nt] = load8(wt + (int64_t)(co_base + nt * 16 + r16) * a.kpad + k0 + g * 8);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B[nt], acc[mt][nt], 0, 0, 0);
+    }
+    // epilogue: lane holds rows 4g..4g+3 of column r16 of each tile
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t r = row0 + mt * 16 + 4 * g + i;
+            if (r >= a.m) continue;
+            int n, z, y, x;
+            unvox(a.rows, r, n, z, y, x);
+            int64_t ov;
+            if (GEOM == G_UP2) ov = vox(a.out, n, 2 * z + (par >> 2), 2 * y + ((par >> 1) & 1), 2 * x + (par & 1));
+            else ov = vox(a.out, n, z, y, x);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = co_base + nt * 16 + r16;
+                if (co >= a.cstore) continue;
+                float v = acc[mt][nt][i];
+                if (a.bias) v += a.bias[co];
+                if (a.relu) v = v > 0.f ? v : 0.f;
+                if (a.aux && !((float)a.aux[ov * a.acs + a.aco + co] > 0.f)) v = 0.f;
+                if (a.yf) a.yf[ov * a.cstore + co] = v;
+                else a.y[ov * a.ycs + a.yco + co] = (bf16)v;
+            }
+        }
+    }
+}
+
+// Weight gradient: dw[par][co][t*cin + ci] += sum_rows dy[out(row)][co] * x[in(row,t)][ci].
+// Block: one tap (blockIdx.y), a 16*NCO x 16*NCI channel tile (blockIdx.z), a
+// chunk of rows (blockIdx.x).  32 rows at a time are staged in LDS transposed
+// (channel-major) so each lane reads 8 consecutive rows with one 16-byte load.
+struct WgradArgs {
+    const bf16* x;
+    int xcs, xco, cin;     // cin = padded input channels (multiple of 8)
+    const bf16* dy;
+    int dcs, dco, coutp;
+    float* dw;             // [npar][coutp][kpad]
+    int kpad, ntaps;
+    Dims rows, in, out;
+    int64_t m;
+    int64_t chunk;         // rows per block
+    int nci_tiles;         // ceil(cin / 64)
+};
+
+template <int GEOM>
+__global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
+    constexpr int TC = 64;  // channel tile (co and ci)
+    __shared__ __attribute__((aligned(16))) bf16 s_dy[TC][40];
+    __shared__ __attribute__((aligned(16))) bf16 s_x[TC][40];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int t = GEOM == G_UP2 ? 0 : blockIdx.y;
+    const int par = GEOM == G_UP2 ? blockIdx.y : 0;
+    const int co_tile = blockIdx.z / a.nci_tiles, ci_tile = blockIdx.z % a.nci_tiles;
+    const int co0 = co_tile * TC, ci0 = ci_tile * TC;
+    const int64_t rbeg = (int64_t)blockIdx.x * a.chunk;
+    const int64_t rend = rbeg + a.chunk < a.m ? rbeg + a.chunk : a.m;
+
+    // 16 sub-tiles (4 co x 4 ci) of 16x16; wave w owns ci sub-tile w for all 4 co
+    f32x4 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    ConvArgs ca;
+    ca.in = a.in;
+    for (int64_t r0 = rbeg; r0 < rend; r0 += 32) {
+        // stage: thread handles row (tid & 31) and 8 channels (tid >> 5) * 8
+        {
+            const int rr = threadIdx.x & 31, cc = (threadIdx.x >> 5) * 8;
+            const int64_t r = r0 + rr;
+            bf16x8 vdy = zero8(), vx = zero8();
+            if (r < rend) {
+                int n, z, y, x;
+                unvox(a.rows, r, n, z, y, x);
+                int64_t ov;
+                if (GEOM == G_UP2) ov = vox(a.out, n, 2 * z + (par >> 2), 2 * y + ((par >> 1) & 1), 2 * x + (par & 1));
+                else ov = vox(a.out, n, z, y, x);
+                if (co0 + cc < a.coutp) vdy = load8(a.dy + ov * a.dcs + a.dco + co0 + cc);
+                int64_t iv;
+                if (ci0 + cc < a.cin && in_voxel<GEOM>(ca, n, z, y, x, t, iv)) vx = load8(a.x + iv * a.xcs + a.xco + ci0 + cc);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                s_dy[cc + j][rr] = vdy[j];
+                s_x[cc + j][rr] = vx[j];
+            }
+        }
+        __syncthreads();
+        const bf16x8 B = load8(&s_x[wave * 16 + r16][g * 8]);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            const bf16x8 A = load8(&s_dy[ct * 16 + r16][g * 8]);
+            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[ct], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // acc[ct]: rows co = co0 + ct*16 + 4g + i, column ci = ci0 + wave*16 + r16
+    const int ci = ci0 + wave * 16 + r16;
+    if (ci >= a.cin) return;
+    float* dw = a.dw + (int64_t)par * a.coutp * a.kpad;
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = co0 + ct * 16 + 4 * g + i;
+            if (co < a.coutp) atomicAdd(&dw[(int64_t)co * a.kpad + t * a.cin + ci], acc[ct][i]);
+        }
+}
+
+// 2x2x2 average pooling: out[v] = mean of the 8 children of v.
+__global__ __launch_bounds__(kThreads) void k_avgpool(const bf16* __restrict__ x, int xcs, int xco, bf16* __restrict__ y,
+                                                       int ycs, int yco, int c, Dims out, Dims in, int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= total) return;
+    const int ch = (int)(i % c);
+    const int64_t v = i / c;
+    int n, z, yy, xx;
+    unvox(out, v, n, z, yy, xx);
+    float s = 0.f;
+    for (int p = 0; p < 8; ++p) {
+        const int64_t iv = vox(in, n, 2 * z + (p >> 2), 2 * yy + ((p >> 1) & 1), 2 * xx + (p & 1));
+        s += (float)x[iv * xcs + xco + ch];
+    }
+    y[v * ycs + yco + ch] = (bf16)(0.125f * s);
+}
+
+// Backward of the pooling, added into dx and masked by (fwd > 0).
+__global__ __launch_bounds__(kThreads) void k_avgpool_bwd(const bf16* __restrict__ dy, int dcs, int dco,
+                                                           bf16* __restrict__ dx, int xcs, int xco,
+                                                           const bf16* __restrict__ fwd, int fcs, int fco, int c,
+                                                           Dims in, Dims out, int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= total) return;
+    const int ch = (int)(i % c);
+    const int64_t v = i / c;  // fine voxel
+    int n, z, yy, xx;
+    unvox(in, v, n, z, yy, xx);
+    const int64_t ov = vox(out, n, z >> 1, yy >> 1, xx >> 1);
+    float g = 0.125f * (float)dy[ov * dcs + dco + ch];
+    if (!((float)fwd[v * fcs + fco + ch] > 0.f)) g = 0.f;
+    bf16* p = dx + v * xcs + xco + ch;
+    *p = (bf16)((float)*p + g);
+}
+
+// FP32 host-layout input [b][c][N] -> BF16 channels-last with cp >= c channels.
+__global__ __launch_bounds__(kThreads) void k_pack_input(const float* __restrict__ src, int c, int cp, int64_t nvox,
+                                                          int64_t batch_vox, bf16* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= batch_vox * cp) return;
+    const int ch = (int)(i % cp);
+    const int64_t v = i / cp;
+    const int64_t b = v / nvox, s = v % nvox;
+    dst[i] = (bf16)(ch < c ? src[(b * c + ch) * nvox + s] : 0.f);
+}
+
+// FP32 channels-last [v][c] -> host layout [b][c][N].
+__global__ __launch_bounds__(kThreads) void k_unpack_output(const float* __restrict__ src, int c, int64_t nvox,
+                                                             int64_t batch_vox, float* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= batch_vox * c) return;
+    const int ch = (int)(i % c);
+    const int64_t v = i / c;
+    const int64_t b = v / nvox, s = v % nvox;
+    dst[(b * c + ch) * nvox + s] = src[i];
+}
+
+// Mean-squared error: dy (BF16, channel stride cp, zero padded) = 2 (y - t) / count;
+// per-block partial sums of (y - t)^2 in FP32 to partial[blockIdx.x].
+__global__ __launch_bounds__(kThreads) void k_mse(const float* __restrict__ y, const float* __restrict__ target, int c,
+                                                   int cp, int64_t nvox, int64_t batch_vox, float inv_count,
+                                                   bf16* __restrict__ dy, float* __restrict__ partial) {
+    __shared__ float s[kThreads];
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    float e2 = 0.f;
+    if (i < batch_vox * cp) {
+        const int ch = (int)(i % cp);
+        const int64_t v = i / cp;
+        float g = 0.f;
+        if (ch < c) {
+            const int64_t b = v / nvox, sidx = v % nvox;
+            const float d = y[v * c + ch] - target[(b * c + ch) * nvox + sidx];
+            e2 = d * d;
+            g = 2.f * d * inv_count;
+        }
+        dy[i] = (bf16)g;
+    }
+    s[threadIdx.x] = e2;
+    __syncthreads();
+    for (int off = kThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// Master layout [npar][cout][taps][cin] (+ bias) <-> padded layouts.
+struct LayerDesc {
+    int geom;      // G_CONV3, G_UP2, G_PW
+    int npar, ntaps;
+    int cin, cinp, cout, coutp;
+    int kpad;      // forward: roundup(ntaps * cinp, 32)
+    int kpad_d;    // input-gradient conv: roundup(ntaps * coutp, 32)
+    int64_t w_off, b_off;
+};
+
+// forward weights wt[par][co][t*cinp + ci] (BF16, zero padded) and bias (FP32, padded)
+__global__ __launch_bounds__(kThreads) void k_pack_fwd(const float* __restrict__ master, LayerDesc L,
+                                                        bf16* __restrict__ wt, float* __restrict__ bias) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t total = (int64_t)L.npar * L.coutp * L.kpad;
+    if (i < L.coutp && bias) bias[i] = i < L.cout ? master[L.b_off + i] : 0.f;
+    if (i >= total) return;
+    const int k = (int)(i % L.kpad);
+    const int64_t r = i / L.kpad;
+    const int co = (int)(r % L.coutp), par = (int)(r / L.coutp);
+    const int t = k / L.cinp, ci = k % L.cinp;
+    float v = 0.f;
+    if (t < L.ntaps && ci < L.cin && co < L.cout)
+        v = master[L.w_off + (((int64_t)par * L.cout + co) * L.ntaps + t) * L.cin + ci];
+    wt[i] = (bf16)v;
+}
+
+// input-gradient weights: the conv from dy (coutp channels) to dx (cinp channels).
+//   conv3: wd[ci][t'*coutp + co] = w[co][26 - t'][ci]
+//   up2:   wd[ci][p*coutp + co]  = w[p][co][ci]      (a stride-2 2x2x2 conv)
+//   pw:    wd[ci][co]            = w[co][ci]
+__global__ __launch_bounds__(kThreads) void k_pack_dgrad(const float* __restrict__ master, LayerDesc L,
+                                                          bf16* __restrict__ wd) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t total = (int64_t)L.cinp * L.kpad_d;
+    if (i >= total) return;
+    const int k = (int)(i % L.kpad_d);
+    const int ci = (int)(i / L.kpad_d);
+    const int tt = k / L.coutp, co = k % L.coutp;
+    const int taps_d = L.geom == G_UP2 ? L.npar : L.ntaps;
+    float v = 0.f;
+    if (tt < taps_d && ci < L.cin && co < L.cout) {
+        if (L.geom == G_CONV3) v = master[L.w_off + ((int64_t)co * 27 + (26 - tt)) * L.cin + ci];
+        else if (L.geom == G_UP2) v = master[L.w_off + ((int64_t)tt * L.cout + co) * L.cin + ci];
+        else v = master[L.w_off + (int64_t)co * L.cin + ci];
+    }
+    wd[i] = (bf16)v;
+}
+
+// padded FP32 weight gradient -> master layout, plus the bias gradient computed
+// separately (k_bias_grad).
+__global__ __launch_bounds__(kThreads) void k_unpack_grad(const float* __restrict__ dwp, LayerDesc L,
+                                                           float* __restrict__ grad) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t total = (int64_t)L.npar * L.cout * L.ntaps * L.cin;
+    if (i >= total) return;
+    const int ci = (int)(i % L.cin);
+    int64_t r = i / L.cin;
+    const int t = (int)(r % L.ntaps);
+    r /= L.ntaps;
+    const int co = (int)(r % L.cout), par = (int)(r / L.cout);
+    grad[L.w_off + i] = dwp[((int64_t)par * L.coutp + co) * L.kpad + t * L.cinp + ci];
+}
+
+// bias gradient: sum over voxels of dy[v][co]; one block per channel, fixed order.
+__global__ __launch_bounds__(kThreads) void k_bias_grad(const bf16* __restrict__ dy, int dcs, int dco, int64_t nvox,
+                                                         int64_t b_off, float* __restrict__ grad) {
+    __shared__ float s[kThreads];
+    const int co = blockIdx.x;
+    float a = 0.f;
+    for (int64_t v = threadIdx.x; v < nvox; v += kThreads) a += (float)dy[v * dcs + dco + co];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = kThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[b_off + co] = s[0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                                    float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
+                                                    float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_bf16(bf16* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = (bf16)0.f;
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_f32(float* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+}  // namespace unet

@@ -59,6 +59,21 @@ def sphere_pack(n_bodies, L, rmin, rmax, seed):
     return [sphere(tuple(c), r) for c, r in out]
 
 
+def force_scale(oracle_solver, rho0: float = 1.0) -> float:
+    """Momentum-exchange scale of the force sums: S = sum over links of 2 w_d rho0,
+    the magnitude of the summands the per-body force cancels down from.  FP64
+    force parity is judged relative to S (a 10^3-term sum cancelling to 1e-2
+    carries ~1e-12 relative rounding against |F| alone)."""
+    from lbm3d_bb_amd.lattice import W
+
+    _, dirs = oracle_solver.links()
+    return float(2.0 * rho0 * W[dirs.astype(np.int64)].sum())
+
+
+def force_err(Fg, Fo, scale: float) -> float:
+    return float(np.abs(np.asarray(Fg) - np.asarray(Fo)).max() / scale)
+
+
 def rel_err(a, b) -> float:
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
@@ -67,4 +82,4 @@ def rel_err(a, b) -> float:
 
 
 __all__ = ["ConstantMotion", "ellipsoid", "rotation_matrix", "sphere", "trt_omegas", "periodic_box", "channel",
-           "reference_channel", "sphere_pack", "rel_err"]
+           "reference_channel", "sphere_pack", "rel_err", "force_scale", "force_err"]

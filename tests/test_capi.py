@@ -21,9 +21,11 @@ def _declared():
 
 
 def test_header_declares_entry_points():
+    from lbm3d_bb_amd.warmstart import UNET_SYMBOLS
+
     names = _declared()
-    assert "lbm_step" in names and "lbm_create" in names
-    assert sorted(names) == sorted(_ffi.exported_symbols())
+    assert "lbm_step" in names and "lbm_create" in names and "lbm_unet_train_step" in names
+    assert sorted(names) == sorted(_ffi.exported_symbols() + UNET_SYMBOLS)
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,9 +1,12 @@
 """HIP path vs the oracle on the same inputs (GPU).
 
-Bars: solid flags and boundary links bit-exact; populations, moments and
-per-body force/torque within a normwise relative tolerance of 1e-12 (FP64
-storage vs the FP64 oracle) or 1e-5 (FP32 storage vs the FP32 oracle);
-normwise = max|gpu - oracle| / max|oracle| over the whole array.
+Bars: solid flags and boundary links bit-exact; populations and moments
+within a normwise relative tolerance of 1e-12 (FP64 storage vs the FP64
+oracle) or 1e-5 (FP32 vs the FP32 oracle), normwise = max|gpu - oracle| /
+max|oracle| over the whole array; per-body force/torque within the same
+tolerance relative to the momentum-exchange scale S = sum_links 2 w_d rho0
+(the force is a sum of ~10^3 link terms of size ~w_d that cancel by orders
+of magnitude, so relative to |F| alone FP64 rounding reaches ~1e-12).
 """
 from __future__ import annotations
 
@@ -14,7 +17,7 @@ from lbm3d_bb_amd.bodies import ConstantMotion, ellipsoid, rotation_matrix, sphe
 from lbm3d_bb_amd.solver import Solver
 from oracle.oracle import OracleSolver
 
-from cases import channel, periodic_box, reference_channel, rel_err, sphere_pack, trt_omegas
+from cases import channel, force_err, force_scale, periodic_box, reference_channel, rel_err, sphere_pack, trt_omegas
 
 pytestmark = pytest.mark.gpu
 
@@ -84,7 +87,7 @@ def test_periodic_sphere_fp64(collision):
     Fg = g.step(60)
     Fo = o.step(60)
     assert np.abs(Fo).max() > 0
-    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
     _compare_state(g, o, TOL["fp64"], collision)
 
 
@@ -97,7 +100,7 @@ def test_multi_body_static_fp64():
         s.init_equilibrium()
     Fg = g.step(40)
     Fo = o.step(40)
-    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
     _compare_state(g, o, TOL["fp64"], "multi")
 
 
@@ -111,7 +114,7 @@ def test_channel_translating_sphere_fp64():
         s.init_equilibrium()
     Fg = g.step(50)
     Fo = o.step(50)
-    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
     _compare_state(g, o, TOL["fp64"], "channel")
     assert np.array_equal(g.flags(), o.flags())
 
@@ -125,7 +128,7 @@ def test_channel_translating_sphere_fp32():
         s.init_equilibrium()
     Fg = g.step(30)
     Fo = o.step(30)
-    assert rel_err(Fg, Fo) < TOL["fp32"] * 10, rel_err(Fg, Fo)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp32"], force_err(Fg, Fo, force_scale(o))
     rg, ug = g.macros()
     ro, uo = o.macros()
     assert rel_err(rg, ro) < TOL["fp32"]
@@ -143,7 +146,7 @@ def test_reference_face_set_fp64():
         s.init_equilibrium()
     Fg = g.step(40)
     Fo = o.step(40)
-    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
     _compare_state(g, o, TOL["fp64"], "refchannel")
 
 
@@ -158,7 +161,7 @@ def test_rotating_ellipsoid_refill_fp64():
     Fg = g.step(30)
     Fo = o.step(30)
     assert np.abs(Fo[:, :, 3:]).max() > 0
-    assert rel_err(Fg, Fo) < TOL["fp64"], rel_err(Fg, Fo)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
     _compare_state(g, o, TOL["fp64"], "ellipsoid")
     assert np.array_equal(g.flags(), o.flags())
 

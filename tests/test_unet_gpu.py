@@ -1,0 +1,77 @@
+"""Warm-start U-Net kernels (GPU) vs torch on the CPU.
+
+The forward output and the gradient of the mean-squared error w.r.t. every
+parameter come from this library's BF16 MFMA kernels; the checker is
+torch.nn.functional (conv3d, conv_transpose3d, avg_pool3d) with autograd in
+FP64 on the same BF16-rounded weights and activations.  Tolerances are
+normwise relative errors: 2e-2 for outputs (BF16 activations, FP32
+accumulation in a different order), 5e-2 per layer for gradients (the device
+also rounds every back-propagated activation gradient to BF16).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _net(shape=(16, 16, 16), batch=2, levels=3, base=16, seed=3):
+    from lbm3d_bb_amd.warmstart import UNet
+
+    return UNet(shape, batch=batch, levels=levels, base=base, seed=seed)
+
+
+def _data(shape, batch, seed=0):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((batch, 2) + shape, dtype=np.float32)
+    x[:, 0] = (rng.random((batch,) + shape) < 0.3)
+    x[:, 1] = rng.random((batch, 1, 1, 1))
+    t = rng.standard_normal((batch, 4) + shape).astype(np.float32) * 0.1
+    return x, t
+
+
+@pytest.mark.parametrize("levels,shape", [(2, (8, 8, 16)), (3, (16, 16, 16)), (4, (16, 16, 24))])
+def test_forward_matches_torch(levels, shape):
+    import torch
+
+    from unet_reference import forward
+
+    net = _net(shape, batch=2, levels=levels)
+    x, _ = _data(shape, 2)
+    y = net.forward(x)
+    p = torch.tensor(net.get_params(), dtype=torch.float64)
+    yr = forward(p, net.layers(), torch.tensor(x, dtype=torch.float64), levels).numpy()
+    assert np.abs(yr).max() > 1e-3
+    assert _rel(y, yr) < 2e-2, _rel(y, yr)
+
+
+def test_gradient_matches_torch_autograd():
+    from unet_reference import loss_and_grad
+
+    shape, levels = (16, 16, 16), 3
+    net = _net(shape, batch=2, levels=levels)
+    x, t = _data(shape, 2, seed=5)
+    loss, g = net.gradient(x, t)
+    lr, gr, _ = loss_and_grad(net.get_params(), net.layers(), x, t, levels)
+    assert abs(loss - lr) / lr < 1e-2
+    for i, L in enumerate(net.layers()):
+        nw = (8 if L.kind == 1 else 1) * L.cout * (27 if L.kind == 0 else 1) * L.cin
+        gw, gwr = g[L.w_off:L.w_off + nw], gr[L.w_off:L.w_off + nw]
+        gb, gbr = g[L.b_off:L.b_off + L.cout], gr[L.b_off:L.b_off + L.cout]
+        assert _rel(gw, gwr) < 5e-2, (i, L, _rel(gw, gwr))
+        assert _rel(gb, gbr) < 5e-2, (i, L, _rel(gb, gbr))
+
+
+def test_training_reduces_loss():
+    shape = (16, 16, 16)
+    net = _net(shape, batch=2, levels=3)
+    x, t = _data(shape, 2, seed=9)
+    first = net.train_step(x, t)
+    for _ in range(60):
+        last = net.train_step(x, t)
+    assert last < 0.5 * first, (first, last)
