@@ -19,7 +19,8 @@ using namespace lbm;
 
 namespace {
 
-constexpr int kChunk = 64;  // steps whose poses are uploaded with one copy
+constexpr int kChunk = 64;          // steps whose poses are uploaded with one copy
+constexpr int kProfileStride = 16;  // lbm_profile times every 16th step
 
 }  // namespace
 
@@ -29,7 +30,7 @@ struct lbm_ctx {
     Phys<double> pd{};
     Phys<float> pf{};
     bool forced = false;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // k_bulk, copies, getters
     int device = 0;
 
     void* fbuf[2] = {nullptr, nullptr};
@@ -316,16 +317,17 @@ int launch_bulk(lbm_ctx* c, int mode, const void* fin, void* fout, double* gout,
     return do_launch(c, r);
 }
 
-int launch_fixup(lbm_ctx* c, int mode, int grid_blocks, const void* fin, void* fout, double* gout, double* rho,
-                 double* u, const char* rec, int write_flags, double* partials) {
-    if (grid_blocks <= 0) return LBM_OK;
-    LaunchReq r = base_req(c, K_FIXUP, mode);
+// One time step (or its pull-only replay for the getters): nfix fix-up blocks
+// followed by the bulk blocks, in a single launch.
+int launch_step(lbm_ctx* c, int mode, int nfix, const void* fin, void* fout, double* gout, double* rho, double* u,
+                const char* rec, int write_flags, double* partials) {
+    LaunchReq r = base_req(c, K_STEP, mode);
     r.fin = fin;
     r.fout = fout;
     r.gout = gout;
     r.rho_out = rho;
     r.u_out = u;
-    r.grid_blocks = grid_blocks;
+    r.grid_blocks = nfix;
     r.rec = rec;
     r.write_flags = write_flags;
     r.partials = partials;
@@ -414,6 +416,9 @@ int build_list(lbm_ctx* c, int32_t* d_link_cell, int8_t* d_link_dir, int64_t* nl
     }
     GEOCHK(c, launch_bb_fill(c->flags, c->g, c->cnt_a, c->cnt_b, c->list, d_link_cell, d_link_dir, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->list_count, c->cnt_a + nblk, 4, hipMemcpyDeviceToDevice, c->stream));
+    // the listed nodes carry the fix-up bit (k_step's bulk blocks skip them)
+    GEOCHK(c, launch_mark_list(c->flags, c->list, nn, 0, c->g.N, c->stream));
+    GEOCHK(c, launch_mark_list(c->flags, c->list, nn, 1, c->g.N, c->stream));
     HIPCHK(c, hipGetLastError());
     c->list_n = nn;
     c->nlinks = tot[1];
@@ -536,7 +541,7 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
         bn[c->nb + i] = bold[i];
     }
     *grid = sg.nfix_blocks + sg.list_blocks;
-    *face_grid = sg.nface_blocks;
+    *face_grid = sg.mode;  // the caller needs the step's mode (k_faces no longer exists)
     return LBM_OK;
 }
 
@@ -574,24 +579,20 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
         const char* rec = c->d_rec + (size_t)s * c->rec_bytes;
         void* fin = c->fbuf[c->cur];
         void* fout = c->fbuf[1 - c->cur];
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-        if (c->profiling) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool sample = c->profiling && ((c->step + s) % kProfileStride == 0);
+        if (sample) {
             e0 = next_event(c);
             e1 = next_event(c);
-            e2 = next_event(c);
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        int lr = launch_bulk(c, MODE_STEP, fin, fout, nullptr, nullptr, nullptr, rec);
+        int lr = launch_step(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, 1,
+                             c->d_partials + (size_t)s * stride);
         if (lr) return lr;
-        if (c->profiling) HIPCHK(c, hipEventRecord(e1, c->stream));
-        lr = launch_fixup(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, 1,
-                          c->d_partials + (size_t)s * stride);
-        if (lr) return lr;
-        if (c->profiling) HIPCHK(c, hipEventRecord(e2, c->stream));
+        if (sample) HIPCHK(c, hipEventRecord(e1, c->stream));
         c->cur = 1 - c->cur;
         c->last_rec = rec;
         c->last_grid = grids[s];
-        c->last_face_grid = fgrids[s];
         c->last_rec_valid = true;
         c->raw_state = false;
     }
@@ -621,10 +622,7 @@ int get_state(lbm_ctx* c, double* d_f, double* d_rho, double* d_u) {
     const void* prev = c->fbuf[1 - c->cur];
     if (c->raw_state) return launch_bulk(c, MODE_GET_RAW, prev, nullptr, d_f, d_rho, d_u, nullptr);
     if (!c->last_rec_valid) return fail(c, LBM_ERR_STATE, "no completed step to read");
-    int rc = launch_bulk(c, MODE_GET, prev, nullptr, d_f, d_rho, d_u, c->last_rec);
-    if (rc == LBM_OK)
-        rc = launch_fixup(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0, nullptr);
-    return rc;
+    return launch_step(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0, nullptr);
 }
 
 }  // namespace
@@ -852,6 +850,7 @@ int lbm_get_flags(lbm_ctx* c, uint8_t* flags) {
     if (!c || !flags) return LBM_ERR_ARG;
     HIPCHK(c, hipMemcpyAsync(flags, c->flags, (size_t)c->g.N, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < c->g.N; ++i) flags[i] &= kFlagBody;
     return LBM_OK;
 }
 
@@ -963,25 +962,21 @@ int lbm_step(lbm_ctx* c, int32_t nsteps, double* force_out) {
             chunk_off[par] = done_total;
             chunk_cnt[par] = (int)done;
             done_total += done;
-            if (c->profiling) {
-                // per-kernel event pairs of this chunk: read after it drained
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-                for (size_t i = 0; i + 3 <= c->ev_used; i += 3) {
-                    float a = 0.f, b = 0.f;
-                    HIPCHK(c, hipEventElapsedTime(&a, c->ev_pool[i], c->ev_pool[i + 1]));
-                    HIPCHK(c, hipEventElapsedTime(&b, c->ev_pool[i + 1], c->ev_pool[i + 2]));
-                    c->bulk_ms += a;
-                    ++c->bulk_launches;
-                    c->link_ms += b;
-                    ++c->link_launches;
-                }
-                c->ev_used = 0;
-            }
         }
         return (int)LBM_OK;
     }();
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->profiling) {
+        // sampled event pairs around the step kernel
+        for (size_t i = 0; i + 2 <= c->ev_used; i += 2) {
+            float a = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&a, c->ev_pool[i], c->ev_pool[i + 1]));
+            c->bulk_ms += a;
+            ++c->bulk_launches;
+        }
+        c->ev_used = 0;
+    }
     for (int par = 0; par < 2; ++par) {
         if (chunk_off[par] >= 0 && force_out && n6 > 0)
             std::memcpy(force_out + (size_t)chunk_off[par] * n6, c->h_forces[par],

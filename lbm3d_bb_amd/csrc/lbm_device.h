@@ -235,7 +235,7 @@ struct SolidView {
 };
 
 __device__ inline int solid_at(const SolidView& sv, int x, int y, int z, const Grid& g) {
-    if (sv.mode == 0) return sv.flags[(int64_t)x + (int64_t)g.nx * y + g.nxy * z];
+    if (sv.mode == 0) return sv.flags[(int64_t)x + (int64_t)g.nx * y + g.nxy * z] & kFlagBody;
     return flag_of(sv.bnew, sv.nb, x, y, z, g);
 }
 
@@ -389,14 +389,12 @@ __device__ __forceinline__ void face_cell(const typename S::T* __restrict__ fin,
 // directions inside the row's own coalesced loads.
 // ---------------------------------------------------------------------------
 template <class S, int COLL, bool FORCED, int MODE>
-__global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict__ fin,
-                                                 typename S::T* __restrict__ fout, double* __restrict__ gout,
-                                                 double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
-                                                 Phys<typename S::C> p, const StepGeom* __restrict__ sgp) {
+__device__ __forceinline__ void bulk_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                          double* __restrict__ gout, double* __restrict__ rho_out,
+                                          double* __restrict__ u_out, const Grid& g, const Phys<typename S::C>& p,
+                                          const StepGeom* __restrict__ sgp, const uint8_t* __restrict__ flags, int x,
+                                          int y, int z) {
     using C = typename S::C;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z;
     if (x >= g.nx || y >= g.ny) return;
     const int nx = g.nx, nxy = (int)g.nxy;
     const int64_t N = g.N;
@@ -415,7 +413,24 @@ __global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict
     const int nyz = (ylo || yhi) + (zlo || zhi);
     const bool xf = xlo || xhi;
     if (nyz + xf >= 2) return;  // domain edge
-    if ((nyz + xf) == 1 && sgp && in_body_box(*sgp, x, y, z, g)) return;
+    if (sgp && sgp->mode == 1) {
+        // moving bodies: the fix-up blocks own their whole regions
+        bool row = false;
+        for (int m = sgp->first_fix_box; m < sgp->nbox && !row; ++m) {
+            const Box& b = sgp->box[m];
+            if (b.kind != 1) continue;
+            int ry = y - b.lo[1], rz = z - b.lo[2];
+            if (g.per & 2) ry = ((ry % g.ny) + g.ny) % g.ny;
+            if (g.per & 4) rz = ((rz % g.nz) + g.nz) % g.nz;
+            row = ry >= 0 && ry < b.ext[1] && rz >= 0 && rz < b.ext[2];
+        }
+        if (row && in_body_box(*sgp, x, y, z, g)) return;
+    } else if (sgp && sgp->nbodies > 0) {
+        // static bodies: boundary nodes carry the fix-up bit, face cells of a
+        // body region belong to the fix-up too
+        if (flags[cell] & kFlagFixup) return;
+        if ((nyz + xf) == 1 && in_body_box(*sgp, x, y, z, g)) return;
+    }
     if (nyz == 1) {
         if (ylo) face_cell<S, COLL, FORCED, MODE, 1, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
         else if (yhi) face_cell<S, COLL, FORCED, MODE, 1, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
@@ -462,6 +477,17 @@ __global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict
     finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
 }
 
+template <class S, int COLL, bool FORCED, int MODE>
+__global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict__ fin,
+                                                 typename S::T* __restrict__ fout, double* __restrict__ gout,
+                                                 double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
+                                                 Phys<typename S::C> p, const StepGeom* __restrict__ sgp,
+                                                 const uint8_t* __restrict__ flags) {
+    bulk_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, flags,
+                                     blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+                                     blockIdx.z);
+}
+
 // ---------------------------------------------------------------------------
 // Force accumulation: per thread one body in registers (a fluid node next to a
 // second body accumulates that one through LDS atomics), per wave a
@@ -496,6 +522,37 @@ __device__ inline void wave_reduce_to_lds(int key, const double vals[6], double*
 }
 
 // ---------------------------------------------------------------------------
+// Plain cell off every non-periodic face: the k_bulk pull, for cells of a
+// moving-body region that are neither boundary nor fresh nodes.
+// ---------------------------------------------------------------------------
+template <class S, int COLL, bool FORCED, int MODE>
+__device__ __forceinline__ void plain_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                           double* __restrict__ gout, double* __restrict__ rho_out,
+                                           double* __restrict__ u_out, const Grid& g, const Phys<typename S::C>& p,
+                                           int x, int y, int z) {
+    using C = typename S::C;
+    const int nx = g.nx, nxy = (int)g.nxy;
+    const int64_t N = g.N;
+    const int cell = x + nx * y + nxy * z;
+    const int wxm = (x == 0) ? nx : 0, wxp = (x == nx - 1) ? -nx : 0;
+    const int wym = (y == 0) ? nxy : 0, wyp = (y == g.ny - 1) ? -nxy : 0;
+    const int wzm = (z == 0) ? (int)N : 0, wzp = (z == g.nz - 1) ? -(int)N : 0;
+    C f[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        int src = cell - (kCx[d] + kCy[d] * nx + kCz[d] * nxy);
+        if (kCx[d] > 0) src += wxm;
+        if (kCx[d] < 0) src += wxp;
+        if (kCy[d] > 0) src += wym;
+        if (kCy[d] < 0) src += wyp;
+        if (kCz[d] > 0) src += wzm;
+        if (kCz[d] < 0) src += wzp;
+        f[d] = S::load(fin + (int64_t)d * g.P + src, d);
+    }
+    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+}
+
+// ---------------------------------------------------------------------------
 // The general cell: faces (any number, with precedence), bounce-back links,
 // refill, momentum exchange.  Every direction is classified first, then all
 // loads are issued together, so the cell costs one memory round trip.
@@ -520,32 +577,43 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
     for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
     C f[Q];
     if (fresh) {
-        // equilibrium of the mean previous-step rho, u of fluid neighbours
+        // equilibrium of the mean previous-step rho, u of fluid neighbours; all
+        // neighbour loads are unrolled (unused ones read the cell itself)
         C sr = C(0), su = C(0), sv_ = C(0), sw = C(0);
         int cnt = 0;
+#pragma unroll
         for (int d = 0; d < 18; ++d) {
             int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
             bool ok = true;
-            for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
-            if (!ok) continue;
-            if (solid_at(sv, q[0], q[1], q[2], g) != 0 || fresh_at(sv, q[0], q[1], q[2], g)) continue;
-            const int64_t nc = (int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2];
-            C nf[Q];
 #pragma unroll
-            for (int e = 0; e < Q; ++e) nf[e] = S::load(fin + (int64_t)e * g.P + nc, e);
-            C r, mx, my, mz;
-            moments<C>(nf, r, mx, my, mz);
+            for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
+            const bool use = ok && solid_at(sv, q[0], q[1], q[2], g) == 0 && !fresh_at(sv, q[0], q[1], q[2], g);
+            const int64_t nc = use ? (int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2] : cell;
+            C r = C(0), mx = C(0), my = C(0), mz = C(0);
+#pragma unroll
+            for (int e = 0; e < Q; ++e) {
+                const C v = S::load(fin + (int64_t)e * g.P + nc, e);
+                r += v;
+                if (kCx[e] > 0) mx += v;
+                if (kCx[e] < 0) mx -= v;
+                if (kCy[e] > 0) my += v;
+                if (kCy[e] < 0) my -= v;
+                if (kCz[e] > 0) mz += v;
+                if (kCz[e] < 0) mz -= v;
+            }
             C ux = mx / r, uy = my / r, uz = mz / r;
             if (FORCED) {
                 ux -= C(0.5) * p.g[0];
                 uy -= C(0.5) * p.g[1];
                 uz -= C(0.5) * p.g[2];
             }
-            sr += r;
-            su += ux;
-            sv_ += uy;
-            sw += uz;
-            ++cnt;
+            if (use) {
+                sr += r;
+                su += ux;
+                sv_ += uy;
+                sw += uz;
+                ++cnt;
+            }
         }
         C r = C(g.rho0), ux = C(0), uy = C(0), uz = C(0);
         if (cnt > 0) {
@@ -679,24 +747,23 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
 // k_fixup: the general path.  Blocks [0, sg.nfix_blocks) walk the boxes of
 // kind 2 (domain edges: every cell) and kind 1 (body regions); blocks
 // [sg.nfix_blocks, + sg.list_blocks) walk the static boundary-node list.
-//  * body region, moving bodies (mode 1): the thread rewrites its cell's flag,
-//    then takes face cells, boundary nodes and fresh nodes;
+//  * body region, moving bodies (mode 1): every cell (k_bulk skips the
+//    region, so the two kernels write disjoint cells and run concurrently);
+//    the thread also rewrites its cell's flag;
 //  * body region, static bodies (mode 0): face cells only;
 //  * list: boundary nodes not on a face.
 // Cells in several regions go to the first.  Every block writes nbodies*6
 // partial force/torque sums (fixed order) to partials[blockIdx].
 // ---------------------------------------------------------------------------
 template <class S, int COLL, bool FORCED, int MODE>
-__global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restrict__ fin,
-                                                  typename S::T* __restrict__ fout, double* __restrict__ gout,
-                                                  double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
-                                                  Phys<typename S::C> p, const StepGeom* __restrict__ sgp,
-                                                  const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
-                                                  uint8_t* __restrict__ flags, const int32_t* __restrict__ list,
-                                                  const int32_t* __restrict__ list_count, double* __restrict__ partials,
-                                                  int write_flags) {
-    __shared__ double s_acc[kWaves][LBM_MAX_BODIES * 6];
-    __shared__ double s_over[LBM_MAX_BODIES * 6];
+__device__ __forceinline__ void fixup_block(const int blk, const typename S::T* __restrict__ fin,
+                                            typename S::T* __restrict__ fout, double* __restrict__ gout,
+                                            double* __restrict__ rho_out, double* __restrict__ u_out, const Grid& g,
+                                            const Phys<typename S::C>& p, const StepGeom* __restrict__ sgp,
+                                            const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
+                                            uint8_t* __restrict__ flags, const int32_t* __restrict__ list,
+                                            const int32_t* __restrict__ list_count, double* __restrict__ partials,
+                                            int write_flags, double (*s_acc)[LBM_MAX_BODIES * 6], double* s_over) {
     const StepGeom& sg = *sgp;
     const int nb = sg.nbodies;
     const int tid = threadIdx.x;
@@ -717,8 +784,7 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restric
     acc.b0 = -1;
     for (int c = 0; c < 6; ++c) acc.a0[c] = 0.0;
 
-    const int blk = blockIdx.x;
-    bool work = false;
+    bool work = false, plain = false;
     int x = 0, y = 0, z = 0;
     if (blk < sg.nfix_blocks) {
         int bi = sg.first_fix_box;
@@ -745,27 +811,31 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restric
             if (!dup && sv.mode == 1 && write_flags) {
                 const int fl = flag_of(bnew, nb, x, y, z, g);
                 const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
-                if (flags[cell] != (uint8_t)fl) flags[cell] = (uint8_t)fl;
+                if ((flags[cell] & kFlagBody) != fl) flags[cell] = (uint8_t)fl;
             }
             if (b.kind == 2) {
                 work = !dup;  // domain edge: every cell
             } else if (!dup) {
-                const bool face = open_face_count(x, y, z, g) == 1;  // edges went to the edge boxes
+                // moving bodies: the bulk pass skipped the whole region, so every
+                // cell is done here (boundary, fresh and face cells on the general
+                // path, the rest on the plain one); static bodies: only the
+                // region's face cells (the list has the rest)
+                const bool face = open_face_count(x, y, z, g) == 1;
                 if (sv.mode == 1) {
-                    const int fl = flag_of(bnew, nb, x, y, z, g);
-                    if (face) {
-                        work = true;
-                    } else if (fl == 0) {
-                        // fluid: work if any neighbour is solid or the node is fresh
-                        bool nearb = g.refill && fresh_at(sv, x, y, z, g);
-                        for (int d = 0; d < 18 && !nearb; ++d) {
-                            int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
+                    work = true;
+                    if (!face) {
+                        bool gen = false;
+                        if (flag_of(bnew, nb, x, y, z, g) == 0) {
+                            gen = g.refill && fresh_at(sv, x, y, z, g);
                             const int L[3] = {g.nx, g.ny, g.nz};
-                            bool ok = true;
-                            for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
-                            if (ok && solid_at(sv, q[0], q[1], q[2], g) != 0) nearb = true;
+                            for (int d = 0; d < 18 && !gen; ++d) {
+                                int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
+                                bool ok = true;
+                                for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
+                                if (ok && solid_at(sv, q[0], q[1], q[2], g) != 0) gen = true;
+                            }
                         }
-                        work = nearb;
+                        plain = !gen;
                     }
                 } else {
                     work = face;
@@ -783,8 +853,10 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restric
             work = open_face_count(x, y, z, g) == 0;  // face cells belong to the regions
         }
     }
-    if (work) {
-        if (MODE == MODE_STEP || MODE == MODE_GET)
+    if (work && (MODE == MODE_STEP || MODE == MODE_GET)) {
+        if (plain)
+            plain_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+        else
             general_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sv, x, y, z, acc, s_over);
     }
     if (MODE == MODE_STEP && nb > 0) {
@@ -797,6 +869,45 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const typename S::T* __restric
             partials[(int64_t)blk * nb * 6 + i] = s;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// k_step: one launch per time step.  Blocks [0, nfix) are the fix-up blocks
+// (edges, body regions, static boundary nodes) and are dispatched first; the
+// remaining blocks are the bulk pass over rows of blockDim-sized x runs.  The
+// two sets write disjoint cells, so they run side by side with no barrier.
+// ---------------------------------------------------------------------------
+struct BulkShape {
+    int bx, by;   // cells per block along x and rows per block
+    int gx, gy;   // blocks along x and y
+};
+
+template <class S, int COLL, bool FORCED, int MODE>
+__global__ __launch_bounds__(kBlock) void k_step(const typename S::T* __restrict__ fin,
+                                                 typename S::T* __restrict__ fout, double* __restrict__ gout,
+                                                 double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
+                                                 Phys<typename S::C> p, const StepGeom* __restrict__ sgp,
+                                                 const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
+                                                 uint8_t* __restrict__ flags, const int32_t* __restrict__ list,
+                                                 const int32_t* __restrict__ list_count,
+                                                 double* __restrict__ partials, int write_flags, BulkShape bs,
+                                                 int nfix) {
+    __shared__ double s_acc[kWaves][LBM_MAX_BODIES * 6];
+    __shared__ double s_over[LBM_MAX_BODIES * 6];
+    const int blk = blockIdx.x;
+    if (blk < nfix) {
+        fixup_block<S, COLL, FORCED, MODE>(blk, fin, fout, gout, rho_out, u_out, g, p, sgp, bnew, bold, flags, list,
+                                           list_count, partials, write_flags, s_acc, s_over);
+        return;
+    }
+    const int b = blk - nfix;
+    const int bxi = b % bs.gx;
+    const int r = b / bs.gx;
+    const int byi = r % bs.gy;
+    const int bz = r / bs.gy;
+    const int tx = threadIdx.x % bs.bx, ty = threadIdx.x / bs.bx;
+    bulk_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, flags, bxi * bs.bx + tx,
+                                     byi * bs.by + ty, bz);
 }
 
 // ---------------------------------------------------------------------------

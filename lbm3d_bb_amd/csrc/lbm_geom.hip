@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_flags_full(uint8_t* __restrict__ fla
 }
 
 __device__ inline bool is_bb_node(const uint8_t* __restrict__ flags, int64_t cell, const Grid& g) {
-    if (flags[cell] != 0) return false;
+    if ((flags[cell] & kFlagBody) != 0) return false;
     int x, y, z;
     cell_xyz(cell, g, x, y, z);
     const int L[3] = {g.nx, g.ny, g.nz};
@@ -52,13 +52,13 @@ __device__ inline bool is_bb_node(const uint8_t* __restrict__ flags, int64_t cel
         int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
         bool ok = true;
         for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
-        if (ok && flags[(int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2]] != 0) return true;
+        if (ok && (flags[(int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2]] & kFlagBody) != 0) return true;
     }
     return false;
 }
 
 __device__ inline int link_count(const uint8_t* __restrict__ flags, int64_t cell, const Grid& g) {
-    if (flags[cell] != 0) return 0;
+    if ((flags[cell] & kFlagBody) != 0) return 0;
     int x, y, z;
     cell_xyz(cell, g, x, y, z);
     const int L[3] = {g.nx, g.ny, g.nz};
@@ -67,7 +67,7 @@ __device__ inline int link_count(const uint8_t* __restrict__ flags, int64_t cell
         int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
         bool ok = true;
         for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
-        if (ok && flags[(int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2]] != 0) ++n;
+        if (ok && (flags[(int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2]] & kFlagBody) != 0) ++n;
     }
     return n;
 }
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_bb_fill(const uint8_t* __restrict__ 
                 int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
                 bool ok = true;
                 for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
-                if (ok && flags[(int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2]] != 0) {
+                if (ok && (flags[(int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2]] & kFlagBody) != 0) {
                     link_cell[pos] = (int32_t)cell;
                     link_dir[pos] = (int8_t)d;
                     ++pos;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_conservation(const double* __restric
     __shared__ double s[kWaves][4];
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double v[4] = {0, 0, 0, 0};
-    if (cell < N && flags[cell] == 0) {
+    if (cell < N && (flags[cell] & kFlagBody) == 0) {
         const double r = rho[cell];
         v[0] = r;
         v[1] = r * u[cell];
@@ -214,6 +214,17 @@ __global__ __launch_bounds__(kBlock) void k_conservation(const double* __restric
     }
 }
 
+
+// Sets (set = 1) the fix-up bit on the listed cells, or clears it everywhere (set = 0).
+__global__ __launch_bounds__(kBlock) void k_mark_list(uint8_t* __restrict__ flags, const int32_t* __restrict__ list,
+                                                      int64_t n, int set, int64_t N) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (set) {
+        if (i < n) flags[list[i]] |= kFlagFixup;
+    } else if (i < N) {
+        flags[i] &= kFlagBody;
+    }
+}
 
 #define LAUNCH_CHECK()                          \
     do {                                        \
@@ -248,6 +259,13 @@ int launch_force_finalize(const double* partials, int64_t step_stride, const cha
                           int nsteps, int n6, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_force_finalize, dim3((unsigned)n6, (unsigned)nsteps), dim3(kBlock), 0, st,
                        partials, step_stride, recs, rec_bytes, n6, out);
+    LAUNCH_CHECK();
+}
+
+int launch_mark_list(uint8_t* flags, const int32_t* list, int64_t n, int set, int64_t N, hipStream_t st) {
+    const int64_t m = set ? n : N;
+    if (m <= 0) return 0;
+    hipLaunchKernelGGL(k_mark_list, dim3((unsigned)nblocks_of(m)), dim3(kBlock), 0, st, flags, list, n, set, N);
     LAUNCH_CHECK();
 }
 

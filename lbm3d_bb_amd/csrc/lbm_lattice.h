@@ -42,7 +42,8 @@ struct StoreF32 {
     using C = float;
     static constexpr int kBytes = 4;
     __device__ static inline C load(const T* p, int d) { (void)d; return *p; }
-    __device__ static inline void store(T* p, int d, C v) { (void)d; *p = v; }
+    // populations are written once per step and read by the next: stream them
+    __device__ static inline void store(T* p, int d, C v) { (void)d; __builtin_nontemporal_store(v, p); }
 };
 
 struct StoreF16S {
@@ -52,7 +53,10 @@ struct StoreF16S {
     __device__ static inline C load(const T* p, int d) { return __half2float(*p) + weight<float>(d); }
     // __float2half_rn: round to nearest even (not the round-toward-zero of the
     // packed v_cvt_pkrtz form, which would bias the deviations).
-    __device__ static inline void store(T* p, int d, C v) { *p = __float2half_rn(v - weight<float>(d)); }
+    __device__ static inline void store(T* p, int d, C v) {
+        const __half h = __float2half_rn(v - weight<float>(d));
+        __builtin_nontemporal_store(*reinterpret_cast<const unsigned short*>(&h), reinterpret_cast<unsigned short*>(p));
+    }
 };
 
 struct StoreBF16S {

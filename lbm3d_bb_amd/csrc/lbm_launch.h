@@ -6,25 +6,35 @@
 
 namespace lbm {
 
-template <class S, int MODE, int COLL, bool FORCED>
-static void launch_bulk_t(const LaunchReq& r, const Phys<typename S::C>& p) {
-    using T = typename S::T;
-    const int bx = r.g.nx >= 256 ? 256 : (r.g.nx >= 128 ? 128 : 64);
-    const int by = kBlock / bx;
-    const dim3 grid((unsigned)((r.g.nx + bx - 1) / bx), (unsigned)((r.g.ny + by - 1) / by), (unsigned)r.g.nz);
-    hipLaunchKernelGGL((k_bulk<S, COLL, FORCED, MODE>), grid, dim3(bx, by), 0, r.stream,
-                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, (const StepGeom*)r.rec);
+inline BulkShape bulk_shape(const Grid& g) {
+    BulkShape bs;
+    bs.bx = g.nx >= 256 ? 256 : (g.nx >= 128 ? 128 : 64);
+    bs.by = kBlock / bs.bx;
+    bs.gx = (g.nx + bs.bx - 1) / bs.bx;
+    bs.gy = (g.ny + bs.by - 1) / bs.by;
+    return bs;
 }
 
 template <class S, int MODE, int COLL, bool FORCED>
-static void launch_fixup_t(const LaunchReq& r, const Phys<typename S::C>& p) {
+static void launch_bulk_t(const LaunchReq& r, const Phys<typename S::C>& p) {
+    using T = typename S::T;
+    const BulkShape bs = bulk_shape(r.g);
+    hipLaunchKernelGGL((k_bulk<S, COLL, FORCED, MODE>), dim3(bs.gx, bs.gy, r.g.nz), dim3(bs.bx, bs.by), 0, r.stream,
+                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, (const StepGeom*)r.rec,
+                       (const uint8_t*)r.flags);
+}
+
+template <class S, int MODE, int COLL, bool FORCED>
+static void launch_step_t(const LaunchReq& r, const Phys<typename S::C>& p) {
     using T = typename S::T;
     const StepGeom* sg = (const StepGeom*)r.rec;
     const DevBody* bn = (const DevBody*)(r.rec + sizeof(StepGeom));
     const DevBody* bo = bn + r.nb;
-    hipLaunchKernelGGL((k_fixup<S, COLL, FORCED, MODE>), dim3((unsigned)r.grid_blocks), dim3(kBlock), 0, r.stream,
-                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, sg, bn, bo, r.flags, r.list,
-                       r.list_count, r.partials, r.write_flags);
+    const BulkShape bs = bulk_shape(r.g);
+    const int64_t nbulk = (int64_t)bs.gx * bs.gy * r.g.nz;
+    hipLaunchKernelGGL((k_step<S, COLL, FORCED, MODE>), dim3((unsigned)(r.grid_blocks + nbulk)), dim3(kBlock), 0,
+                       r.stream, (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, sg, bn, bo, r.flags,
+                       r.list, r.list_count, r.partials, r.write_flags, bs, r.grid_blocks);
 }
 
 template <class S, int MODE>
@@ -34,9 +44,8 @@ static void launch_mode(const LaunchReq& r, const Phys<typename S::C>& p) {
         if (trt) r.forced ? launch_bulk_t<S, MODE, LBM_TRT, true>(r, p) : launch_bulk_t<S, MODE, LBM_TRT, false>(r, p);
         else r.forced ? launch_bulk_t<S, MODE, LBM_BGK, true>(r, p) : launch_bulk_t<S, MODE, LBM_BGK, false>(r, p);
     } else if (MODE == MODE_STEP || MODE == MODE_GET) {
-        if (r.grid_blocks <= 0) return;
-        if (trt) r.forced ? launch_fixup_t<S, MODE, LBM_TRT, true>(r, p) : launch_fixup_t<S, MODE, LBM_TRT, false>(r, p);
-        else r.forced ? launch_fixup_t<S, MODE, LBM_BGK, true>(r, p) : launch_fixup_t<S, MODE, LBM_BGK, false>(r, p);
+        if (trt) r.forced ? launch_step_t<S, MODE, LBM_TRT, true>(r, p) : launch_step_t<S, MODE, LBM_TRT, false>(r, p);
+        else r.forced ? launch_step_t<S, MODE, LBM_BGK, true>(r, p) : launch_step_t<S, MODE, LBM_BGK, false>(r, p);
     }
 }
 
@@ -51,7 +60,7 @@ int launch(const LaunchReq& r) {
     const dim3 grid((unsigned)nblocks_of(r.g.N)), block(kBlock);
     switch (r.kind) {
         case K_BULK:
-        case K_FIXUP:
+        case K_STEP:
             switch (r.mode) {
                 case MODE_STEP: launch_mode<S, MODE_STEP>(r, p); break;
                 case MODE_GET: launch_mode<S, MODE_GET>(r, p); break;

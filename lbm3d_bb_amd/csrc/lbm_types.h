@@ -12,6 +12,11 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxBoxes = 6 + 12 + LBM_MAX_BODIES;
 
+// Flag byte: low 7 bits 0 (fluid) or k+1 (solid of body k); bit 7 marks a fluid
+// boundary node of static geometry, finished by the fix-up blocks.
+constexpr uint8_t kFlagBody = 0x7f;
+constexpr uint8_t kFlagFixup = 0x80;
+
 // Body as the kernels see it: FP32 mask pose (inside test) and FP64 half-step
 // kinematics (moving-wall velocity and torque arm).
 struct DevBody {
@@ -75,7 +80,7 @@ enum { MODE_STEP = 0, MODE_GET = 1, MODE_COLLIDE_ONLY = 2, MODE_GET_RAW = 3 };
 
 // One kernel launch request; the per-storage translation units implement
 // launch<S>() for it, so the host code never instantiates kernels itself.
-enum { K_BULK = 0, K_FIXUP = 1, K_EQUILIBRIUM = 2, K_FROM_DOUBLE = 3, K_FACES = 4 };
+enum { K_BULK = 0, K_STEP = 1, K_EQUILIBRIUM = 2, K_FROM_DOUBLE = 3 };
 
 struct LaunchReq {
     int kind;
@@ -123,6 +128,7 @@ int launch_bb_fill(const uint8_t* flags, const Grid& g, const int32_t* node_off,
                    int32_t* nodes, int32_t* link_cell, int8_t* link_dir, hipStream_t st);
 int launch_force_finalize(const double* partials, int64_t step_stride, const char* recs, size_t rec_bytes,
                           int nsteps, int n6, double* out, hipStream_t st);
+int launch_mark_list(uint8_t* flags, const int32_t* list, int64_t n, int set, int64_t N, hipStream_t st);
 int launch_conservation(const double* rho, const double* u, const uint8_t* flags, int64_t N, double* partials,
                         hipStream_t st);
 
