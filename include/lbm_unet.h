@@ -60,6 +60,8 @@ int lbm_unet_train_step(lbm_unet* net, const float* x, const float* target, int3
  * (no update), for gradient checks. */
 int lbm_unet_gradient(lbm_unet* net, const float* x, const float* target, int32_t batch, float* loss,
                       float* grad);
+/* Adam learning rate for the following train steps (a schedule's knob). */
+int lbm_unet_set_lr(lbm_unet* net, float lr);
 /* Timing of the last train step / forward (ms, device events) and its FLOP count. */
 int lbm_unet_last_timing(lbm_unet* net, double* ms, double* flops);
 

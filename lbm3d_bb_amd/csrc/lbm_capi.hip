@@ -20,6 +20,9 @@ using namespace lbm;
 namespace {
 
 constexpr int kChunk = 64;          // steps whose poses are uploaded with one copy
+// Bytes in front of the first population plane: the bulk pass's shifted pair
+// loads read one element before a row start (the value is replaced for x = 0).
+constexpr size_t kFrontPad = 256;
 constexpr int kProfileStride = 16;  // lbm_profile times every 16th step
 
 }  // namespace
@@ -33,9 +36,12 @@ struct lbm_ctx {
     hipStream_t stream = nullptr;   // k_bulk, copies, getters
     int device = 0;
 
-    void* fbuf[2] = {nullptr, nullptr};
+    void* fbuf[2] = {nullptr, nullptr};      // population planes (fbuf_raw + kFrontPad)
+    void* fbuf_raw[2] = {nullptr, nullptr};  // allocations
+    void* side[2] = {nullptr, nullptr};      // face moments of fbuf[i]'s f* (Grid::face_base slots)
+    size_t side_bytes = 0;
     int cur = 0;               // buffer holding f* (pending stream)
-    size_t fbytes = 0;
+    size_t fbytes = 0;         // 19 planes of P elements
     uint8_t* flags = nullptr;
 
     // static boundary-node list (solidity from flags)
@@ -79,6 +85,7 @@ struct lbm_ctx {
 
     // last step (for getters)
     const char* last_rec = nullptr;
+    BulkGeom last_bg{};
     int last_grid = 0;
     int last_face_grid = 0;
     bool last_rec_valid = false;
@@ -255,8 +262,23 @@ void add_edge_boxes(const lbm_ctx* c, StepGeom& sg) {
         }
 }
 
+// Whether two boxes share a cell (wrapping on periodic axes).
+bool boxes_meet(const lbm_ctx* c, const Box& a, const Box& b) {
+    const int L[3] = {c->g.nx, c->g.ny, c->g.nz};
+    for (int ax = 0; ax < 3; ++ax) {
+        if (a.ext[ax] <= 0 || b.ext[ax] <= 0) return false;
+        if ((c->g.per >> ax) & 1) {
+            const int r = (((b.lo[ax] - a.lo[ax]) % L[ax]) + L[ax]) % L[ax];
+            if (!(r < a.ext[ax] || L[ax] - r < b.ext[ax])) return false;
+        } else if (!(a.lo[ax] < b.lo[ax] + b.ext[ax] && b.lo[ax] < a.lo[ax] + a.ext[ax])) {
+            return false;
+        }
+    }
+    return true;
+}
+
 // Block numbering: face slabs (k_faces) and edge/body boxes (k_fixup) each from 0.
-void finish_boxes(StepGeom& sg) {
+void finish_boxes(const lbm_ctx* c, StepGeom& sg) {
     int fblocks = 0, xblocks = 0;
     sg.first_fix_box = sg.nbox;
     for (int i = 0; i < sg.nbox; ++i) {
@@ -272,6 +294,11 @@ void finish_boxes(StepGeom& sg) {
     }
     sg.nface_blocks = fblocks;
     sg.nfix_blocks = xblocks;
+    for (int i = sg.first_fix_box; i < sg.nbox; ++i) {
+        sg.box[i].ovl = 0;
+        for (int m = sg.first_fix_box; m < i && !sg.box[i].ovl; ++m)
+            if (boxes_meet(c, sg.box[m], sg.box[i])) sg.box[i].ovl = 1;
+    }
 }
 
 LaunchReq base_req(const lbm_ctx* c, int kind, int mode) {
@@ -292,7 +319,17 @@ LaunchReq base_req(const lbm_ctx* c, int kind, int mode) {
     return r;
 }
 
-int do_launch(lbm_ctx* c, const LaunchReq& r) {
+// the face-moment buffer that belongs to population buffer p
+void* side_of(const lbm_ctx* c, const void* p) {
+    if (p == c->fbuf[0]) return c->side[0];
+    if (p == c->fbuf[1]) return c->side[1];
+    return nullptr;
+}
+
+int do_launch(lbm_ctx* c, const LaunchReq& r0) {
+    LaunchReq r = r0;
+    r.g.side_in = side_of(c, r.fin);
+    r.g.side_out = side_of(c, r.fout);
     int e = 0;
     switch (c->cfg.storage) {
         case LBM_FP64: e = launch<StoreF64>(r); break;
@@ -319,9 +356,33 @@ int launch_bulk(lbm_ctx* c, int mode, const void* fin, void* fout, double* gout,
 
 // One time step (or its pull-only replay for the getters): nfix fix-up blocks
 // followed by the bulk blocks, in a single launch.
+// The bulk rows' view of a step record (built from the host copy).
+BulkGeom bulk_geom_of(const StepGeom& sg) {
+    BulkGeom bg;
+    std::memset(&bg, 0, sizeof(bg));
+    bg.mode = sg.mode;
+    bg.nbodies = sg.nbodies;
+    int n = 0;
+    for (int i = 0; i < sg.nbox; ++i) {
+        if (sg.box[i].kind != 1) continue;
+        if (n == kBulkBoxes) {
+            n = -1;
+            break;
+        }
+        for (int a = 0; a < 3; ++a) {
+            bg.lo[a][n] = sg.box[i].lo[a];
+            bg.ext[a][n] = sg.box[i].ext[a];
+        }
+        ++n;
+    }
+    bg.nbox = n;
+    return bg;
+}
+
 int launch_step(lbm_ctx* c, int mode, int nfix, const void* fin, void* fout, double* gout, double* rho, double* u,
-                const char* rec, int write_flags, double* partials) {
+                const char* rec, const BulkGeom& bg, int write_flags, double* partials) {
     LaunchReq r = base_req(c, K_STEP, mode);
+    r.bg = bg;
     r.fin = fin;
     r.fout = fout;
     r.gout = gout;
@@ -505,7 +566,7 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
             }
             add_body_box(c, sg, lo_f, hi_f);
         }
-        finish_boxes(sg);
+        finish_boxes(c, sg);
         sg.list_blocks = 0;
         c->mask_bodies = bnew;
         c->list_valid = false;
@@ -529,7 +590,7 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
                 add_body_box(c, sg, lo_f, hi_f);
             }
         }
-        finish_boxes(sg);
+        finish_boxes(c, sg);
         sg.list_blocks = c->nb > 0 ? (int)nblocks_of(c->list_n) : 0;
         bold = bnew;
     }
@@ -586,12 +647,14 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
             e1 = next_event(c);
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        int lr = launch_step(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, 1,
+        const BulkGeom bg = bulk_geom_of(*(const StepGeom*)(c->h_rec[par] + (size_t)s * c->rec_bytes));
+        int lr = launch_step(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, bg, 1,
                              c->d_partials + (size_t)s * stride);
         if (lr) return lr;
         if (sample) HIPCHK(c, hipEventRecord(e1, c->stream));
         c->cur = 1 - c->cur;
         c->last_rec = rec;
+        c->last_bg = bg;
         c->last_grid = grids[s];
         c->last_rec_valid = true;
         c->raw_state = false;
@@ -622,7 +685,7 @@ int get_state(lbm_ctx* c, double* d_f, double* d_rho, double* d_u) {
     const void* prev = c->fbuf[1 - c->cur];
     if (c->raw_state) return launch_bulk(c, MODE_GET_RAW, prev, nullptr, d_f, d_rho, d_u, nullptr);
     if (!c->last_rec_valid) return fail(c, LBM_ERR_STATE, "no completed step to read");
-    return launch_step(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0, nullptr);
+    return launch_step(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, c->last_bg, 0, nullptr);
 }
 
 }  // namespace
@@ -643,6 +706,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     }
     const int64_t N = (int64_t)cfg->nx * cfg->ny * cfg->nz;
     if (N >= ((int64_t)1 << 31)) return LBM_ERR_ARG;  // cell indices are int32 in the lists
+    // 16-bit storage: a bulk thread owns an aligned pair of x cells
+    if ((cfg->storage == LBM_FP16S || cfg->storage == LBM_BF16S) && (cfg->nx % 2) != 0) return LBM_ERR_ARG;
     if (!(cfg->omega_plus > 0.0 && cfg->omega_plus < 2.0)) return LBM_ERR_ARG;
     if (cfg->collision == LBM_TRT && !(cfg->omega_minus > 0.0 && cfg->omega_minus < 2.0)) return LBM_ERR_ARG;
 
@@ -659,6 +724,22 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     for (int f = 0; f < 6; ++f) {
         c->g.face_kind[f] = cfg->faces[f].kind;
         for (int k = 0; k < 3; ++k) c->g.face_u[f][k] = cfg->faces[f].u[k];
+    }
+    {
+        // face-moment slots of the velocity / anti-bounce-back faces
+        int64_t slots = 0;
+        const int64_t area[3] = {(int64_t)cfg->ny * cfg->nz, (int64_t)cfg->nx * cfg->nz, (int64_t)cfg->nx * cfg->ny};
+        for (int f = 0; f < 6; ++f) {
+            const int k = cfg->faces[f].kind;
+            if (k == LBM_FACE_VELOCITY || k == LBM_FACE_PRESSURE_ABB) {
+                c->g.face_base[f] = (int)slots;
+                slots += area[f / 2];
+            } else {
+                c->g.face_base[f] = -1;
+            }
+        }
+        const int cbytes = cfg->storage == LBM_FP64 ? 8 : 4;
+        c->side_bytes = (size_t)std::max<int64_t>(slots, 1) * 4 * cbytes;
     }
     c->g.wall_point = cfg->wall_point;
     c->g.refill = cfg->refill;
@@ -684,7 +765,14 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         return LBM_ERR_HIP;
     }
     int rc = LBM_OK;
-    for (int i = 0; i < 2 && rc == LBM_OK; ++i) rc = dev_alloc(c, &c->fbuf[i], c->fbytes);
+    for (int i = 0; i < 2 && rc == LBM_OK; ++i) {
+        rc = dev_alloc(c, &c->fbuf_raw[i], c->fbytes + kFrontPad);
+        if (rc == LBM_OK) c->fbuf[i] = (char*)c->fbuf_raw[i] + kFrontPad;
+    }
+    for (int i = 0; i < 2 && rc == LBM_OK; ++i) {
+        rc = dev_alloc(c, &c->side[i], c->side_bytes);
+        if (rc == LBM_OK && hipMemsetAsync(c->side[i], 0, c->side_bytes, c->stream) != hipSuccess) rc = LBM_ERR_HIP;
+    }
     if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->flags, (size_t)N);
     if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->list_count, 16);
     if (rc == LBM_OK) {
@@ -708,7 +796,8 @@ int lbm_destroy(lbm_ctx* c) {
     if (!c) return LBM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 2; ++i) {
-        if (c->fbuf[i]) (void)hipFree(c->fbuf[i]);
+        if (c->fbuf_raw[i]) (void)hipFree(c->fbuf_raw[i]);
+        if (c->side[i]) (void)hipFree(c->side[i]);
         if (c->h_rec[i]) (void)hipHostFree(c->h_rec[i]);
         if (c->h_forces[i]) (void)hipHostFree(c->h_forces[i]);
         if (c->rec_ev[i]) (void)hipEventDestroy(c->rec_ev[i]);

@@ -258,12 +258,78 @@ __device__ inline bool in_box(const Box& b, int x, int y, int z, const Grid& g) 
     return true;
 }
 
+// in_box for a box whose lo lies in [0, L) on periodic axes (as the host
+// builds them): one conditional wrap instead of a remainder
+__device__ inline bool in_box_normalised(const Box& b, int x, int y, int z, const Grid& g) {
+    const int q[3] = {x, y, z};
+    const int L[3] = {g.nx, g.ny, g.nz};
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int r = q[a] - b.lo[a];
+        if (r < 0 && ((g.per >> a) & 1)) r += L[a];
+        in = in && r >= 0 && r < b.ext[a];
+    }
+    return in;
+}
+
 // Body regions (box kind 1) cover every cell that can be a boundary node, a
 // fresh node or a face cell next to a body, in both geometry modes.
 __device__ inline bool in_body_box(const StepGeom& sg, int x, int y, int z, const Grid& g) {
     for (int m = 0; m < sg.nbox; ++m)
         if (sg.box[m].kind == 1 && in_box(sg.box[m], x, y, z, g)) return true;
     return false;
+}
+
+// Body-region tests of the bulk rows from the by-value BulkGeom (one
+// conditional wrap: lo is normalised into [0, L) on periodic axes).
+__device__ inline bool in_bulk_box(const BulkGeom& bg, int m, int x, int y, int z, const Grid& g) {
+    const int q[3] = {x, y, z};
+    const int L[3] = {g.nx, g.ny, g.nz};
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int r = q[a] - bg.lo[a][m];
+        if (r < 0 && ((g.per >> a) & 1)) r += L[a];
+        in = in && r >= 0 && r < bg.ext[a][m];
+    }
+    return in;
+}
+
+__device__ inline bool row_in_bulk_box(const BulkGeom& bg, int m, int y, int z, const Grid& g) {
+    int ry = y - bg.lo[1][m], rz = z - bg.lo[2][m];
+    if (ry < 0 && (g.per & 2)) ry += g.ny;
+    if (rz < 0 && (g.per & 4)) rz += g.nz;
+    return ry >= 0 && ry < bg.ext[1][m] && rz >= 0 && rz < bg.ext[2][m];
+}
+
+__device__ inline bool cell_in_body_region(const BulkGeom& bg, const StepGeom* sgp, int x, int y, int z,
+                                           const Grid& g) {
+    if (bg.nbox < 0) return in_body_box(*sgp, x, y, z, g);
+    bool hit = false;
+#pragma unroll
+    for (int m = 0; m < kBulkBoxes; ++m)
+        if (m < bg.nbox) hit = hit || in_bulk_box(bg, m, x, y, z, g);
+    return hit;
+}
+
+__device__ inline bool row_in_body_region(const BulkGeom& bg, const StepGeom* sgp, int y, int z, const Grid& g) {
+    if (bg.nbox < 0) {
+        for (int m = sgp->first_fix_box; m < sgp->nbox; ++m) {
+            const Box& b = sgp->box[m];
+            if (b.kind != 1) continue;
+            int ry = y - b.lo[1], rz = z - b.lo[2];
+            if (g.per & 2) ry = ((ry % g.ny) + g.ny) % g.ny;
+            if (g.per & 4) rz = ((rz % g.nz) + g.nz) % g.nz;
+            if (ry >= 0 && ry < b.ext[1] && rz >= 0 && rz < b.ext[2]) return true;
+        }
+        return false;
+    }
+    bool hit = false;
+#pragma unroll
+    for (int m = 0; m < kBulkBoxes; ++m)
+        if (m < bg.nbox) hit = hit || row_in_bulk_box(bg, m, y, z, g);
+    return hit;
 }
 
 // number of non-periodic faces the cell lies on
@@ -327,6 +393,71 @@ __device__ inline void own_moments(const typename S::C own[Q], const Phys<typena
 }
 
 // ---------------------------------------------------------------------------
+// Face moments.  A velocity or anti-bounce-back face rule needs the moments of
+// the cell's own f* (src/MainCore.jl:316,336: the previous step's rho, u).
+// The step that produces a face cell's f* also stores them in its face slot,
+// so the next step reads one slot instead of the cell's 19 populations.
+// ---------------------------------------------------------------------------
+__device__ inline int face_slot(const Grid& g, int f, int x, int y, int z) {
+    const int a = f >> 1;
+    const int i = a == 0 ? y + g.ny * z : (a == 1 ? x + g.nx * z : x + g.nx * y);
+    return g.face_base[f] + i;
+}
+
+__device__ inline bool rule_face(const Grid& g, int f) { return g.face_base[f] >= 0; }
+
+template <typename C>
+__device__ inline void side_read(const Grid& g, int slot, C& rho, C up[3], C& uu2) {
+    const C* s = reinterpret_cast<const C*>(g.side_in) + 4 * (int64_t)slot;
+    rho = s[0];
+    up[0] = s[1];
+    up[1] = s[2];
+    up[2] = s[3];
+    uu2 = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
+}
+
+// moments of a finished (post-collision) f* into face slot `slot`
+template <class S, bool FORCED>
+__device__ inline void side_write(const Grid& g, const typename S::C f[Q], const Phys<typename S::C>& p, int slot) {
+    using C = typename S::C;
+    C rho, up[3], uu2;
+    own_moments<S, FORCED>(f, p, rho, up, uu2);
+    C* s = reinterpret_cast<C*>(g.side_out) + 4 * (int64_t)slot;
+    s[0] = rho;
+    s[1] = up[0];
+    s[2] = up[1];
+    s[3] = up[2];
+}
+
+// every face slot of cell (x, y, z): cells on several rule faces (edges,
+// corners) fill each
+template <class S, bool FORCED>
+__device__ inline void side_write_all(const Grid& g, const typename S::C f[Q], const Phys<typename S::C>& p, int x,
+                                      int y, int z) {
+    const int q[3] = {x, y, z};
+    const int L[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; ++a) {
+        if (q[a] == 0 && rule_face(g, 2 * a)) side_write<S, FORCED>(g, f, p, face_slot(g, 2 * a, x, y, z));
+        if (q[a] == L[a] - 1 && rule_face(g, 2 * a + 1))
+            side_write<S, FORCED>(g, f, p, face_slot(g, 2 * a + 1, x, y, z));
+    }
+}
+
+// finish_cell plus the face slots of the cell (slot >= 0: that one; -1: every
+// rule face the cell lies on)
+template <class S, int COLL, bool FORCED, int MODE>
+__device__ inline void finish_face_cell(typename S::C f[Q], const Phys<typename S::C>& p, int64_t cell, const Grid& g,
+                                        typename S::T* __restrict__ fout, double* __restrict__ gout,
+                                        double* __restrict__ rho_out, double* __restrict__ u_out, int x, int y, int z,
+                                        int slot) {
+    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, g.N, g.P, fout, gout, rho_out, u_out);
+    if (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY) {
+        if (slot >= 0) side_write<S, FORCED>(g, f, p, slot);
+        else side_write_all<S, FORCED>(g, f, p, x, y, z);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Face cell on exactly one non-periodic y or z face (AXIS, SIDE): directions
 // whose source lies outside the face take the face rule (or zero gradient);
 // all others are plain pulls with periodic wrap on the remaining axes.
@@ -345,9 +476,12 @@ __device__ __forceinline__ void face_cell(const typename S::T* __restrict__ fin,
     const int wym = (AXIS != 1 && y == 0) ? nxy : 0, wyp = (AXIS != 1 && y == g.ny - 1) ? -nxy : 0;
     const int wzm = (AXIS != 2 && z == 0) ? (int)N : 0, wzp = (AXIS != 2 && z == g.nz - 1) ? -(int)N : 0;
     const int axis_stride = AXIS == 1 ? nx : nxy;
-    C own[Q], f[Q];
-#pragma unroll
-    for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
+    const bool zg = kind == LBM_FACE_ZERO_GRADIENT;
+    // previous-step moments of the cell, from its face slot (face rules only)
+    const int slot = zg ? -1 : face_slot(g, fidx, x, y, z);
+    C rho_prev, up[3], uu2;
+    side_read<C>(g, zg ? 0 : slot, rho_prev, up, uu2);  // unconditional: slot 0 always exists
+    C f[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         const int ca = AXIS == 1 ? kCy[d] : kCz[d];
@@ -359,122 +493,274 @@ __device__ __forceinline__ void face_cell(const typename S::T* __restrict__ fin,
         if (kCy[d] < 0) src += wyp;
         if (kCz[d] > 0) src += wzm;
         if (kCz[d] < 0) src += wzp;
-        // zero gradient: the source clamped back onto the face plane
-        if (out) src = kind == LBM_FACE_ZERO_GRADIENT ? src + ca * axis_stride : cell;
-        f[d] = S::load(fin + (int64_t)d * g.P + src, d);
-    }
-    if (kind != LBM_FACE_ZERO_GRADIENT) {
-        C rho_prev, up[3], uu2;
-        own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            const int ca = AXIS == 1 ? kCy[d] : kCz[d];
-            const bool out = SIDE == 0 ? (ca > 0) : (ca < 0);
-            if (out) f[d] = face_value<C>(kind, fidx, d, g, own[kOpp[d]], rho_prev, up, uu2);
+        if (out) {
+            // zero gradient: the source clamped back onto the face plane; the
+            // other rules start from the cell's own f*[opp(d)].  One load
+            // either way (an address select, no branch around the load).
+            const typename S::T* a = fin + (int64_t)d * g.P + (src + ca * axis_stride);
+            const typename S::T* b = fin + (int64_t)kOpp[d] * g.P + cell;
+            const C v = S::load(zg ? a : b, zg ? d : kOpp[d]);
+            const C fv = face_value<C>(kind, fidx, d, g, v, rho_prev, up, uu2);
+            f[d] = zg ? v : fv;
+        } else {
+            f[d] = S::load(fin + (int64_t)d * g.P + src, d);
         }
     }
-    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+    finish_face_cell<S, COLL, FORCED, MODE>(f, p, cell, g, fout, gout, rho_out, u_out, x, y, z, slot);
 }
 
 // ---------------------------------------------------------------------------
-// k_bulk: a block is a run of blockDim.x consecutive x cells (up to 256, a
-// whole row when Nx <= 256) in blockDim.y rows, so every wave streams 64
-// consecutive cells of one row in each direction plane, the +-1 shifted loads
-// of the x neighbours stay inside the block's own cache lines, and no thread
-// divides.  It finishes every cell except
-//   * domain edges (cells on two or more non-periodic faces), and
-//   * face cells inside a body region,
-// which k_fixup does.  A row on a y or z face takes the face path as a whole
-// wave; a cell on an x face takes the face rule for its five outgoing
-// directions inside the row's own coalesced loads.
+// Bulk pass.  A thread owns V = S::kVec consecutive x cells of one row; a
+// block is blockDim.x such groups along x in blockDim.y rows, so every wave
+// streams 64 V consecutive cells of one row in each direction plane and the
+// +-1 shifted pulls stay inside the block's own cache lines.  Every direction
+// is one unconditional load (periodic wraps and zero-gradient clamps are
+// address or value selects): a load under a divergent branch would make the
+// compiler wait for the loads already in flight.  It finishes every cell
+// except
+//   * domain edges (cells on two or more non-periodic faces) and x-face cells
+//     of a velocity or anti-bounce-back face,
+//   * cells the fix-up blocks own: static boundary nodes (flag bit), face
+//     cells inside a body region, and every cell of a moving body's region,
+// which the fix-up blocks do.  A row on a y or z face takes the face path as
+// a whole wave.
 // ---------------------------------------------------------------------------
-template <class S, int COLL, bool FORCED, int MODE>
-__device__ __forceinline__ void bulk_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                          double* __restrict__ gout, double* __restrict__ rho_out,
-                                          double* __restrict__ u_out, const Grid& g, const Phys<typename S::C>& p,
-                                          const StepGeom* __restrict__ sgp, const uint8_t* __restrict__ flags, int x,
-                                          int y, int z) {
+
+// collide every active cell of a group and store it (one vector store per
+// direction when the whole group is active)
+template <class S, int COLL, bool FORCED, int MODE, int V>
+__device__ inline void finish_group(typename S::C (&f)[V][Q], const bool (&act)[V], const Phys<typename S::C>& p,
+                                    int cell0, const Grid& g, typename S::T* __restrict__ fout,
+                                    double* __restrict__ gout, double* __restrict__ rho_out,
+                                    double* __restrict__ u_out) {
     using C = typename S::C;
-    if (x >= g.nx || y >= g.ny) return;
-    const int nx = g.nx, nxy = (int)g.nxy;
-    const int64_t N = g.N;
-    const int cell = x + nx * y + nxy * z;
-    C f[Q];
-    if (MODE == MODE_COLLIDE_ONLY || MODE == MODE_GET_RAW) {
+    if (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY) {
 #pragma unroll
-        for (int d = 0; d < Q; ++d) f[d] = S::load(fin + (int64_t)d * g.P + cell, d);
-        finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+        for (int v = 0; v < V; ++v) {
+            C rho, u[3];
+            collide<C, COLL, FORCED>(f[v], p, rho, u);
+        }
+        bool all = true;
+#pragma unroll
+        for (int v = 0; v < V; ++v) all = all && act[v];
+        if (V == 2 && all) {
+#pragma unroll
+            for (int d = 0; d < Q; ++d) S::store2(fout + (int64_t)d * g.P + cell0, d, f[0][d], f[V - 1][d]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (act[v]) {
+#pragma unroll
+                    for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * g.P + cell0 + v, d, f[v][d]);
+                }
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (act[v]) finish_cell<S, COLL, FORCED, MODE>(f[v], p, cell0 + v, g.N, g.P, fout, gout, rho_out, u_out);
+    }
+}
+
+// x treatment of an interior row, a wave-uniform choice between branch-free
+// copies of the pull: periodic; zero-gradient faces only (clamps are value
+// selects); at least one velocity / anti-bounce-back face (its rule starts
+// from the face cell's own f*[opp(d)], one extra load per direction pair
+// that every other lane points at an element it already reads).
+enum { XM_PERIODIC = 0, XM_CLAMP = 1, XM_RULE = 2 };
+
+// Pull + collide of one group on a row off the y and z faces.
+template <class S, int COLL, bool FORCED, int MODE, int XM>
+__device__ __forceinline__ void interior_row(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                             double* __restrict__ gout, double* __restrict__ rho_out,
+                                             double* __restrict__ u_out, const Grid& g,
+                                             const Phys<typename S::C>& p, const bool (&act_in)[S::kVec],
+                                             const uint8_t* fixup_flags, int x0, int y, int z) {
+    using C = typename S::C;
+    constexpr int V = S::kVec;
+    constexpr bool PX = XM == XM_PERIODIC;
+    // fix-up bits of static boundary nodes, loaded together with the populations
+    uint32_t fl = 0;
+    if (fixup_flags) fl = V == 2 ? *reinterpret_cast<const uint16_t*>(fixup_flags) : *fixup_flags;
+    const int nx = g.nx, nxy = (int)g.nxy;
+    const int row = nx * y + nxy * z;
+    const bool left = x0 == 0, right = x0 + V == nx;
+    // lanes applying a face rule at x = 0 (rl) or x = nx - 1 (rr)
+    const bool rl = XM == XM_RULE && left && rule_face(g, 0);
+    const bool rr = XM == XM_RULE && right && rule_face(g, 1);
+    // periodic y/z wraps by index arithmetic (zero on non-periodic axes)
+    const int wym = (y == 0) ? nxy : 0, wyp = (y == g.ny - 1) ? -nxy : 0;
+    const int wzm = (z == 0) ? (int)g.N : 0, wzp = (z == g.nz - 1) ? -(int)g.N : 0;
+    // x: the pull source of x = 0 (c_x > 0) or x = nx - 1 (c_x < 0) wraps
+    // (periodic) or is clamped onto the face plane (zero gradient; a rule
+    // face replaces the value)
+    const int wxm = left ? (PX ? nx : 1) : 0, wxp = right ? (PX ? -nx : -1) : 0;
+    C rho_prev = C(0), up[3] = {C(0), C(0), C(0)}, uu2 = C(0);
+    int slot = -1;
+    if (XM == XM_RULE) {
+        // every lane reads a valid slot; only rl / rr lanes use it
+        slot = rule_face(g, 0) ? face_slot(g, 0, 0, y, z) : face_slot(g, 1, nx - 1, y, z);
+        if (rr && rule_face(g, 0)) slot = face_slot(g, 1, nx - 1, y, z);
+        side_read<C>(g, slot, rho_prev, up, uu2);
+    }
+    C f[V][Q];
+    C ex[5];  // own f*[opp(d)] of a rule lane, per c_x > 0 direction pair
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        int srow = row - (kCy[d] * nx + kCz[d] * nxy);
+        if (kCy[d] > 0) srow += wym;
+        if (kCy[d] < 0) srow += wyp;
+        if (kCz[d] > 0) srow += wzm;
+        if (kCz[d] < 0) srow += wzp;
+        const typename S::T* plane = fin + (int64_t)d * g.P + srow;
+        if (V == 1) {
+            int sx = x0 - kCx[d];
+            if (kCx[d] > 0) sx += wxm;
+            if (kCx[d] < 0) sx += wxp;
+            f[0][d] = S::load(plane + sx, d);
+        } else if (kCx[d] == 0) {
+            S::load2(plane + x0, d, f[0][d], f[V - 1][d]);
+        } else if (kCx[d] > 0) {
+            // pair (x0 - 1, x0); x0 - 1 = -1 on the left group
+            C a, b;
+            S::load2(plane + x0 - 1, d, a, b);
+            if (PX) a = S::load(plane + x0 - 1 + wxm, d);  // the same element except on the left group
+            else a = left ? b : a;                         // zero gradient: x = 0 itself
+            f[0][d] = a;
+            f[V - 1][d] = b;
+        } else {
+            // pair (x0 + 1, x0 + 2); x0 + 2 = nx on the right group
+            C a, b;
+            S::load2(plane + x0 + 1, d, a, b);
+            if (PX) b = S::load(plane + x0 + 2 + wxp, d);
+            else b = right ? a : b;
+            f[0][d] = a;
+            f[V - 1][d] = b;
+        }
+        if (XM == XM_RULE && kCx[d] > 0) {
+            // d leaves x = 0 inward (rule at the left face reads f*[opp d] at x = 0);
+            // opp(d) leaves x = nx - 1 (rule at the right face reads f*[d] there)
+            const typename S::T* own_l = fin + (int64_t)kOpp[d] * g.P + row;
+            const typename S::T* own_r = fin + (int64_t)d * g.P + row + nx - 1;
+            const typename S::T* dflt = plane + x0 - kCx[d];
+            const int k = (d == 0) ? 0 : (d == 6 ? 1 : (d == 8 ? 2 : (d == 12 ? 3 : 4)));
+            ex[k] = S::load(rl ? own_l : (rr ? own_r : dflt), rl ? kOpp[d] : d);
+        }
+    }
+    if (XM == XM_RULE) {
+        // ALU-only branches: the loads above are already issued
+        if (rl) {
+            const int kind = g.face_kind[0];
+#pragma unroll
+            for (int d = 0; d < Q; ++d) {
+                if (kCx[d] <= 0) continue;
+                const int k = (d == 0) ? 0 : (d == 6 ? 1 : (d == 8 ? 2 : (d == 12 ? 3 : 4)));
+                f[0][d] = face_value<C>(kind, 0, d, g, ex[k], rho_prev, up, uu2);
+            }
+        }
+        if (rr) {
+            const int kind = g.face_kind[1];
+#pragma unroll
+            for (int d = 0; d < Q; ++d) {
+                if (kCx[d] >= 0) continue;
+                const int k = (d == 1) ? 0 : (d == 7 ? 1 : (d == 9 ? 2 : (d == 13 ? 3 : 4)));
+                f[V - 1][d] = face_value<C>(kind, 1, d, g, ex[k], rho_prev, up, uu2);
+            }
+        }
+    }
+    bool act[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) act[v] = act_in[v] && !((fl >> (8 * v)) & kFlagFixup);
+    finish_group<S, COLL, FORCED, MODE, V>(f, act, p, row + x0, g, fout, gout, rho_out, u_out);
+    if (XM == XM_RULE && (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY)) {
+        if (rl && act[0]) side_write<S, FORCED>(g, f[0], p, face_slot(g, 0, 0, y, z));
+        if (rr && act[V - 1]) side_write<S, FORCED>(g, f[V - 1], p, face_slot(g, 1, nx - 1, y, z));
+    }
+}
+
+template <class S, int COLL, bool FORCED, int MODE>
+__device__ __forceinline__ void bulk_group(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                           double* __restrict__ gout, double* __restrict__ rho_out,
+                                           double* __restrict__ u_out, const Grid& g, const Phys<typename S::C>& p,
+                                           const StepGeom* __restrict__ sgp, const BulkGeom& bg,
+                                           const uint8_t* __restrict__ flags, int x0, int y, int z) {
+    using C = typename S::C;
+    constexpr int V = S::kVec;  // V == 2 requires an even nx (checked at context creation)
+    if (x0 >= g.nx || y >= g.ny) return;
+    const int nx = g.nx, nxy = (int)g.nxy;
+    const int row = nx * y + nxy * z;
+    const int cell0 = row + x0;
+    bool act[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) act[v] = true;
+    if (MODE == MODE_COLLIDE_ONLY || MODE == MODE_GET_RAW) {
+        C f[V][Q];
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            if (V == 2) S::load2(fin + (int64_t)d * g.P + cell0, d, f[0][d], f[V - 1][d]);
+            else f[0][d] = S::load(fin + (int64_t)d * g.P + cell0, d);
+        }
+        finish_group<S, COLL, FORCED, MODE, V>(f, act, p, cell0, g, fout, gout, rho_out, u_out);
+        if (MODE == MODE_COLLIDE_ONLY) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) side_write_all<S, FORCED>(g, f[v], p, x0 + v, y, z);
+        }
         return;
     }
     const bool px = g.per & 1;
-    const bool xlo = !px && x == 0, xhi = !px && x == nx - 1;
     const bool ylo = !(g.per & 2) && y == 0, yhi = !(g.per & 2) && y == g.ny - 1;
     const bool zlo = !(g.per & 4) && z == 0, zhi = !(g.per & 4) && z == g.nz - 1;
     const int nyz = (ylo || yhi) + (zlo || zhi);
-    const bool xf = xlo || xhi;
-    if (nyz + xf >= 2) return;  // domain edge
-    if (sgp && sgp->mode == 1) {
-        // moving bodies: the fix-up blocks own their whole regions
-        bool row = false;
-        for (int m = sgp->first_fix_box; m < sgp->nbox && !row; ++m) {
-            const Box& b = sgp->box[m];
-            if (b.kind != 1) continue;
-            int ry = y - b.lo[1], rz = z - b.lo[2];
-            if (g.per & 2) ry = ((ry % g.ny) + g.ny) % g.ny;
-            if (g.per & 4) rz = ((rz % g.nz) + g.nz) % g.nz;
-            row = ry >= 0 && ry < b.ext[1] && rz >= 0 && rz < b.ext[2];
-        }
-        if (row && in_body_box(*sgp, x, y, z, g)) return;
-    } else if (sgp && sgp->nbodies > 0) {
-        // static bodies: boundary nodes carry the fix-up bit, face cells of a
-        // body region belong to the fix-up too
-        if (flags[cell] & kFlagFixup) return;
-        if ((nyz + xf) == 1 && in_body_box(*sgp, x, y, z, g)) return;
+    if (nyz >= 2) return;  // domain edge row
+    const bool left = x0 == 0, right = x0 + V == nx;  // the group holds x = 0 / x = nx - 1
+    const uint8_t* fixup_flags = nullptr;
+    // x-face cells of a face row are domain edges
+    if (!px && nyz == 1) {
+        if (left) act[0] = false;
+        if (right) act[V - 1] = false;
     }
+    if (bg.mode == 1) {
+        // moving bodies: the fix-up blocks own their whole regions
+        if (row_in_body_region(bg, sgp, y, z, g)) {
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (cell_in_body_region(bg, sgp, x0 + v, y, z, g)) act[v] = false;
+        }
+    } else if (bg.nbodies > 0) {
+        // static bodies: boundary nodes carry the fix-up bit (read with the
+        // populations, below), face cells of a body region belong to the fix-up
+        fixup_flags = flags + cell0;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const bool xf = !px && ((v == 0 && left) || (v == V - 1 && right));
+            if ((nyz == 1 || xf) && cell_in_body_region(bg, sgp, x0 + v, y, z, g)) act[v] = false;
+        }
+    }
+    bool any = false;
+#pragma unroll
+    for (int v = 0; v < V; ++v) any = any || act[v];
+    if (!any) return;
     if (nyz == 1) {
-        if (ylo) face_cell<S, COLL, FORCED, MODE, 1, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
-        else if (yhi) face_cell<S, COLL, FORCED, MODE, 1, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
-        else if (zlo) face_cell<S, COLL, FORCED, MODE, 2, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
-        else face_cell<S, COLL, FORCED, MODE, 2, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (!act[v] || (fixup_flags && (fixup_flags[v] & kFlagFixup))) continue;
+            const int x = x0 + v;
+            if (ylo) face_cell<S, COLL, FORCED, MODE, 1, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+            else if (yhi) face_cell<S, COLL, FORCED, MODE, 1, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+            else if (zlo) face_cell<S, COLL, FORCED, MODE, 2, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+            else face_cell<S, COLL, FORCED, MODE, 2, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
+        }
         return;
     }
-    // periodic wraps (zero on non-periodic axes)
-    const int wxm = (px && x == 0) ? nx : 0, wxp = (px && x == nx - 1) ? -nx : 0;
-    const int wym = (y == 0) ? nxy : 0, wyp = (y == g.ny - 1) ? -nxy : 0;
-    const int wzm = (z == 0) ? (int)N : 0, wzp = (z == g.nz - 1) ? -(int)N : 0;
-    const int xkind = xlo ? g.face_kind[0] : (xhi ? g.face_kind[1] : -1);
-#pragma unroll
-    for (int d = 0; d < Q; ++d) {
-        int src = cell - (kCx[d] + kCy[d] * nx + kCz[d] * nxy);
-        if (kCx[d] > 0) src += wxm;
-        if (kCx[d] < 0) src += wxp;
-        if (kCy[d] > 0) src += wym;
-        if (kCy[d] < 0) src += wyp;
-        if (kCz[d] > 0) src += wzm;
-        if (kCz[d] < 0) src += wzp;
-        if (kCx[d] != 0) {
-            // x-face lane, outgoing direction: zero gradient clamps onto the face plane,
-            // the other rules read the cell itself
-            const bool out = kCx[d] > 0 ? xlo : xhi;
-            if (out) src = xkind == LBM_FACE_ZERO_GRADIENT ? src + kCx[d] : cell;
-        }
-        f[d] = S::load(fin + (int64_t)d * g.P + src, d);
-    }
-    if (xf && xkind != LBM_FACE_ZERO_GRADIENT) {
-        C own[Q];
-#pragma unroll
-        for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
-        C rho_prev, up[3], uu2;
-        own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
-        const int fidx = xlo ? 0 : 1;
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            if (kCx[d] == 0) continue;
-            const bool out = kCx[d] > 0 ? xlo : xhi;
-            if (out) f[d] = face_value<C>(xkind, fidx, d, g, own[kOpp[d]], rho_prev, up, uu2);
-        }
-    }
-    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+    // interior row; x periodicity is a wave-uniform choice between two
+    // branch-free copies of the pull
+    if (px)
+        interior_row<S, COLL, FORCED, MODE, XM_PERIODIC>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0,
+                                                         y, z);
+    else if (rule_face(g, 0) || rule_face(g, 1))
+        interior_row<S, COLL, FORCED, MODE, XM_RULE>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
+    else
+        interior_row<S, COLL, FORCED, MODE, XM_CLAMP>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y,
+                                                      z);
 }
 
 template <class S, int COLL, bool FORCED, int MODE>
@@ -482,10 +768,10 @@ __global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict
                                                  typename S::T* __restrict__ fout, double* __restrict__ gout,
                                                  double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
                                                  Phys<typename S::C> p, const StepGeom* __restrict__ sgp,
-                                                 const uint8_t* __restrict__ flags) {
-    bulk_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, flags,
-                                     blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
-                                     blockIdx.z);
+                                                 BulkGeom bg, const uint8_t* __restrict__ flags) {
+    bulk_group<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, bg, flags,
+                                      (blockIdx.x * blockDim.x + threadIdx.x) * S::kVec,
+                                      blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -554,8 +840,9 @@ __device__ __forceinline__ void plain_cell(const typename S::T* __restrict__ fin
 
 // ---------------------------------------------------------------------------
 // The general cell: faces (any number, with precedence), bounce-back links,
-// refill, momentum exchange.  Every direction is classified first, then all
-// loads are issued together, so the cell costs one memory round trip.
+// refill, momentum exchange.  The cell's previous-step moments come first
+// from its own f*; then each direction is classified and its value loaded or
+// formed in one unrolled pass, so only the 19 pulled values stay live.
 // ---------------------------------------------------------------------------
 enum { DIR_PULL = 0, DIR_FACE = 1, DIR_LINK = 2 };
 
@@ -572,48 +859,35 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
     // solid cells are streamed like fluid ones (no links), as in the reference
     const bool self_solid = sv.nb > 0 && solid_at(sv, x, y, z, g) != 0;
     const bool fresh = sv.nb > 0 && !self_solid && g.refill && fresh_at(sv, x, y, z, g);
-    C own[Q];
-#pragma unroll
-    for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
     C f[Q];
     if (fresh) {
-        // equilibrium of the mean previous-step rho, u of fluid neighbours; all
-        // neighbour loads are unrolled (unused ones read the cell itself)
+        // equilibrium of the mean previous-step rho, u of fluid neighbours
         C sr = C(0), su = C(0), sv_ = C(0), sw = C(0);
         int cnt = 0;
-#pragma unroll
+#pragma unroll 3
         for (int d = 0; d < 18; ++d) {
-            int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
+            int q[3] = {x + c_at(0, d), y + c_at(1, d), z + c_at(2, d)};
             bool ok = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
-            const bool use = ok && solid_at(sv, q[0], q[1], q[2], g) == 0 && !fresh_at(sv, q[0], q[1], q[2], g);
-            const int64_t nc = use ? (int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2] : cell;
-            C r = C(0), mx = C(0), my = C(0), mz = C(0);
+            if (!ok || solid_at(sv, q[0], q[1], q[2], g) != 0 || fresh_at(sv, q[0], q[1], q[2], g)) continue;
+            const int64_t nc = (int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2];
+            C own[Q];
 #pragma unroll
-            for (int e = 0; e < Q; ++e) {
-                const C v = S::load(fin + (int64_t)e * g.P + nc, e);
-                r += v;
-                if (kCx[e] > 0) mx += v;
-                if (kCx[e] < 0) mx -= v;
-                if (kCy[e] > 0) my += v;
-                if (kCy[e] < 0) my -= v;
-                if (kCz[e] > 0) mz += v;
-                if (kCz[e] < 0) mz -= v;
-            }
+            for (int e = 0; e < Q; ++e) own[e] = S::load(fin + (int64_t)e * g.P + nc, e);
+            C r, mx, my, mz;
+            moments<C>(own, r, mx, my, mz);
             C ux = mx / r, uy = my / r, uz = mz / r;
             if (FORCED) {
                 ux -= C(0.5) * p.g[0];
                 uy -= C(0.5) * p.g[1];
                 uz -= C(0.5) * p.g[2];
             }
-            if (use) {
-                sr += r;
-                su += ux;
-                sv_ += uy;
-                sw += uz;
-                ++cnt;
-            }
+            sr += r;
+            su += ux;
+            sv_ += uy;
+            sw += uz;
+            ++cnt;
         }
         C r = C(g.rho0), ux = C(0), uy = C(0), uz = C(0);
         if (cnt > 0) {
@@ -629,18 +903,44 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             const C cu = C(kCx[d]) * ux + C(kCy[d]) * uy + C(kCz[d]) * uz;
             f[d] = weight<C>(d) * r * (C(1) + cu * C(3) + C(0.5) * cu * cu * C(9) - C(0.5) * u2 * C(3));
         }
-        finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+        finish_face_cell<S, COLL, FORCED, MODE>(f, p, cell, g, fout, gout, rho_out, u_out, x, y, z, -1);
         return;
     }
-    // 1) classify every direction and issue its load right away (the loaded
-    //    values are only consumed after the loop, so the loads overlap)
-    int code[Q];  // DIR_* | (face index or body index) << 2
+    // previous-step moments of the cell (src/MainCore.jl:316,336) and the
+    // solidity of every in-domain pull source x - c_d, in one memory round
+    // trip; the body index of a link source is packed 10 bits per direction
+    C rho_prev, up[3], uu2;
+    uint32_t codes[(Q + 2) / 3];
+#pragma unroll
+    for (int i = 0; i < (Q + 2) / 3; ++i) codes[i] = 0u;
+    {
+        C own[Q];
+#pragma unroll
+        for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
+        if (!self_solid && sv.nb > 0) {
+#pragma unroll
+            for (int d = 0; d < 18; ++d) {
+                int sx = x - kCx[d], sy = y - kCy[d], sz = z - kCz[d];
+                const bool okx = wrap_coord(sx, g.nx, g.per & 1), oky = wrap_coord(sy, g.ny, g.per & 2),
+                           okz = wrap_coord(sz, g.nz, g.per & 4);
+                const bool ok = okx && oky && okz;
+                // outside a non-periodic face: read the cell itself and drop it
+                const int sbr = solid_at(sv, ok ? sx : x, ok ? sy : y, ok ? sz : z, g);
+                const int sb = ok ? sbr : 0;
+                if (sb != 0) codes[d / 3] |= (uint32_t)(DIR_LINK | ((sb - 1) << 2)) << (10 * (d % 3));
+            }
+        }
+        own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // 1) faces (with precedence) and the plain pulls
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         const int cd[3] = {kCx[d], kCy[d], kCz[d]};
         int s[3];
         int fas[3] = {-1, -1, -1};
         bool out = false;
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
             s[a] = q0[a] - cd[a];
             if (s[a] < 0 || s[a] >= L[a]) {
@@ -652,7 +952,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
                 }
             }
         }
-        code[d] = DIR_PULL;
+        bool pull = ((codes[d / 3] >> (10 * (d % 3))) & 3u) == 0u;
         if (out) {
             // edge precedence: VELOCITY > PRESSURE_ABB > ZERO_GRADIENT, lower axis first
             int chosen = -1, kind = LBM_FACE_ZERO_GRADIENT;
@@ -667,37 +967,32 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
                 for (int a = 0; a < 3; ++a)
                     if (fas[a] >= 0) s[a] = q0[a];
             } else {
-                code[d] = DIR_FACE | (chosen << 2);
-                s[0] = x;
-                s[1] = y;
-                s[2] = z;
-            }
-        } else if (!self_solid && sv.nb > 0) {
-            const int sb = solid_at(sv, s[0], s[1], s[2], g);
-            if (sb != 0) {
-                code[d] = DIR_LINK | ((sb - 1) << 2);
-                s[0] = x;
-                s[1] = y;
-                s[2] = z;
+                codes[d / 3] |= (uint32_t)(DIR_FACE | (chosen << 2)) << (10 * (d % 3));
+                pull = false;
             }
         }
-        f[d] = S::load(fin + (int64_t)d * g.P + (s[0] + g.nx * s[1] + (int)g.nxy * s[2]), d);
+        // a face or link direction loads the cell's own f*[opp(d)], which its
+        // rule below starts from, so the rules need no further memory access
+        const typename S::T* a = fin + (int64_t)d * g.P + (s[0] + g.nx * s[1] + (int)g.nxy * s[2]);
+        const typename S::T* b = fin + (int64_t)kOpp[d] * g.P + cell;
+        f[d] = S::load(pull ? a : b, pull ? d : kOpp[d]);
     }
-    // 2) boundary values
-    C rho_prev, up[3], uu2;
-    own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
+    __builtin_amdgcn_sched_barrier(0);
+    // 2) face and link values; wall point of the moving-wall velocity:
+    //    physical x + c_dl/2, or the reference's x - c_dl/2 (src/SolidBC.jl:195-196)
+    const double hs = g.wall_point == LBM_WALL_REFERENCE ? -0.5 : 0.5;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-        const int kd = code[d] & 3, aux = code[d] >> 2;
-        if (kd == DIR_FACE) {
-            f[d] = face_value<C>(face_kind_at(g, aux), aux, d, g, own[kOpp[d]], rho_prev, up, uu2);
-        } else if (kd == DIR_LINK) {
+        const uint32_t code = (codes[d / 3] >> (10 * (d % 3))) & 0x3ffu;
+        const int kind_d = (int)(code & 3u), aux = (int)(code >> 2);
+        if (kind_d == DIR_FACE) {
+            f[d] = face_value<C>(face_kind_at(g, aux), aux, d, g, f[d], rho_prev, up, uu2);
+        } else if (kind_d == DIR_LINK) {
             // boundary link (x, dl), dl = opp(d): the solid node is x + c_dl
             const int dl = kOpp[d];
             const DevBody& B = sv.bnew[aux];
-            // wall point: physical x + c_dl/2, or the reference's x - c_dl/2
-            const double hs = g.wall_point == LBM_WALL_REFERENCE ? -0.5 : 0.5;
             double r[3];
+#pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const int cdl = a == 0 ? kCx[dl] : (a == 1 ? kCy[dl] : kCz[dl]);
                 r[a] = (double)q0[a] + 0.5 + hs * (double)cdl - B.ch[a];
@@ -711,7 +1006,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             const C uwy = C(B.v[1] + (B.om[2] * r[0] - B.om[0] * r[2]));
             const C uwz = C(B.v[2] + (B.om[0] * r[1] - B.om[1] * r[0]));
             const C cuw = C(kCx[dl]) * uwx + C(kCy[dl]) * uwy + C(kCz[dl]) * uwz;
-            const C fin_dl = own[dl];
+            const C fin_dl = f[d];
             const C val = fin_dl - C(2) * weight<C>(dl) * rho_prev * cuw * C(3);
             f[d] = val;
             if (MODE == MODE_STEP) {
@@ -719,6 +1014,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
                 const double t = (double)(fin_dl + val);
                 const double P[3] = {t * kCx[dl], t * kCy[dl], t * kCz[dl]};
                 double ra[3];
+#pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     const int cdl = a == 0 ? kCx[dl] : (a == 1 ? kCy[dl] : kCz[dl]);
                     ra[a] = (double)q0[a] + 0.5 + 0.5 * (double)cdl - B.ch[a];
@@ -730,17 +1026,16 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
                 }
                 const double v6[6] = {P[0], P[1], P[2], ra[1] * P[2] - ra[2] * P[1], ra[2] * P[0] - ra[0] * P[2],
                                       ra[0] * P[1] - ra[1] * P[0]};
-                const int bi = aux;
-                if (acc.b0 < 0 || acc.b0 == bi) {
-                    acc.b0 = bi;
+                if (acc.b0 < 0 || acc.b0 == aux) {
+                    acc.b0 = aux;
                     for (int c = 0; c < 6; ++c) acc.a0[c] += v6[c];
                 } else {
-                    for (int c = 0; c < 6; ++c) atomicAdd(&lds_overflow[bi * 6 + c], v6[c]);
+                    for (int c = 0; c < 6; ++c) atomicAdd(&lds_overflow[aux * 6 + c], v6[c]);
                 }
             }
         }
     }
-    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+    finish_face_cell<S, COLL, FORCED, MODE>(f, p, cell, g, fout, gout, rho_out, u_out, x, y, z, -1);
 }
 
 // ---------------------------------------------------------------------------
@@ -763,13 +1058,27 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
                                             const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
                                             uint8_t* __restrict__ flags, const int32_t* __restrict__ list,
                                             const int32_t* __restrict__ list_count, double* __restrict__ partials,
-                                            int write_flags, double (*s_acc)[LBM_MAX_BODIES * 6], double* s_over) {
+                                            int write_flags, double* lds) {
+    // The fix-up waves share their CUs with bulk waves; their latency is the
+    // tail of the step, so they take issue priority.
+    __builtin_amdgcn_s_setprio(3);
     const StepGeom& sg = *sgp;
     const int nb = sg.nbodies;
+    const int n6 = nb * 6;
     const int tid = threadIdx.x;
-    for (int i = tid; i < nb * 6; i += kBlock) {
-        for (int w = 0; w < kWaves; ++w) s_acc[w][i] = 0.0;
-        s_over[i] = 0.0;
+    // LDS: per wave nbodies x 6 partial sums, then the cross-body overflow sums
+    double* s_over = lds + kWaves * n6;
+    for (int i = tid; i < (kWaves + 1) * n6; i += kBlock) lds[i] = 0.0;
+    // then the bodies (new poses, then old), when few enough
+    const bool staged = nb <= kLdsBodies;
+    DevBody* s_bodies = reinterpret_cast<DevBody*>(lds + (kWaves + 1) * n6);
+    if (staged) {
+        const int nw = 2 * nb * (int)(sizeof(DevBody) / 4);
+        const uint32_t* src_new = reinterpret_cast<const uint32_t*>(bnew);
+        const uint32_t* src_old = reinterpret_cast<const uint32_t*>(bold);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_bodies);
+        const int half = nw / 2;
+        for (int i = tid; i < nw; i += kBlock) dst[i] = i < half ? src_new[i] : src_old[i - half];
     }
     __syncthreads();
 
@@ -777,8 +1086,8 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
     sv.mode = sg.mode;
     sv.nb = nb;
     sv.flags = flags;
-    sv.bnew = bnew;
-    sv.bold = bold;
+    sv.bnew = staged ? s_bodies : bnew;
+    sv.bold = staged ? s_bodies + nb : bold;
 
     ForceAcc acc;
     acc.b0 = -1;
@@ -787,16 +1096,29 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
     bool work = false, plain = false;
     int x = 0, y = 0, z = 0;
     if (blk < sg.nfix_blocks) {
-        int bi = sg.first_fix_box;
-        while (bi + 1 < sg.nbox && sg.box[bi + 1].first_block <= blk) ++bi;
+        // the box of this block: the number of boxes starting at or before
+        // it, from independent loads of the first kScanBoxes start blocks
+        int bi;
+        if (sg.nbox <= kScanBoxes) {
+            int cnt = 0;
+#pragma unroll
+            for (int k = 0; k < kScanBoxes; ++k) {
+                const int fb = sg.box[k].first_block;
+                cnt += (k >= sg.first_fix_box && k < sg.nbox && fb <= blk) ? 1 : 0;
+            }
+            bi = sg.first_fix_box + cnt - 1;
+        } else {
+            bi = sg.first_fix_box;
+            while (bi + 1 < sg.nbox && sg.box[bi + 1].first_block <= blk) ++bi;
+        }
         const Box b = sg.box[bi];
-        const int64_t t = (int64_t)(blk - b.first_block) * kBlock + tid;
-        const int64_t vol = (int64_t)b.ext[0] * b.ext[1] * b.ext[2];
+        const int t = (blk - b.first_block) * kBlock + tid;  // box volumes stay far below 2^31
+        const int vol = b.ext[0] * b.ext[1] * b.ext[2];
         if (t < vol) {
-            const int i = (int)(t % b.ext[0]);
-            const int64_t r = t / b.ext[0];
-            const int j = (int)(r % b.ext[1]);
-            const int k = (int)(r / b.ext[1]);
+            const int i = t % b.ext[0];
+            const int r = t / b.ext[0];
+            const int j = r % b.ext[1];
+            const int k = r / b.ext[1];
             x = b.lo[0] + i;
             y = b.lo[1] + j;
             z = b.lo[2] + k;
@@ -807,11 +1129,22 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
             if (z < 0) z += g.nz;
             if (z >= g.nz) z -= g.nz;
             bool dup = false;
-            for (int m = sg.first_fix_box; m < bi && !dup; ++m) dup = in_box(sg.box[m], x, y, z, g);
+            if (b.ovl) {
+                if (bi <= kScanBoxes) {
+#pragma unroll
+                    for (int m = 0; m < kScanBoxes; ++m) {
+                        const bool in = in_box_normalised(sg.box[m], x, y, z, g);
+                        dup = dup || (m >= sg.first_fix_box && m < bi && in);
+                    }
+                } else {
+                    for (int m = sg.first_fix_box; m < bi && !dup; ++m) dup = in_box(sg.box[m], x, y, z, g);
+                }
+            }
             if (!dup && sv.mode == 1 && write_flags) {
-                const int fl = flag_of(bnew, nb, x, y, z, g);
+                // moving bodies: the region's flags follow the new pose (the
+                // static fix-up bits are rebuilt when the bodies stop)
                 const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
-                if ((flags[cell] & kFlagBody) != fl) flags[cell] = (uint8_t)fl;
+                flags[cell] = (uint8_t)flag_of(sv.bnew, nb, x, y, z, g);
             }
             if (b.kind == 2) {
                 work = !dup;  // domain edge: every cell
@@ -824,15 +1157,20 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
                 if (sv.mode == 1) {
                     work = true;
                     if (!face) {
+                        // a fluid cell with a solid neighbour, or a fresh one, takes
+                        // the general path
                         bool gen = false;
-                        if (flag_of(bnew, nb, x, y, z, g) == 0) {
+                        if (flag_of(sv.bnew, nb, x, y, z, g) == 0) {
                             gen = g.refill && fresh_at(sv, x, y, z, g);
                             const int L[3] = {g.nx, g.ny, g.nz};
-                            for (int d = 0; d < 18 && !gen; ++d) {
+#pragma unroll
+                            for (int d = 0; d < 18; ++d) {
                                 int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
                                 bool ok = true;
+#pragma unroll
                                 for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
-                                if (ok && solid_at(sv, q[0], q[1], q[2], g) != 0) gen = true;
+                                const int sb = flag_of(sv.bnew, nb, ok ? q[0] : x, ok ? q[1] : y, ok ? q[2] : z, g);
+                                gen = gen || (ok && sb != 0);
                             }
                         }
                         plain = !gen;
@@ -861,12 +1199,12 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
     }
     if (MODE == MODE_STEP && nb > 0) {
         const int wave = tid >> 6;
-        wave_reduce_to_lds(acc.b0, acc.a0, s_acc[wave]);
+        wave_reduce_to_lds(acc.b0, acc.a0, lds + wave * n6);
         __syncthreads();
-        for (int i = tid; i < nb * 6; i += kBlock) {
+        for (int i = tid; i < n6; i += kBlock) {
             double s = s_over[i];
-            for (int w = 0; w < kWaves; ++w) s += s_acc[w][i];
-            partials[(int64_t)blk * nb * 6 + i] = s;
+            for (int w = 0; w < kWaves; ++w) s += lds[w * n6 + i];
+            partials[(int64_t)blk * n6 + i] = s;
         }
     }
 }
@@ -878,12 +1216,36 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
 // two sets write disjoint cells, so they run side by side with no barrier.
 // ---------------------------------------------------------------------------
 struct BulkShape {
-    int bx, by;   // cells per block along x and rows per block
+    int bx, by;   // cell groups per block along x and rows per block
     int gx, gy;   // blocks along x and y
 };
 
+template <class S>
+inline BulkShape bulk_shape(const Grid& g) {
+    const int groups = (g.nx + S::kVec - 1) / S::kVec;
+    BulkShape bs;
+    bs.bx = groups >= 256 ? 256 : (groups >= 128 ? 128 : 64);
+    bs.by = kBlock / bs.bx;
+    bs.gx = (groups + bs.bx - 1) / bs.bx;
+    bs.gy = (g.ny + bs.by - 1) / bs.by;
+    return bs;
+}
+
+// dynamic LDS of a step launch: the fix-up blocks' force partials
+inline size_t step_lds_bytes(int nbodies) {
+    size_t b = (size_t)(kWaves + 1) * (size_t)nbodies * 6 * sizeof(double);
+    if (nbodies <= kLdsBodies) b += 2 * (size_t)nbodies * sizeof(DevBody);
+    return b;
+}
+
+// Minimum waves per SIMD the step kernel is compiled for.  The bulk rows need
+// few registers; the rare boundary path needs many, and a higher floor makes
+// it spill to scratch instead of lowering the occupancy of every bulk wave.
+#ifndef LBM_STEP_MIN_WAVES
+#define LBM_STEP_MIN_WAVES 1
+#endif
 template <class S, int COLL, bool FORCED, int MODE>
-__global__ __launch_bounds__(kBlock) void k_step(const typename S::T* __restrict__ fin,
+__global__ __launch_bounds__(kBlock, LBM_STEP_MIN_WAVES) void k_step(const typename S::T* __restrict__ fin,
                                                  typename S::T* __restrict__ fout, double* __restrict__ gout,
                                                  double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
                                                  Phys<typename S::C> p, const StepGeom* __restrict__ sgp,
@@ -891,13 +1253,12 @@ __global__ __launch_bounds__(kBlock) void k_step(const typename S::T* __restrict
                                                  uint8_t* __restrict__ flags, const int32_t* __restrict__ list,
                                                  const int32_t* __restrict__ list_count,
                                                  double* __restrict__ partials, int write_flags, BulkShape bs,
-                                                 int nfix) {
-    __shared__ double s_acc[kWaves][LBM_MAX_BODIES * 6];
-    __shared__ double s_over[LBM_MAX_BODIES * 6];
+                                                 BulkGeom bg, int nfix) {
+    extern __shared__ double lds_dyn[];
     const int blk = blockIdx.x;
     if (blk < nfix) {
         fixup_block<S, COLL, FORCED, MODE>(blk, fin, fout, gout, rho_out, u_out, g, p, sgp, bnew, bold, flags, list,
-                                           list_count, partials, write_flags, s_acc, s_over);
+                                           list_count, partials, write_flags, lds_dyn);
         return;
     }
     const int b = blk - nfix;
@@ -906,8 +1267,9 @@ __global__ __launch_bounds__(kBlock) void k_step(const typename S::T* __restrict
     const int byi = r % bs.gy;
     const int bz = r / bs.gy;
     const int tx = threadIdx.x % bs.bx, ty = threadIdx.x / bs.bx;
-    bulk_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, flags, bxi * bs.bx + tx,
-                                     byi * bs.by + ty, bz);
+    bulk_group<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, bg, flags,
+                                      (bxi * bs.bx + tx) * S::kVec,
+                                      byi * bs.by + ty, bz);
 }
 
 // ---------------------------------------------------------------------------

@@ -6,21 +6,12 @@
 
 namespace lbm {
 
-inline BulkShape bulk_shape(const Grid& g) {
-    BulkShape bs;
-    bs.bx = g.nx >= 256 ? 256 : (g.nx >= 128 ? 128 : 64);
-    bs.by = kBlock / bs.bx;
-    bs.gx = (g.nx + bs.bx - 1) / bs.bx;
-    bs.gy = (g.ny + bs.by - 1) / bs.by;
-    return bs;
-}
-
 template <class S, int MODE, int COLL, bool FORCED>
 static void launch_bulk_t(const LaunchReq& r, const Phys<typename S::C>& p) {
     using T = typename S::T;
-    const BulkShape bs = bulk_shape(r.g);
+    const BulkShape bs = bulk_shape<S>(r.g);
     hipLaunchKernelGGL((k_bulk<S, COLL, FORCED, MODE>), dim3(bs.gx, bs.gy, r.g.nz), dim3(bs.bx, bs.by), 0, r.stream,
-                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, (const StepGeom*)r.rec,
+                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, (const StepGeom*)r.rec, r.bg,
                        (const uint8_t*)r.flags);
 }
 
@@ -30,11 +21,11 @@ static void launch_step_t(const LaunchReq& r, const Phys<typename S::C>& p) {
     const StepGeom* sg = (const StepGeom*)r.rec;
     const DevBody* bn = (const DevBody*)(r.rec + sizeof(StepGeom));
     const DevBody* bo = bn + r.nb;
-    const BulkShape bs = bulk_shape(r.g);
+    const BulkShape bs = bulk_shape<S>(r.g);
     const int64_t nbulk = (int64_t)bs.gx * bs.gy * r.g.nz;
-    hipLaunchKernelGGL((k_step<S, COLL, FORCED, MODE>), dim3((unsigned)(r.grid_blocks + nbulk)), dim3(kBlock), 0,
-                       r.stream, (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, sg, bn, bo, r.flags,
-                       r.list, r.list_count, r.partials, r.write_flags, bs, r.grid_blocks);
+    hipLaunchKernelGGL((k_step<S, COLL, FORCED, MODE>), dim3((unsigned)(r.grid_blocks + nbulk)), dim3(kBlock),
+                       (unsigned)step_lds_bytes(r.nb), r.stream, (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, sg, bn, bo, r.flags,
+                       r.list, r.list_count, r.partials, r.write_flags, bs, r.bg, r.grid_blocks);
 }
 
 template <class S, int MODE>

@@ -10,7 +10,11 @@ namespace lbm {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxBoxes = 6 + 12 + LBM_MAX_BODIES;
+constexpr int kMaxBoxes = 12 + LBM_MAX_BODIES;  // domain edges, bodies
+constexpr int kScanBoxes = 32;                   // boxes a fix-up block scans with independent loads
+// Bodies the fix-up blocks stage in LDS (new and old pose) so the analytic
+// solidity tests and the link kinematics read LDS instead of global memory.
+constexpr int kLdsBodies = 16;
 
 // Flag byte: low 7 bits 0 (fluid) or k+1 (solid of body k); bit 7 marks a fluid
 // boundary node of static geometry, finished by the fix-up blocks.
@@ -32,14 +36,15 @@ struct DevBody {
 };
 
 // A box of cells: lo may lie outside [0, L) on a periodic axis (wrapped).
-// kind 0 = face slab, 1 = body region.
+// kind 1 = body region, 2 = every cell on the general path (domain edges and
+// the x faces with a velocity or anti-bounce-back rule).
 struct Box {
     int lo[3];
     int ext[3];
     int kind;
     int first_block;
     int face;  // face slab: 2 * axis + side (0..5); body region: -1
-    int pad;
+    int ovl;   // 1: may share cells with an earlier fix-up box (cells there belong to the first)
 };
 
 struct Grid {
@@ -53,6 +58,13 @@ struct Grid {
     int wall_point;
     int refill;
     double rho0;
+    // Face moments: for every cell on a velocity or anti-bounce-back face, the
+    // (rho, u) its rule reads next step, four compute-type values per slot,
+    // written when the cell's f* is produced.  face_base[f] is the first slot
+    // of face f (-1: periodic or zero-gradient face, no slots).
+    int face_base[6];
+    void* side_in;   // per launch: moments of the face cells of fin's f*
+    void* side_out;  // per launch: the same for fout
 };
 
 template <typename C>
@@ -61,19 +73,31 @@ struct Phys {
     C g[3];
 };
 
-// Geometry of one step for the fix-up pass.
-// Boxes: face slabs (kind 0) first, walked by k_faces; then domain edges
-// (kind 2) and body regions (kind 1), walked by k_fixup.
+// Geometry of one step for the fix-up blocks of k_step: domain edges and
+// general-path x faces (kind 2) first, then body regions (kind 1); a cell in
+// several boxes belongs to the first.
 struct StepGeom {
     int nbox;
-    int nface_blocks;  // k_faces grid (blocks over the face slabs)
-    int nfix_blocks;   // k_fixup blocks over edges and body regions
-    int first_fix_box; // index of the first edge/body box
+    int nface_blocks;  // unused (0)
+    int nfix_blocks;   // fix-up blocks over the boxes
+    int first_fix_box; // index of the first box the fix-up blocks walk
     int nbodies;
     int mode;          // 0: solidity from flags (static), 1: analytic (moving bodies)
     int list_blocks;   // k_fixup blocks over the static boundary-node list (mode 0)
     int pad;
     Box box[kMaxBoxes];
+};
+
+// What the bulk rows need of a step's geometry, passed by value with the
+// kernel arguments so a bulk wave reads it without a dependent load from the
+// step record: the mode, the body count and up to kBulkBoxes body regions.
+constexpr int kBulkBoxes = 8;
+struct BulkGeom {
+    int mode;     // 0: static (fix-up bit in the flags), 1: moving (analytic regions)
+    int nbodies;
+    int nbox;     // body regions listed below; -1: more than kBulkBoxes (read the step record)
+    int lo[3][kBulkBoxes];   // lo in [0, L) on periodic axes
+    int ext[3][kBulkBoxes];
 };
 
 enum { MODE_STEP = 0, MODE_GET = 1, MODE_COLLIDE_ONLY = 2, MODE_GET_RAW = 3 };
@@ -95,6 +119,7 @@ struct LaunchReq {
     double* gout;
     double* rho_out;
     double* u_out;
+    BulkGeom bg;
     // fix-up
     int grid_blocks;
     const char* rec;

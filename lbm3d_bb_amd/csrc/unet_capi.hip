@@ -607,6 +607,12 @@ int lbm_unet_train_step(lbm_unet* n, const float* x, const float* target, int32_
     return 0;
 }
 
+int lbm_unet_set_lr(lbm_unet* n, float lr) {
+    if (!n || !(lr > 0.0f)) return -1;
+    n->cfg.lr = lr;
+    return 0;
+}
+
 int lbm_unet_last_timing(lbm_unet* n, double* ms, double* flops) {
     if (!n) return -1;
     if (ms) *ms = n->last_ms;

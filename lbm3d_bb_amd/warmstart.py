@@ -37,6 +37,7 @@ _SIGS = {
     "lbm_unet_forward": [ct.c_void_p, _F, ct.c_int32, _F],
     "lbm_unet_train_step": [ct.c_void_p, _F, _F, ct.c_int32, _F],
     "lbm_unet_gradient": [ct.c_void_p, _F, _F, ct.c_int32, _F, _F],
+    "lbm_unet_set_lr": [ct.c_void_p, ct.c_float],
     "lbm_unet_last_timing": [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)],
 }
 UNET_SYMBOLS = list(_SIGS)
@@ -146,16 +147,26 @@ class UNet:
         return ms.value, fl.value
 
 
+    def set_lr(self, lr: float) -> None:
+        self._chk(self.lib.lbm_unet_set_lr(self._h, float(lr)))
+
+    def save(self, path: str) -> None:
+        np.save(path, self.get_params())
+
+    def load(self, path: str) -> None:
+        self.set_params(np.load(path, allow_pickle=False))
+
+
 # --------------------------------------------------------------------------
 # Network inputs/outputs for the flow problem
 # --------------------------------------------------------------------------
-RE_SCALE = 100.0
+LOG_RE_MAX = float(np.log(100.0))
 
 
 def encode_input(mask: np.ndarray, re: float) -> np.ndarray:
-    """(mask, Re/100) channels, shape (2, nz, ny, nx)."""
+    """(solid mask, ln(Re)/ln(100)) channels, shape (2, nz, ny, nx)."""
     m = (np.asarray(mask) != 0).astype(np.float32)
-    return np.stack([m, np.full_like(m, re / RE_SCALE)])
+    return np.stack([m, np.full_like(m, np.log(re) / LOG_RE_MAX)])
 
 
 def encode_target(rho: np.ndarray, u: np.ndarray, u_ref: float, shape) -> np.ndarray:
@@ -171,3 +182,149 @@ def decode_output(y: np.ndarray, u_ref: float) -> tuple[np.ndarray, np.ndarray]:
     rho = 1.0 + u_ref * y[0].reshape(-1).astype(np.float64)
     u = u_ref * y[1:4].reshape(3, -1).astype(np.float64)
     return rho, u
+
+
+# --------------------------------------------------------------------------
+# Warm start with fallback, steady solves, data, training, evaluation
+# --------------------------------------------------------------------------
+def plausible(rho: np.ndarray, u: np.ndarray, u_ref: float) -> tuple[bool, str]:
+    """A predicted field is used only if it is finite, its density stays in
+    (0.5, 1.5) and its speed below max(0.3, 6 u_ref): anything else is a
+    garbage prediction and the uniform start is taken instead."""
+    if not (np.all(np.isfinite(rho)) and np.all(np.isfinite(u))):
+        return False, "non-finite prediction"
+    if rho.min() <= 0.5 or rho.max() >= 1.5:
+        return False, f"density out of range [{rho.min():.3g}, {rho.max():.3g}]"
+    vmax = float(np.sqrt((u * u).sum(axis=0)).max())
+    if vmax >= max(0.3, 6.0 * u_ref):
+        return False, f"speed {vmax:.3g} too large"
+    return True, "ok"
+
+
+def predict(net: UNet, case) -> tuple[np.ndarray, np.ndarray]:
+    """(rho, u) predicted for a pack case; u is (3, N), solid cells at rest."""
+    L = case.L
+    mask = case.mask()
+    y = net.forward(encode_input(mask, case.re)[None])[0]
+    rho, u = decode_output(y, case.u_in)
+    solid = mask.reshape(-1)
+    u[:, solid] = 0.0
+    return rho, u
+
+
+def warm_start(solver, net: UNet | None, case) -> dict:
+    """Initialises ``solver`` at equilibrium of the network's (rho, u), or of
+    the uniform state when there is no network or its prediction is not
+    plausible.  Returns which start was taken and why."""
+    if net is not None:
+        rho, u = predict(net, case)
+        ok, why = plausible(rho, u, case.u_in)
+        if ok:
+            solver.init_equilibrium(rho, u)
+            return {"start": "net", "reason": why}
+    else:
+        why = "no network"
+    solver.init_equilibrium()
+    return {"start": "uniform", "reason": why}
+
+
+def solve_steady(case, net: UNet | None = None, use_net: bool = False, tol: float = 1e-4, every: int = 100,
+                 patience: int = 3, max_steps: int = 100000, storage: str = "fp32", fields: bool = False) -> dict:
+    """Runs one pack case to a converged drag (the solver's own residual rule,
+    lbm3d_bb_amd.convergence) from the uniform or the network start."""
+    from .convergence import run_to_convergence
+    from .solver import Solver
+
+    s = Solver(case.config(storage))
+    try:
+        s.set_bodies(case.bodies)
+        start = warm_start(s, net if use_net else None, case)
+        res = run_to_convergence(s, tol=tol, every=every, patience=patience, max_steps=max_steps)
+        if use_net and start["start"] == "net" and not res["converged"]:
+            # a prediction that does not lead to convergence: restart uniformly
+            s.init_equilibrium()
+            start = {"start": "uniform", "reason": "net start did not converge"}
+            r2 = run_to_convergence(s, tol=tol, every=every, patience=patience, max_steps=max_steps)
+            r2["steps"] += res["steps"]
+            res = r2
+        res.update(start)
+        res.pop("history", None)
+        if fields:
+            res["rho"], res["u"] = s.macros()
+    finally:
+        s.close()
+    return res
+
+
+def generate_dataset(seeds, L: int, log=None, **kw) -> dict:
+    """Steady FP32 fields of the packs ``seeds`` from uniform starts: network
+    inputs X (n, 2, L, L, L), targets Y (n, 4, L, L, L) and the plain
+    steps-to-converged-drag of every sample."""
+    from .packs import make_pack
+
+    n = len(seeds)
+    X = np.empty((n, 2, L, L, L), dtype=np.float32)
+    Y = np.empty((n, 4, L, L, L), dtype=np.float32)
+    steps = np.zeros(n, dtype=np.int64)
+    conv = np.zeros(n, dtype=bool)
+    re = np.zeros(n)
+    for i, sd in enumerate(seeds):
+        case = make_pack(int(sd), L)
+        r = solve_steady(case, fields=True, **kw)
+        X[i] = encode_input(case.mask(), case.re)
+        Y[i] = encode_target(r["rho"], r["u"], case.u_in, (L, L, L))
+        steps[i], conv[i], re[i] = r["steps"], r["converged"], case.re
+        if log:
+            log(f"sample {i + 1}/{n} seed {sd} Re {case.re:.2f} spheres {len(case.bodies)} "
+                f"steps {r['steps']} converged {r['converged']}")
+    return {"X": X, "Y": Y, "steps": steps, "converged": conv, "re": re, "seeds": np.asarray(seeds)}
+
+
+def train(net: UNet, X: np.ndarray, Y: np.ndarray, epochs: int, batch: int, lr: float = 1e-3, seed: int = 0,
+          log=None) -> list[float]:
+    """Adam on the mean-squared error with a cosine learning-rate decay; the
+    samples are reshuffled every epoch.  Returns the mean loss per epoch."""
+    rng = np.random.default_rng(seed)
+    n = X.shape[0]
+    per_epoch = max(n // batch, 1)
+    total = epochs * per_epoch
+    hist = []
+    t = 0
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        losses = []
+        for k in range(per_epoch):
+            idx = np.sort(order[k * batch:(k + 1) * batch])
+            net.set_lr(lr * 0.5 * (1.0 + np.cos(np.pi * t / total)) + 1e-6)
+            losses.append(net.train_step(X[idx], Y[idx]))
+            t += 1
+        hist.append(float(np.mean(losses)))
+        if log:
+            log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.5f}")
+    return hist
+
+
+def evaluate(net: UNet, seeds, L: int, log=None, **kw) -> dict:
+    """Steps to a converged drag from the uniform and from the network start on
+    held-out packs, and the relative difference of the two converged drags."""
+    from .packs import make_pack
+
+    rows = []
+    for i, sd in enumerate(seeds):
+        case = make_pack(int(sd), L)
+        ru = solve_steady(case, **kw)
+        rn = solve_steady(case, net=net, use_net=True, **kw)
+        d_rel = abs(rn["drag"] - ru["drag"]) / max(abs(ru["drag"]), 1e-300)
+        rows.append({"seed": int(sd), "re": case.re, "spheres": len(case.bodies), "steps_uniform": ru["steps"],
+                     "steps_net": rn["steps"], "converged_uniform": ru["converged"],
+                     "converged_net": rn["converged"], "start": rn["start"], "drag_uniform": ru["drag"],
+                     "drag_net": rn["drag"], "drag_rel_diff": d_rel})
+        if log:
+            log(f"eval {i + 1}/{len(seeds)} seed {sd} Re {case.re:.2f} steps uniform {ru['steps']} net {rn['steps']} "
+                f"({rn['start']}) drag diff {d_rel:.2e}")
+    su = np.array([r["steps_uniform"] for r in rows], dtype=np.float64)
+    sn = np.array([r["steps_net"] for r in rows], dtype=np.float64)
+    return {"rows": rows, "mean_steps_uniform": float(su.mean()), "mean_steps_net": float(sn.mean()),
+            "speedup_of_means": float(su.mean() / sn.mean()), "median_ratio": float(np.median(su / sn)),
+            "max_drag_rel_diff": float(max(r["drag_rel_diff"] for r in rows)),
+            "net_starts": int(sum(r["start"] == "net" for r in rows))}
