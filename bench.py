@@ -10,12 +10,19 @@ inlet speed).  A step is one full time step of that case; the timed region
 starts with the populations resident in HBM.
 
 Prints one JSON line.  MLUPS counts every cell, solid ones included
-(src/Utilities.jl:913-916).  ``roofline`` is the fused stream-collide kernel's
-algorithmic bytes per launch of the per-step kernel k_step (2 x 19 x 4 B per
-cell, all cells: the launch covers bulk rows, faces and body regions) over its
-mean launch time from HIP events recorded on the solver's stream around every
-16th launch in the timed region.  ``cpu_baseline`` times the oracle (a plain C restatement,
-OpenMP) on the same workload for a bounded number of steps.
+(src/Utilities.jl:913-916).  ``roofline`` is the per-step kernel k_step (one
+launch per time step: bulk rows, faces and body regions): its algorithmic
+bytes per launch (2 x 19 x s bytes per cell, s the storage size, all cells)
+over its mean launch time from HIP events recorded on the solver's stream
+around every 16th launch in the timed region; ``traffic`` is the HBM bytes per
+launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, when
+profiles/traffic_<workload>.json holds it.  ``cpu_baseline`` times the oracle (a
+plain C restatement, OpenMP) on the same workload for a bounded number of
+steps.
+
+--copy-probe MiB runs, after the timed region, ten device-to-device copies of
+that many MiB (a plain copy kernel of known size, the calibration of the
+rocprofv3 byte counters).
 """
 from __future__ import annotations
 
@@ -32,25 +39,87 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 
+_DT = {"fp64": "f64", "fp32": "f32", "fp16s": "f16 storage, f32 compute", "bf16s": "bf16 storage, f32 compute"}
+
+# channel256*: the translating-sphere channel above (the default, the metric's
+# configuration); box512_spheres64*: 512x256x256 periodic box driven by a body
+# force with 64 fixed spheres (radius uniform in [6, 10], centres by rejection
+# sampling from seed 0); channel1024_ellipsoids8: 1024x384x384 channel in
+# shifted FP16 with 8 ellipsoids (16, 8, 8) at random orientations, each
+# rotating at 1e-3 rad/step about a random axis.
 WORKLOADS = {
-    "channel256": dict(nx=256, ny=128, nz=128, storage="fp32", dtype="f32"),
-    "channel256_fp16s": dict(nx=256, ny=128, nz=128, storage="fp16s", dtype="f16 storage, f32 compute"),
-    "channel256_bf16s": dict(nx=256, ny=128, nz=128, storage="bf16s", dtype="bf16 storage, f32 compute"),
+    "channel256": dict(case="channel", nx=256, ny=128, nz=128, storage="fp32"),
+    "channel256_fp16s": dict(case="channel", nx=256, ny=128, nz=128, storage="fp16s"),
+    "channel256_bf16s": dict(case="channel", nx=256, ny=128, nz=128, storage="bf16s"),
+    "box512_spheres64": dict(case="box", nx=512, ny=256, nz=256, storage="fp32"),
+    "box512_spheres64_fp16s": dict(case="box", nx=512, ny=256, nz=256, storage="fp16s"),
+    "box512_spheres64_fp64": dict(case="box", nx=512, ny=256, nz=256, storage="fp64"),
+    "channel1024_ellipsoids8": dict(case="ellipsoids", nx=1024, ny=384, nz=384, storage="fp16s"),
 }
+for _w in WORKLOADS.values():
+    _w["dtype"] = _DT[_w["storage"]]
+
+
+def _trt(tau_p: float) -> tuple[float, float]:
+    return 1.0 / tau_p, 1.0 / (0.5 + (3.0 / 16.0) / (tau_p - 0.5))
 
 
 def make_case(w):
-    from lbm3d_bb_amd.bodies import ConstantMotion
-    from lbm3d_bb_amd.solver import Config, no_slip, velocity, zero_gradient
+    """(Config, motion laws or None, fixed bodies or None, description) of a workload."""
+    from lbm3d_bb_amd.bodies import ConstantMotion, rotation_matrix, sphere
+    from lbm3d_bb_amd.solver import Config, no_slip, periodic, velocity, zero_gradient
 
-    tau_p = 0.56
-    tau_m = 0.5 + (3.0 / 16.0) / (tau_p - 0.5)
-    cfg = Config(w["nx"], w["ny"], w["nz"], collision="TRT", storage=w["storage"], omega_plus=1.0 / tau_p,
-                 omega_minus=1.0 / tau_m,
-                 faces=[velocity((0.05, 0, 0)), zero_gradient(), no_slip(), no_slip(), no_slip(), no_slip()],
-                 u0=(0.05, 0.0, 0.0), refill=True)
-    law = ConstantMotion("sphere", (48.0, 64.0, 64.0), (12.0,), vel=(0.01, 0.0, 0.0))
-    return cfg, law
+    walls = [velocity((0.05, 0, 0)), zero_gradient(), no_slip(), no_slip(), no_slip(), no_slip()]
+    if w["case"] == "channel":
+        wp, wm = _trt(0.56)
+        cfg = Config(w["nx"], w["ny"], w["nz"], collision="TRT", storage=w["storage"], omega_plus=wp, omega_minus=wm,
+                     faces=walls, u0=(0.05, 0.0, 0.0), refill=True)
+        law = ConstantMotion("sphere", (48.0, 64.0, 64.0), (12.0,), vel=(0.01, 0.0, 0.0))
+        desc = (f"{cfg.nx}x{cfg.ny}x{cfg.nz} D3Q19 TRT channel, {w['storage']} populations, translating sphere "
+                "R=12 with moving bounce-back, refill, per-step MEA force")
+        return cfg, [law], None, desc
+    rng = np.random.default_rng(0)
+    L = np.array([w["nx"], w["ny"], w["nz"]], dtype=float)
+    if w["case"] == "box":
+        wp, wm = _trt(0.6)
+        cfg = Config(w["nx"], w["ny"], w["nz"], collision="TRT", storage=w["storage"], omega_plus=wp, omega_minus=wm,
+                     faces=[periodic()] * 6, g=(1e-7, 0.0, 0.0))
+        placed = []
+        while len(placed) < 64:
+            r = rng.uniform(6.0, 10.0)
+            c = rng.uniform(0.0, 1.0, size=3) * L
+            ok = all(np.linalg.norm(np.minimum(np.abs(c - c2), L - np.abs(c - c2))) >= r + r2 + 2.0
+                     for c2, r2 in placed)
+            if ok:
+                placed.append((c, r))
+        bodies = [sphere(tuple(c), r) for c, r in placed]
+        desc = (f"{cfg.nx}x{cfg.ny}x{cfg.nz} periodic D3Q19 TRT box, body force, {w['storage']} populations, "
+                "64 fixed spheres r in [6,10] (seed 0), per-step MEA force")
+        return cfg, None, bodies, desc
+    wp, wm = _trt(0.56)
+    cfg = Config(w["nx"], w["ny"], w["nz"], collision="TRT", storage=w["storage"], omega_plus=wp, omega_minus=wm,
+                 faces=walls, u0=(0.05, 0.0, 0.0), refill=True)
+    laws, placed = [], []
+    while len(laws) < 8:
+        c = np.array([rng.uniform(64.0, L[0] - 64.0), rng.uniform(24.0, L[1] - 24.0), rng.uniform(24.0, L[2] - 24.0)])
+        if any(np.linalg.norm(c - c2) < 36.0 for c2 in placed):
+            continue
+        placed.append(c)
+        axis0, axis1 = rng.standard_normal(3), rng.standard_normal(3)
+        rot0 = rotation_matrix(axis0, rng.uniform(0.0, 2.0 * np.pi))
+        om = 1e-3 * axis1 / np.linalg.norm(axis1)
+        laws.append(ConstantMotion("ellipsoid", tuple(c), (16.0, 8.0, 8.0), omega=tuple(om), rot0=rot0))
+    desc = (f"{cfg.nx}x{cfg.ny}x{cfg.nz} D3Q19 TRT channel, {w['storage']} populations, 8 ellipsoids (16,8,8) "
+            "rotating at 1e-3 rad/step, moving bounce-back, refill, per-step MEA force and torque")
+    return cfg, laws, None, desc
+
+
+def setup(solver, laws, bodies) -> None:
+    if laws is not None:
+        solver.set_motion(laws)
+    else:
+        solver.set_bodies(bodies)
+    solver.init_equilibrium()
 
 
 def bytes_per_cell(storage: str) -> int:
@@ -61,11 +130,10 @@ def cpu_baseline(w, budget_s: float) -> dict:
     """The oracle on the same workload, timed for a bounded number of steps."""
     from oracle.oracle import OracleSolver, load_oracle
 
-    cfg, law = make_case(w)
+    cfg, laws, bodies, _ = make_case(w)
     prec = "fp64" if w["storage"] == "fp64" else "fp32"
     o = OracleSolver(cfg, prec)
-    o.set_motion([law])
-    o.init_equilibrium()
+    setup(o, laws, bodies)
     o.step(1)  # warm-up (first geometry build)
     n = 0
     t0 = time.perf_counter()
@@ -92,19 +160,25 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-launch HIP events (no roofline)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
+                    help="after the run, ten device-to-device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): this build has no multi-GPU path; "
-              "the sharding design is a slab decomposition along x, described in DESIGN.md.", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): there is no multi-GPU path; the path "
+              "would shard as x slabs, one per GPU, exchanging the five x-moving populations of each face plane "
+              "every step.", file=sys.stderr)
         return 2
 
+    if args.copy_probe > 0:
+        import torch  # initialised before the library so the probe's copies share its HIP runtime
+
+        torch.cuda.init()
     from lbm3d_bb_amd.solver import Solver
 
     w = WORKLOADS[args.workload]
-    cfg, law = make_case(w)
+    cfg, laws, bodies, desc = make_case(w)
     s = Solver(cfg)
-    s.set_motion([law])
-    s.init_equilibrium()
+    setup(s, laws, bodies)
     if args.warmup:
         s.step(args.warmup)
     s.sync()
@@ -144,10 +218,9 @@ def main() -> int:
         "dtype": w["dtype"],
         "data": "synthetic (uniform start, prescribed body motion; no dataset)",
         "config": {
-            "workload": f"{args.workload}: {cfg.nx}x{cfg.ny}x{cfg.nz} D3Q19 TRT channel, {w['storage']} populations, "
-                        "translating sphere R=12 with moving bounce-back, refill, per-step MEA force",
+            "workload": f"{args.workload}: {desc}",
             "cells": N,
-            "tau_plus": 0.56,
+            "tau_plus": round(1.0 / cfg.omega_plus, 6),
             "lambda": 0.1875,
             "step_hbm_frac_of_peak": round(bpc * N * args.steps / el / 1e9 / HBM_PEAK_GBS, 4),
             "step_kernel_ms": round(step_avg_s * 1e3, 5),
@@ -166,10 +239,26 @@ def main() -> int:
             "traffic": traffic,
         },
     }
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def copy_probe(mib: int) -> None:
+    """Ten device-to-device copies of `mib` MiB: the plain copy kernel whose
+    rocprofv3 FETCH_SIZE / WRITE_SIZE calibrate the byte counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
 
 
 if __name__ == "__main__":

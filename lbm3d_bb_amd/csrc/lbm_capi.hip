@@ -524,12 +524,22 @@ constexpr int kStopChunk = 1;
 // flags, or the static list after a motion phase) returns kStopChunk unless
 // allow_sync, so that the rebuild sees the flags of every earlier step.
 // Returns the fix-up grid size via *grid.
-void add_body_box(const lbm_ctx* c, StepGeom& sg, const float lo_f[3], const float hi_f[3]) {
+// faces_only: keep the box only if it reaches a non-periodic face (static
+// bodies: the fix-up blocks walk a region only for its face cells).
+void add_body_box(const lbm_ctx* c, StepGeom& sg, const float lo_f[3], const float hi_f[3], bool faces_only = false) {
     Box b{};
     for (int a = 0; a < 3; ++a) index_box(c, lo_f[a], hi_f[a], a, b.lo[a], b.ext[a]);
     b.kind = 1;
     b.face = -1;
-    if ((int64_t)b.ext[0] * b.ext[1] * b.ext[2] > 0) sg.box[sg.nbox++] = b;
+    if ((int64_t)b.ext[0] * b.ext[1] * b.ext[2] <= 0) return;
+    if (faces_only) {
+        const int L[3] = {c->g.nx, c->g.ny, c->g.nz};
+        bool face = false;
+        for (int a = 0; a < 3; ++a)
+            if (!((c->g.per >> a) & 1) && (b.lo[a] == 0 || b.lo[a] + b.ext[a] == L[a])) face = true;
+        if (!face) return;
+    }
+    sg.box[sg.nbox++] = b;
 }
 
 int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bool allow_sync) {
@@ -577,9 +587,9 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
             if (rc) return rc;
             c->list_valid = true;
         }
-        // body regions (their face cells go to the general path)
+        // body regions that reach a face (their face cells go to the general path)
         if (c->flags_only) {
-            if (c->have_flags_box) add_body_box(c, sg, c->flags_lo, c->flags_hi);
+            if (c->have_flags_box) add_body_box(c, sg, c->flags_lo, c->flags_hi, true);
         } else {
             for (int i = 0; i < c->nb; ++i) {
                 float lo_f[3], hi_f[3];
@@ -587,7 +597,7 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
                     lo_f[a] = bnew[i].c[a] - c->reach[i];
                     hi_f[a] = bnew[i].c[a] + c->reach[i];
                 }
-                add_body_box(c, sg, lo_f, hi_f);
+                add_body_box(c, sg, lo_f, hi_f, true);
             }
         }
         finish_boxes(c, sg);

@@ -462,10 +462,11 @@ __device__ inline void finish_face_cell(typename S::C f[Q], const Phys<typename 
 // whose source lies outside the face take the face rule (or zero gradient);
 // all others are plain pulls with periodic wrap on the remaining axes.
 // ---------------------------------------------------------------------------
-template <class S, int COLL, bool FORCED, int MODE, int AXIS, int SIDE>
-__device__ __forceinline__ void face_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                          double* __restrict__ gout, double* __restrict__ rho_out, double* __restrict__ u_out,
-                          const Grid& g, const Phys<typename S::C>& p, int x, int y, int z) {
+// Pulled populations of a face cell (the face rule applied); returns its
+// face slot (-1 on a zero-gradient face).
+template <class S, bool FORCED, int AXIS, int SIDE>
+__device__ __forceinline__ int face_pull(const typename S::T* __restrict__ fin, const Grid& g,
+                                         const Phys<typename S::C>& p, int x, int y, int z, typename S::C (&f)[Q]) {
     using C = typename S::C;
     const int64_t N = g.N;
     const int nx = g.nx, nxy = (int)g.nxy;
@@ -481,7 +482,6 @@ __device__ __forceinline__ void face_cell(const typename S::T* __restrict__ fin,
     const int slot = zg ? -1 : face_slot(g, fidx, x, y, z);
     C rho_prev, up[3], uu2;
     side_read<C>(g, zg ? 0 : slot, rho_prev, up, uu2);  // unconditional: slot 0 always exists
-    C f[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
         const int ca = AXIS == 1 ? kCy[d] : kCz[d];
@@ -506,7 +506,8 @@ __device__ __forceinline__ void face_cell(const typename S::T* __restrict__ fin,
             f[d] = S::load(fin + (int64_t)d * g.P + src, d);
         }
     }
-    finish_face_cell<S, COLL, FORCED, MODE>(f, p, cell, g, fout, gout, rho_out, u_out, x, y, z, slot);
+    (void)p;
+    return slot;
 }
 
 // ---------------------------------------------------------------------------
@@ -558,6 +559,32 @@ __device__ inline void finish_group(typename S::C (&f)[V][Q], const bool (&act)[
 #pragma unroll
         for (int v = 0; v < V; ++v)
             if (act[v]) finish_cell<S, COLL, FORCED, MODE>(f[v], p, cell0 + v, g.N, g.P, fout, gout, rho_out, u_out);
+    }
+}
+
+// One group on a row of a single y or z face.
+template <class S, int COLL, bool FORCED, int MODE, int AXIS, int SIDE>
+__device__ __forceinline__ void face_row(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                         double* __restrict__ gout, double* __restrict__ rho_out,
+                                         double* __restrict__ u_out, const Grid& g, const Phys<typename S::C>& p,
+                                         const bool (&act_in)[S::kVec], const uint8_t* fixup_flags, int x0, int y,
+                                         int z) {
+    using C = typename S::C;
+    constexpr int V = S::kVec;
+    uint32_t fl = 0;
+    if (fixup_flags) fl = V == 2 ? *reinterpret_cast<const uint16_t*>(fixup_flags) : *fixup_flags;
+    C f[V][Q];
+    int slot[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) slot[v] = face_pull<S, FORCED, AXIS, SIDE>(fin, g, p, x0 + v, y, z, f[v]);
+    bool act[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) act[v] = act_in[v] && !((fl >> (8 * v)) & kFlagFixup);
+    finish_group<S, COLL, FORCED, MODE, V>(f, act, p, g.nx * y + (int)g.nxy * z + x0, g, fout, gout, rho_out, u_out);
+    if (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY) {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (act[v] && slot[v] >= 0) side_write<S, FORCED>(g, f[v], p, slot[v]);
     }
 }
 
@@ -620,28 +647,27 @@ __device__ __forceinline__ void interior_row(const typename S::T* __restrict__ f
         } else if (kCx[d] == 0) {
             S::load2(plane + x0, d, f[0][d], f[V - 1][d]);
         } else if (kCx[d] > 0) {
-            // pair (x0 - 1, x0); x0 - 1 = -1 on the left group
+            // pair (x0 - 1, x0); the left group loads (0, 1) instead, so no
+            // load leaves the row: x = -1 is a wrap, a clamp onto x = 0 or a
+            // face rule
             C a, b;
-            S::load2(plane + x0 - 1, d, a, b);
-            if (PX) a = S::load(plane + x0 - 1 + wxm, d);  // the same element except on the left group
-            else a = left ? b : a;                         // zero gradient: x = 0 itself
-            f[0][d] = a;
-            f[V - 1][d] = b;
+            S::load2(left ? plane + x0 : plane + x0 - 1, d, a, b);
+            f[V - 1][d] = left ? a : b;
+            f[0][d] = PX ? S::load(plane + x0 - 1 + wxm, d) : a;  // the same element as a off the left group
         } else {
-            // pair (x0 + 1, x0 + 2); x0 + 2 = nx on the right group
+            // pair (x0 + 1, x0 + 2); the right group loads (nx - 2, nx - 1)
             C a, b;
-            S::load2(plane + x0 + 1, d, a, b);
-            if (PX) b = S::load(plane + x0 + 2 + wxp, d);
-            else b = right ? a : b;
-            f[0][d] = a;
-            f[V - 1][d] = b;
+            S::load2(right ? plane + x0 : plane + x0 + 1, d, a, b);
+            f[0][d] = right ? b : a;
+            f[V - 1][d] = PX ? S::load(plane + x0 + 2 + wxp, d) : b;
         }
         if (XM == XM_RULE && kCx[d] > 0) {
             // d leaves x = 0 inward (rule at the left face reads f*[opp d] at x = 0);
-            // opp(d) leaves x = nx - 1 (rule at the right face reads f*[d] there)
+            // opp(d) leaves x = nx - 1 (rule at the right face reads f*[d] there);
+            // other lanes read an element of their own pair
             const typename S::T* own_l = fin + (int64_t)kOpp[d] * g.P + row;
             const typename S::T* own_r = fin + (int64_t)d * g.P + row + nx - 1;
-            const typename S::T* dflt = plane + x0 - kCx[d];
+            const typename S::T* dflt = plane + x0;
             const int k = (d == 0) ? 0 : (d == 6 ? 1 : (d == 8 ? 2 : (d == 12 ? 3 : 4)));
             ex[k] = S::load(rl ? own_l : (rr ? own_r : dflt), rl ? kOpp[d] : d);
         }
@@ -740,15 +766,13 @@ __device__ __forceinline__ void bulk_group(const typename S::T* __restrict__ fin
     for (int v = 0; v < V; ++v) any = any || act[v];
     if (!any) return;
     if (nyz == 1) {
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            if (!act[v] || (fixup_flags && (fixup_flags[v] & kFlagFixup))) continue;
-            const int x = x0 + v;
-            if (ylo) face_cell<S, COLL, FORCED, MODE, 1, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
-            else if (yhi) face_cell<S, COLL, FORCED, MODE, 1, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
-            else if (zlo) face_cell<S, COLL, FORCED, MODE, 2, 0>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
-            else face_cell<S, COLL, FORCED, MODE, 2, 1>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
-        }
+        // a row on one y or z face: the whole wave takes the face path
+        if (ylo) face_row<S, COLL, FORCED, MODE, 1, 0>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
+        else if (yhi)
+            face_row<S, COLL, FORCED, MODE, 1, 1>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
+        else if (zlo)
+            face_row<S, COLL, FORCED, MODE, 2, 0>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
+        else face_row<S, COLL, FORCED, MODE, 2, 1>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
         return;
     }
     // interior row; x periodicity is a wave-uniform choice between two

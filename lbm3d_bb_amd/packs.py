@@ -3,10 +3,13 @@
 A pack is 4-32 non-overlapping spheres with radii uniform in [5, 12] cells at
 a 128^3 lattice (both scaled with the lattice size L / 128), centres drawn by
 rejection sampling from one numpy Generator seeded per sample.  The flow case
-is a channel along x: velocity inlet u_in at x = 0, zero-gradient outlet at
-x = L - 1, periodic y and z, D3Q19 TRT with Lambda = 3/16, FP32 populations.
-Its Reynolds number Re = u_in D_ref / nu with D_ref = 2 * 8.5 * L / 128 (the
-mean diameter of the radius range) fixes nu, hence tau+ = 1/2 + 3 nu.
+is a channel along x: velocity inlet u_in = 0.08 at x = 0, zero-gradient
+outlet at x = L - 1, periodic y and z, D3Q19 TRT with Lambda = 3/16, FP32
+populations.  Its Reynolds number Re = u_in D_ref / nu with D_ref =
+2 * 8.5 * L / 128 (the mean diameter of the radius range) fixes nu, hence
+tau+ = 1/2 + 3 nu.  Re is drawn uniformly in [1, 100], capped where nu
+would fall below NU_MIN (tau+ < 0.53, where the scheme is no longer stable
+on these packs): the cap only binds on lattices coarser than 128^3.
 
 The reference solves one body at a time from a uniform start
 (src/Initialization.jl:57-83); packs, the channel faces and the Reynolds
@@ -27,6 +30,8 @@ RE_RANGE = (1.0, 100.0)
 D_REF_128 = 17.0             # 2 x mean radius of RADIUS_RANGE
 LAMBDA = 3.0 / 16.0
 GAP = 2.0                    # minimum surface-to-surface distance (cells)
+U_IN = 0.08
+NU_MIN = 0.01
 X_MARGIN = 0.15              # sphere-free fraction of the domain at the inlet and the outlet
 
 
@@ -83,13 +88,18 @@ def d_ref(L: int) -> float:
     return D_REF_128 * L / 128.0
 
 
-def make_pack(seed: int, L: int = 128, u_in: float = 0.05, re: float | None = None) -> PackCase:
-    """A seeded pack; Re is drawn log-uniformly in [1, 100] unless given."""
+def re_max(L: int, u_in: float = U_IN) -> float:
+    """Largest Reynolds number with nu >= NU_MIN on an L^3 lattice."""
+    return min(RE_RANGE[1], u_in * d_ref(L) / NU_MIN)
+
+
+def make_pack(seed: int, L: int = 128, u_in: float = U_IN, re: float | None = None) -> PackCase:
+    """A seeded pack; Re is drawn uniformly in [1, re_max(L)] unless given."""
     rng = np.random.default_rng(seed)
     s = L / 128.0
     n = int(rng.integers(COUNT_RANGE[0], COUNT_RANGE[1] + 1))
     if re is None:
-        re = float(np.exp(rng.uniform(np.log(RE_RANGE[0]), np.log(RE_RANGE[1]))))
+        re = float(rng.uniform(RE_RANGE[0], re_max(L, u_in)))
     centers, radii = [], []
     tries = 0
     while len(centers) < n:

@@ -259,25 +259,31 @@ def solve_steady(case, net: UNet | None = None, use_net: bool = False, tol: floa
 def generate_dataset(seeds, L: int, log=None, **kw) -> dict:
     """Steady FP32 fields of the packs ``seeds`` from uniform starts: network
     inputs X (n, 2, L, L, L), targets Y (n, 4, L, L, L) and the plain
-    steps-to-converged-drag of every sample."""
+    steps-to-converged-drag of every sample.  Packs whose run does not reach a
+    converged, finite drag (an unsteady wake) are left out of X and Y."""
     from .packs import make_pack
 
+    seeds = list(seeds)
     n = len(seeds)
     X = np.empty((n, 2, L, L, L), dtype=np.float32)
     Y = np.empty((n, 4, L, L, L), dtype=np.float32)
     steps = np.zeros(n, dtype=np.int64)
     conv = np.zeros(n, dtype=bool)
     re = np.zeros(n)
+    k = 0
     for i, sd in enumerate(seeds):
         case = make_pack(int(sd), L)
         r = solve_steady(case, fields=True, **kw)
-        X[i] = encode_input(case.mask(), case.re)
-        Y[i] = encode_target(r["rho"], r["u"], case.u_in, (L, L, L))
-        steps[i], conv[i], re[i] = r["steps"], r["converged"], case.re
+        ok = bool(r["converged"]) and bool(np.all(np.isfinite(r["rho"]))) and bool(np.all(np.isfinite(r["u"])))
+        steps[i], conv[i], re[i] = r["steps"], ok, case.re
+        if ok:
+            X[k] = encode_input(case.mask(), case.re)
+            Y[k] = encode_target(r["rho"], r["u"], case.u_in, (L, L, L))
+            k += 1
         if log:
             log(f"sample {i + 1}/{n} seed {sd} Re {case.re:.2f} spheres {len(case.bodies)} "
-                f"steps {r['steps']} converged {r['converged']}")
-    return {"X": X, "Y": Y, "steps": steps, "converged": conv, "re": re, "seeds": np.asarray(seeds)}
+                f"steps {r['steps']} converged {r['converged']}{'' if ok else ' (left out)'}")
+    return {"X": X[:k], "Y": Y[:k], "steps": steps, "converged": conv, "re": re, "seeds": np.asarray(seeds)}
 
 
 def train(net: UNet, X: np.ndarray, Y: np.ndarray, epochs: int, batch: int, lr: float = 1e-3, seed: int = 0,
