@@ -1,11 +1,15 @@
 """Drag-residual convergence monitor.
 
 The run is converged at step n (a multiple of ``every``) when the relative
-change of the total drag over the last ``every`` steps,
-r(n) = |F(n) - F(n - every)| / |F(n)|, is below ``tol`` for ``patience``
-consecutive checks.  F is the x-force summed over all bodies (the reference's
-C_D numerator, src/MEA.jl:57-59).  The same rule is applied to every start
-state, so a warm start can shorten a run but not change the converged answer.
+change of the window-mean drag, r(n) = |Fm(n) - Fm(n - every)| / |Fm(n)|, is
+below ``tol`` for ``patience`` consecutive checks, where Fm(n) is the mean
+over the ``every`` steps ending at n of F, the x-force summed over all bodies
+(the reference's C_D numerator, src/MEA.jl:57-59).  The window mean filters
+the pressure waves that every start state launches (they bounce between the
+faces with a period of the order of the lattice crossing time), which would
+otherwise reset the count without bearing on the steady drag.  The same rule
+is applied to every start state, so a warm start can shorten a run but not
+change the converged answer.
 """
 from __future__ import annotations
 
@@ -46,7 +50,7 @@ def run_to_convergence(solver, tol: float = 1e-4, every: int = 100, patience: in
     while steps < max_steps:
         F = solver.step(every, forces=True)
         steps += every
-        drag = float(F[-1, :, axis].sum())
+        drag = float(F[:, :, axis].sum(axis=1).mean())
         if not np.isfinite(drag):
             break
         if mon.update(steps, drag):

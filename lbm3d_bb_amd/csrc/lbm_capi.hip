@@ -39,6 +39,7 @@ struct lbm_ctx {
     void* fbuf[2] = {nullptr, nullptr};      // population planes (fbuf_raw + kFrontPad)
     void* fbuf_raw[2] = {nullptr, nullptr};  // allocations
     void* side[2] = {nullptr, nullptr};      // face moments of fbuf[i]'s f* (Grid::face_base slots)
+    double* d_face_u = nullptr;              // [6][3] face velocities
     size_t side_bytes = 0;
     int cur = 0;               // buffer holding f* (pending stream)
     size_t fbytes = 0;         // 19 planes of P elements
@@ -85,7 +86,6 @@ struct lbm_ctx {
 
     // last step (for getters)
     const char* last_rec = nullptr;
-    BulkGeom last_bg{};
     int last_grid = 0;
     int last_face_grid = 0;
     bool last_rec_valid = false;
@@ -356,7 +356,7 @@ int launch_bulk(lbm_ctx* c, int mode, const void* fin, void* fout, double* gout,
 
 // One time step (or its pull-only replay for the getters): nfix fix-up blocks
 // followed by the bulk blocks, in a single launch.
-// The bulk rows' view of a step record (built from the host copy).
+// The bulk rows' view of a step record.
 BulkGeom bulk_geom_of(const StepGeom& sg) {
     BulkGeom bg;
     std::memset(&bg, 0, sizeof(bg));
@@ -380,9 +380,8 @@ BulkGeom bulk_geom_of(const StepGeom& sg) {
 }
 
 int launch_step(lbm_ctx* c, int mode, int nfix, const void* fin, void* fout, double* gout, double* rho, double* u,
-                const char* rec, const BulkGeom& bg, int write_flags, double* partials) {
+                const char* rec, int write_flags, double* partials) {
     LaunchReq r = base_req(c, K_STEP, mode);
-    r.bg = bg;
     r.fin = fin;
     r.fout = fout;
     r.gout = gout;
@@ -605,6 +604,7 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
         bold = bnew;
     }
     c->step_bodies = bnew;
+    sg.bg = bulk_geom_of(sg);
     std::memcpy(rec, &sg, sizeof(sg));
     DevBody* bn = (DevBody*)(rec + sizeof(StepGeom));
     for (int i = 0; i < c->nb; ++i) {
@@ -657,14 +657,12 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
             e1 = next_event(c);
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        const BulkGeom bg = bulk_geom_of(*(const StepGeom*)(c->h_rec[par] + (size_t)s * c->rec_bytes));
-        int lr = launch_step(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, bg, 1,
+        int lr = launch_step(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, 1,
                              c->d_partials + (size_t)s * stride);
         if (lr) return lr;
         if (sample) HIPCHK(c, hipEventRecord(e1, c->stream));
         c->cur = 1 - c->cur;
         c->last_rec = rec;
-        c->last_bg = bg;
         c->last_grid = grids[s];
         c->last_rec_valid = true;
         c->raw_state = false;
@@ -695,7 +693,7 @@ int get_state(lbm_ctx* c, double* d_f, double* d_rho, double* d_u) {
     const void* prev = c->fbuf[1 - c->cur];
     if (c->raw_state) return launch_bulk(c, MODE_GET_RAW, prev, nullptr, d_f, d_rho, d_u, nullptr);
     if (!c->last_rec_valid) return fail(c, LBM_ERR_STATE, "no completed step to read");
-    return launch_step(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, c->last_bg, 0, nullptr);
+    return launch_step(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0, nullptr);
 }
 
 }  // namespace
@@ -733,7 +731,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         if (cfg->faces[2 * a].kind == LBM_FACE_PERIODIC) c->g.per |= 1 << a;
     for (int f = 0; f < 6; ++f) {
         c->g.face_kind[f] = cfg->faces[f].kind;
-        for (int k = 0; k < 3; ++k) c->g.face_u[f][k] = cfg->faces[f].u[k];
+
     }
     {
         // face-moment slots of the velocity / anti-bounce-back faces
@@ -750,6 +748,12 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         }
         const int cbytes = cfg->storage == LBM_FP64 ? 8 : 4;
         c->side_bytes = (size_t)std::max<int64_t>(slots, 1) * 4 * cbytes;
+    }
+    {
+        // 16-bit storage scale 2^12 (FP16S; BF16S shares FP32's exponent range)
+        const bool f16 = cfg->storage == LBM_FP16S;
+        c->g.sh.k = f16 ? 4096.0f : 1.0f;
+        c->g.sh.ik = f16 ? 1.0f / 4096.0f : 1.0f;
     }
     c->g.wall_point = cfg->wall_point;
     c->g.refill = cfg->refill;
@@ -783,6 +787,16 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         rc = dev_alloc(c, &c->side[i], c->side_bytes);
         if (rc == LBM_OK && hipMemsetAsync(c->side[i], 0, c->side_bytes, c->stream) != hipSuccess) rc = LBM_ERR_HIP;
     }
+    if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->d_face_u, 18 * sizeof(double));
+    if (rc == LBM_OK) {
+        double fu[18];
+        for (int f = 0; f < 6; ++f)
+            for (int k = 0; k < 3; ++k) fu[3 * f + k] = cfg->faces[f].u[k];
+        if (hipMemcpyAsync(c->d_face_u, fu, sizeof(fu), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = LBM_ERR_HIP;
+        c->g.face_u = c->d_face_u;
+    }
     if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->flags, (size_t)N);
     if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->list_count, 16);
     if (rc == LBM_OK) {
@@ -812,7 +826,8 @@ int lbm_destroy(lbm_ctx* c) {
         if (c->h_forces[i]) (void)hipHostFree(c->h_forces[i]);
         if (c->rec_ev[i]) (void)hipEventDestroy(c->rec_ev[i]);
     }
-    void* ptrs[] = {c->flags, c->list, c->list_count, c->cnt_a, c->cnt_b, c->d_rec, c->d_partials, c->d_forces};
+    void* ptrs[] = {c->flags, c->list, c->list_count, c->cnt_a, c->cnt_b, c->d_rec, c->d_partials, c->d_forces,
+                    c->d_face_u};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);

@@ -199,13 +199,13 @@ __device__ inline void macros_of(const C f[Q], const Phys<C>& p, C& rho, C u[3])
 template <class S, int COLL, bool FORCED, int MODE>
 __device__ inline void finish_cell(typename S::C f[Q], const Phys<typename S::C>& p, int64_t cell, int64_t N, int64_t P,
                                    typename S::T* __restrict__ fout, double* __restrict__ gout,
-                                   double* __restrict__ rho_out, double* __restrict__ u_out) {
+                                   double* __restrict__ rho_out, double* __restrict__ u_out, const Shift& sh) {
     using C = typename S::C;
     if (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY) {
         C rho, u[3];
         collide<C, COLL, FORCED>(f, p, rho, u);
 #pragma unroll
-        for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * P + cell, d, f[d]);
+        for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * P + cell, d, f[d], sh);
     } else {
         if (gout) {
 #pragma unroll
@@ -281,7 +281,7 @@ __device__ inline bool in_body_box(const StepGeom& sg, int x, int y, int z, cons
     return false;
 }
 
-// Body-region tests of the bulk rows from the by-value BulkGeom (one
+// Body-region tests of the bulk rows from the record's BulkGeom (one
 // conditional wrap: lo is normalised into [0, L) on periodic axes).
 __device__ inline bool in_bulk_box(const BulkGeom& bg, int m, int x, int y, int z, const Grid& g) {
     const int q[3] = {x, y, z};
@@ -350,25 +350,16 @@ __device__ inline int face_kind_at(const Grid& g, int i) {
     return k;
 }
 
-__device__ inline double face_u_at(const Grid& g, int i, int c) {
-    double v = g.face_u[0][c];
-    v = i == 1 ? g.face_u[1][c] : v;
-    v = i == 2 ? g.face_u[2][c] : v;
-    v = i == 3 ? g.face_u[3][c] : v;
-    v = i == 4 ? g.face_u[4][c] : v;
-    v = i == 5 ? g.face_u[5][c] : v;
-    return v;
-}
+__device__ inline double face_u_at(const Grid& g, int i, int c) { return g.face_u[3 * i + c]; }
 
 // Face rules for direction d whose source lies outside face fidx:
 // VELOCITY src/OtherBC.jl:143-148, PRESSURE_ABB src/OtherBC.jl:156-165.
+// uf: the face's wall / inlet velocity (read by the caller ahead of its loads)
 template <typename C>
-__device__ inline C face_value(int kind, int fidx, int d, const Grid& g, C own_opp, C rho_prev, const C up[3],
-                               C uu2) {
+__device__ inline C face_value(int kind, const C uf[3], int d, C own_opp, C rho_prev, const C up[3], C uu2) {
     const C w = weight<C>(d);
     if (kind == LBM_FACE_VELOCITY) {
-        const C cu = C(kCx[d]) * (-C(face_u_at(g, fidx, 0))) + C(kCy[d]) * (-C(face_u_at(g, fidx, 1))) +
-                     C(kCz[d]) * (-C(face_u_at(g, fidx, 2)));
+        const C cu = C(kCx[d]) * (-uf[0]) + C(kCy[d]) * (-uf[1]) + C(kCz[d]) * (-uf[2]);
         return own_opp - C(2) * w * rho_prev * cu * C(3);
     }
     const C cu = C(kCx[d]) * up[0] + C(kCy[d]) * up[1] + C(kCz[d]) * up[2];
@@ -450,7 +441,7 @@ __device__ inline void finish_face_cell(typename S::C f[Q], const Phys<typename 
                                         typename S::T* __restrict__ fout, double* __restrict__ gout,
                                         double* __restrict__ rho_out, double* __restrict__ u_out, int x, int y, int z,
                                         int slot) {
-    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, g.N, g.P, fout, gout, rho_out, u_out);
+    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, g.N, g.P, fout, gout, rho_out, u_out, g.sh);
     if (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY) {
         if (slot >= 0) side_write<S, FORCED>(g, f, p, slot);
         else side_write_all<S, FORCED>(g, f, p, x, y, z);
@@ -478,6 +469,7 @@ __device__ __forceinline__ int face_pull(const typename S::T* __restrict__ fin, 
     const int wzm = (AXIS != 2 && z == 0) ? (int)N : 0, wzp = (AXIS != 2 && z == g.nz - 1) ? -(int)N : 0;
     const int axis_stride = AXIS == 1 ? nx : nxy;
     const bool zg = kind == LBM_FACE_ZERO_GRADIENT;
+    const C uf[3] = {C(face_u_at(g, fidx, 0)), C(face_u_at(g, fidx, 1)), C(face_u_at(g, fidx, 2))};
     // previous-step moments of the cell, from its face slot (face rules only)
     const int slot = zg ? -1 : face_slot(g, fidx, x, y, z);
     C rho_prev, up[3], uu2;
@@ -499,11 +491,11 @@ __device__ __forceinline__ int face_pull(const typename S::T* __restrict__ fin, 
             // either way (an address select, no branch around the load).
             const typename S::T* a = fin + (int64_t)d * g.P + (src + ca * axis_stride);
             const typename S::T* b = fin + (int64_t)kOpp[d] * g.P + cell;
-            const C v = S::load(zg ? a : b, zg ? d : kOpp[d]);
-            const C fv = face_value<C>(kind, fidx, d, g, v, rho_prev, up, uu2);
+            const C v = S::load_pick(zg ? a : b, kOpp[d], d, zg, g.sh);
+            const C fv = face_value<C>(kind, uf, d, v, rho_prev, up, uu2);
             f[d] = zg ? v : fv;
         } else {
-            f[d] = S::load(fin + (int64_t)d * g.P + src, d);
+            f[d] = S::load(fin + (int64_t)d * g.P + src, d, g.sh);
         }
     }
     (void)p;
@@ -546,19 +538,20 @@ __device__ inline void finish_group(typename S::C (&f)[V][Q], const bool (&act)[
         for (int v = 0; v < V; ++v) all = all && act[v];
         if (V == 2 && all) {
 #pragma unroll
-            for (int d = 0; d < Q; ++d) S::store2(fout + (int64_t)d * g.P + cell0, d, f[0][d], f[V - 1][d]);
+            for (int d = 0; d < Q; ++d) S::store2(fout + (int64_t)d * g.P + cell0, d, f[0][d], f[V - 1][d], g.sh);
         } else {
 #pragma unroll
             for (int v = 0; v < V; ++v)
                 if (act[v]) {
 #pragma unroll
-                    for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * g.P + cell0 + v, d, f[v][d]);
+                    for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * g.P + cell0 + v, d, f[v][d], g.sh);
                 }
         }
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v)
-            if (act[v]) finish_cell<S, COLL, FORCED, MODE>(f[v], p, cell0 + v, g.N, g.P, fout, gout, rho_out, u_out);
+            if (act[v])
+                finish_cell<S, COLL, FORCED, MODE>(f[v], p, cell0 + v, g.N, g.P, fout, gout, rho_out, u_out, g.sh);
     }
 }
 
@@ -622,8 +615,14 @@ __device__ __forceinline__ void interior_row(const typename S::T* __restrict__ f
     // face replaces the value)
     const int wxm = left ? (PX ? nx : 1) : 0, wxp = right ? (PX ? -nx : -1) : 0;
     C rho_prev = C(0), up[3] = {C(0), C(0), C(0)}, uu2 = C(0);
+    C uf_lo[3] = {C(0), C(0), C(0)}, uf_hi[3] = {C(0), C(0), C(0)};
     int slot = -1;
     if (XM == XM_RULE) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            uf_lo[c] = C(face_u_at(g, 0, c));
+            uf_hi[c] = C(face_u_at(g, 1, c));
+        }
         // every lane reads a valid slot; only rl / rr lanes use it
         slot = rule_face(g, 0) ? face_slot(g, 0, 0, y, z) : face_slot(g, 1, nx - 1, y, z);
         if (rr && rule_face(g, 0)) slot = face_slot(g, 1, nx - 1, y, z);
@@ -643,23 +642,23 @@ __device__ __forceinline__ void interior_row(const typename S::T* __restrict__ f
             int sx = x0 - kCx[d];
             if (kCx[d] > 0) sx += wxm;
             if (kCx[d] < 0) sx += wxp;
-            f[0][d] = S::load(plane + sx, d);
+            f[0][d] = S::load(plane + sx, d, g.sh);
         } else if (kCx[d] == 0) {
-            S::load2(plane + x0, d, f[0][d], f[V - 1][d]);
+            S::load2(plane + x0, d, f[0][d], f[V - 1][d], g.sh);
         } else if (kCx[d] > 0) {
             // pair (x0 - 1, x0); the left group loads (0, 1) instead, so no
             // load leaves the row: x = -1 is a wrap, a clamp onto x = 0 or a
             // face rule
             C a, b;
-            S::load2(left ? plane + x0 : plane + x0 - 1, d, a, b);
+            S::load2(left ? plane + x0 : plane + x0 - 1, d, a, b, g.sh);
             f[V - 1][d] = left ? a : b;
-            f[0][d] = PX ? S::load(plane + x0 - 1 + wxm, d) : a;  // the same element as a off the left group
+            f[0][d] = PX ? S::load(plane + x0 - 1 + wxm, d, g.sh) : a;  // the same element as a off the left group
         } else {
             // pair (x0 + 1, x0 + 2); the right group loads (nx - 2, nx - 1)
             C a, b;
-            S::load2(right ? plane + x0 : plane + x0 + 1, d, a, b);
+            S::load2(right ? plane + x0 : plane + x0 + 1, d, a, b, g.sh);
             f[0][d] = right ? b : a;
-            f[V - 1][d] = PX ? S::load(plane + x0 + 2 + wxp, d) : b;
+            f[V - 1][d] = PX ? S::load(plane + x0 + 2 + wxp, d, g.sh) : b;
         }
         if (XM == XM_RULE && kCx[d] > 0) {
             // d leaves x = 0 inward (rule at the left face reads f*[opp d] at x = 0);
@@ -669,7 +668,7 @@ __device__ __forceinline__ void interior_row(const typename S::T* __restrict__ f
             const typename S::T* own_r = fin + (int64_t)d * g.P + row + nx - 1;
             const typename S::T* dflt = plane + x0;
             const int k = (d == 0) ? 0 : (d == 6 ? 1 : (d == 8 ? 2 : (d == 12 ? 3 : 4)));
-            ex[k] = S::load(rl ? own_l : (rr ? own_r : dflt), rl ? kOpp[d] : d);
+            ex[k] = S::load_pick(rl ? own_l : (rr ? own_r : dflt), d, kOpp[d], rl, g.sh);
         }
     }
     if (XM == XM_RULE) {
@@ -680,7 +679,7 @@ __device__ __forceinline__ void interior_row(const typename S::T* __restrict__ f
             for (int d = 0; d < Q; ++d) {
                 if (kCx[d] <= 0) continue;
                 const int k = (d == 0) ? 0 : (d == 6 ? 1 : (d == 8 ? 2 : (d == 12 ? 3 : 4)));
-                f[0][d] = face_value<C>(kind, 0, d, g, ex[k], rho_prev, up, uu2);
+                f[0][d] = face_value<C>(kind, uf_lo, d, ex[k], rho_prev, up, uu2);
             }
         }
         if (rr) {
@@ -689,7 +688,7 @@ __device__ __forceinline__ void interior_row(const typename S::T* __restrict__ f
             for (int d = 0; d < Q; ++d) {
                 if (kCx[d] >= 0) continue;
                 const int k = (d == 1) ? 0 : (d == 7 ? 1 : (d == 9 ? 2 : (d == 13 ? 3 : 4)));
-                f[V - 1][d] = face_value<C>(kind, 1, d, g, ex[k], rho_prev, up, uu2);
+                f[V - 1][d] = face_value<C>(kind, uf_hi, d, ex[k], rho_prev, up, uu2);
             }
         }
     }
@@ -707,8 +706,8 @@ template <class S, int COLL, bool FORCED, int MODE>
 __device__ __forceinline__ void bulk_group(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                                            double* __restrict__ gout, double* __restrict__ rho_out,
                                            double* __restrict__ u_out, const Grid& g, const Phys<typename S::C>& p,
-                                           const StepGeom* __restrict__ sgp, const BulkGeom& bg,
-                                           const uint8_t* __restrict__ flags, int x0, int y, int z) {
+                                           const StepGeom* __restrict__ sgp, const uint8_t* __restrict__ flags,
+                                           int x0, int y, int z) {
     using C = typename S::C;
     constexpr int V = S::kVec;  // V == 2 requires an even nx (checked at context creation)
     if (x0 >= g.nx || y >= g.ny) return;
@@ -722,8 +721,8 @@ __device__ __forceinline__ void bulk_group(const typename S::T* __restrict__ fin
         C f[V][Q];
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
-            if (V == 2) S::load2(fin + (int64_t)d * g.P + cell0, d, f[0][d], f[V - 1][d]);
-            else f[0][d] = S::load(fin + (int64_t)d * g.P + cell0, d);
+            if (V == 2) S::load2(fin + (int64_t)d * g.P + cell0, d, f[0][d], f[V - 1][d], g.sh);
+            else f[0][d] = S::load(fin + (int64_t)d * g.P + cell0, d, g.sh);
         }
         finish_group<S, COLL, FORCED, MODE, V>(f, act, p, cell0, g, fout, gout, rho_out, u_out);
         if (MODE == MODE_COLLIDE_ONLY) {
@@ -744,14 +743,17 @@ __device__ __forceinline__ void bulk_group(const typename S::T* __restrict__ fin
         if (left) act[0] = false;
         if (right) act[V - 1] = false;
     }
-    if (bg.mode == 1) {
+    const int bmode = sgp ? sgp->bg.mode : 0, bnb = sgp ? sgp->bg.nbodies : 0;
+    if (bmode == 1) {
+        const BulkGeom& bg = sgp->bg;
         // moving bodies: the fix-up blocks own their whole regions
         if (row_in_body_region(bg, sgp, y, z, g)) {
 #pragma unroll
             for (int v = 0; v < V; ++v)
                 if (cell_in_body_region(bg, sgp, x0 + v, y, z, g)) act[v] = false;
         }
-    } else if (bg.nbodies > 0) {
+    } else if (bnb > 0) {
+        const BulkGeom& bg = sgp->bg;
         // static bodies: boundary nodes carry the fix-up bit (read with the
         // populations, below), face cells of a body region belong to the fix-up
         fixup_flags = flags + cell0;
@@ -775,12 +777,16 @@ __device__ __forceinline__ void bulk_group(const typename S::T* __restrict__ fin
         else face_row<S, COLL, FORCED, MODE, 2, 1>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
         return;
     }
-    // interior row; x periodicity is a wave-uniform choice between two
-    // branch-free copies of the pull
+    // interior row; the x treatment is a wave-uniform choice between
+    // branch-free copies of the pull.  A wave lies in one row (blockDim.x is a
+    // multiple of 64), so its first lane tells whether it holds x = 0 or
+    // x = nx - 1: only those waves pay for the face-rule loads.
+    const int wx0 = __builtin_amdgcn_readfirstlane(x0);
+    const bool wave_left = wx0 == 0, wave_right = wx0 + 64 * V >= nx;
     if (px)
         interior_row<S, COLL, FORCED, MODE, XM_PERIODIC>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0,
                                                          y, z);
-    else if (rule_face(g, 0) || rule_face(g, 1))
+    else if ((wave_left && rule_face(g, 0)) || (wave_right && rule_face(g, 1)))
         interior_row<S, COLL, FORCED, MODE, XM_RULE>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
     else
         interior_row<S, COLL, FORCED, MODE, XM_CLAMP>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y,
@@ -792,8 +798,8 @@ __global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict
                                                  typename S::T* __restrict__ fout, double* __restrict__ gout,
                                                  double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
                                                  Phys<typename S::C> p, const StepGeom* __restrict__ sgp,
-                                                 BulkGeom bg, const uint8_t* __restrict__ flags) {
-    bulk_group<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, bg, flags,
+                                                 const uint8_t* __restrict__ flags) {
+    bulk_group<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, flags,
                                       (blockIdx.x * blockDim.x + threadIdx.x) * S::kVec,
                                       blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z);
 }
@@ -857,9 +863,9 @@ __device__ __forceinline__ void plain_cell(const typename S::T* __restrict__ fin
         if (kCy[d] < 0) src += wyp;
         if (kCz[d] > 0) src += wzm;
         if (kCz[d] < 0) src += wzp;
-        f[d] = S::load(fin + (int64_t)d * g.P + src, d);
+        f[d] = S::load(fin + (int64_t)d * g.P + src, d, g.sh);
     }
-    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out);
+    finish_cell<S, COLL, FORCED, MODE>(f, p, cell, N, g.P, fout, gout, rho_out, u_out, g.sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -898,7 +904,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             const int64_t nc = (int64_t)q[0] + (int64_t)g.nx * q[1] + g.nxy * q[2];
             C own[Q];
 #pragma unroll
-            for (int e = 0; e < Q; ++e) own[e] = S::load(fin + (int64_t)e * g.P + nc, e);
+            for (int e = 0; e < Q; ++e) own[e] = S::load(fin + (int64_t)e * g.P + nc, e, g.sh);
             C r, mx, my, mz;
             moments<C>(own, r, mx, my, mz);
             C ux = mx / r, uy = my / r, uz = mz / r;
@@ -940,7 +946,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
     {
         C own[Q];
 #pragma unroll
-        for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d);
+        for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
         if (!self_solid && sv.nb > 0) {
 #pragma unroll
             for (int d = 0; d < 18; ++d) {
@@ -999,7 +1005,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
         // rule below starts from, so the rules need no further memory access
         const typename S::T* a = fin + (int64_t)d * g.P + (s[0] + g.nx * s[1] + (int)g.nxy * s[2]);
         const typename S::T* b = fin + (int64_t)kOpp[d] * g.P + cell;
-        f[d] = S::load(pull ? a : b, pull ? d : kOpp[d]);
+        f[d] = S::load_pick(pull ? a : b, kOpp[d], d, pull, g.sh);
     }
     __builtin_amdgcn_sched_barrier(0);
     // 2) face and link values; wall point of the moving-wall velocity:
@@ -1010,7 +1016,8 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
         const uint32_t code = (codes[d / 3] >> (10 * (d % 3))) & 0x3ffu;
         const int kind_d = (int)(code & 3u), aux = (int)(code >> 2);
         if (kind_d == DIR_FACE) {
-            f[d] = face_value<C>(face_kind_at(g, aux), aux, d, g, f[d], rho_prev, up, uu2);
+            const C uf[3] = {C(face_u_at(g, aux, 0)), C(face_u_at(g, aux, 1)), C(face_u_at(g, aux, 2))};
+            f[d] = face_value<C>(face_kind_at(g, aux), uf, d, f[d], rho_prev, up, uu2);
         } else if (kind_d == DIR_LINK) {
             // boundary link (x, dl), dl = opp(d): the solid node is x + c_dl
             const int dl = kOpp[d];
@@ -1277,7 +1284,7 @@ __global__ __launch_bounds__(kBlock, LBM_STEP_MIN_WAVES) void k_step(const typen
                                                  uint8_t* __restrict__ flags, const int32_t* __restrict__ list,
                                                  const int32_t* __restrict__ list_count,
                                                  double* __restrict__ partials, int write_flags, BulkShape bs,
-                                                 BulkGeom bg, int nfix) {
+                                                 int nfix) {
     extern __shared__ double lds_dyn[];
     const int blk = blockIdx.x;
     if (blk < nfix) {
@@ -1291,7 +1298,7 @@ __global__ __launch_bounds__(kBlock, LBM_STEP_MIN_WAVES) void k_step(const typen
     const int byi = r % bs.gy;
     const int bz = r / bs.gy;
     const int tx = threadIdx.x % bs.bx, ty = threadIdx.x / bs.bx;
-    bulk_group<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, bg, flags,
+    bulk_group<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sgp, flags,
                                       (bxi * bs.bx + tx) * S::kVec,
                                       byi * bs.by + ty, bz);
 }
@@ -1304,7 +1311,7 @@ __global__ __launch_bounds__(kBlock, LBM_STEP_MIN_WAVES) void k_step(const typen
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_equilibrium(typename S::T* __restrict__ f, const double* __restrict__ rho,
                                                         const double* __restrict__ u, double rho0, double u0x,
-                                                        double u0y, double u0z, int64_t N, int64_t P) {
+                                                        double u0y, double u0z, int64_t N, int64_t P, Shift sh) {
     using C = typename S::C;
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (cell >= N) return;
@@ -1317,17 +1324,17 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(typename S::T* __restric
     for (int d = 0; d < Q; ++d) {
         const C cu = C(kCx[d]) * ux + C(kCy[d]) * uy + C(kCz[d]) * uz;
         const C feq = weight<C>(d) * r * (C(1) + cu * C(3) + C(0.5) * cu * cu * C(9) - C(0.5) * u2 * C(3));
-        S::store(f + (int64_t)d * P + cell, d, feq);
+        S::store(f + (int64_t)d * P + cell, d, feq, sh);
     }
 }
 
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_from_double(typename S::T* __restrict__ f, const double* __restrict__ src,
-                                                        int64_t N, int64_t P) {
+                                                        int64_t N, int64_t P, Shift sh) {
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (cell >= N) return;
 #pragma unroll
-    for (int d = 0; d < Q; ++d) S::store(f + (int64_t)d * P + cell, d, (typename S::C)src[(int64_t)d * N + cell]);
+    for (int d = 0; d < Q; ++d) S::store(f + (int64_t)d * P + cell, d, (typename S::C)src[(int64_t)d * N + cell], sh);
 }
 
 }  // namespace lbm

@@ -8,6 +8,8 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+#include "lbm_types.h"
+
 namespace lbm {
 
 constexpr int Q = 19;
@@ -30,9 +32,11 @@ __host__ __device__ constexpr C weight(int d) {
 
 // ---------------------------------------------------------------------------
 // Storage formats.  Each exposes the stored type T, the compute type C and
-// load/store of one population.  The shifted 16-bit formats keep f_d - w_d (the
-// deviation from the rest state at rho0 = 1), so the 16-bit mantissa spends its
-// bits on the O(Ma) deviation instead of on w_d itself.
+// load/store of one population.  The shifted 16-bit formats keep
+// (f_d - w_d) k: the deviation from the rest state at rho0 = 1, scaled by a
+// power of two (Shift), so the 16-bit mantissa spends its bits on the
+// deviation instead of on w_d itself and small deviations stay out of FP16's
+// subnormal range.
 // ---------------------------------------------------------------------------
 //
 // kVec is the number of consecutive x cells one thread of the bulk pass owns:
@@ -44,10 +48,17 @@ struct StoreF64 {
     using C = double;
     static constexpr int kBytes = 8;
     static constexpr int kVec = 1;
-    __device__ static inline C load(const T* p, int d) { (void)d; return *p; }
-    __device__ static inline void store(T* p, int d, C v) { (void)d; *p = v; }
-    __device__ static inline void load2(const T* p, int d, C& a, C& b) { a = load(p, d); b = load(p + 1, d); }
-    __device__ static inline void store2(T* p, int d, C a, C b) { store(p, d, a); store(p + 1, d, b); }
+    __device__ static inline C load(const T* p, int, const Shift&) { return *p; }
+    __device__ static inline C load_pick(const T* p, int, int, bool, const Shift&) { return *p; }
+    __device__ static inline void store(T* p, int, C v, const Shift&) { *p = v; }
+    __device__ static inline void load2(const T* p, int d, C& a, C& b, const Shift& sh) {
+        a = load(p, d, sh);
+        b = load(p + 1, d, sh);
+    }
+    __device__ static inline void store2(T* p, int d, C a, C b, const Shift& sh) {
+        store(p, d, a, sh);
+        store(p + 1, d, b, sh);
+    }
 };
 
 struct StoreF32 {
@@ -55,11 +66,18 @@ struct StoreF32 {
     using C = float;
     static constexpr int kBytes = 4;
     static constexpr int kVec = 1;
-    __device__ static inline C load(const T* p, int d) { (void)d; return *p; }
+    __device__ static inline C load(const T* p, int, const Shift&) { return *p; }
+    __device__ static inline C load_pick(const T* p, int, int, bool, const Shift&) { return *p; }
     // populations are written once per step and read by the next: stream them
-    __device__ static inline void store(T* p, int d, C v) { (void)d; __builtin_nontemporal_store(v, p); }
-    __device__ static inline void load2(const T* p, int d, C& a, C& b) { a = load(p, d); b = load(p + 1, d); }
-    __device__ static inline void store2(T* p, int d, C a, C b) { store(p, d, a); store(p + 1, d, b); }
+    __device__ static inline void store(T* p, int, C v, const Shift&) { __builtin_nontemporal_store(v, p); }
+    __device__ static inline void load2(const T* p, int d, C& a, C& b, const Shift& sh) {
+        a = load(p, d, sh);
+        b = load(p + 1, d, sh);
+    }
+    __device__ static inline void store2(T* p, int d, C a, C b, const Shift& sh) {
+        store(p, d, a, sh);
+        store(p + 1, d, b, sh);
+    }
 };
 
 struct StoreF16S {
@@ -67,22 +85,29 @@ struct StoreF16S {
     using C = float;
     static constexpr int kBytes = 2;
     static constexpr int kVec = 2;
-    __device__ static inline C load(const T* p, int d) { return __half2float(*p) + weight<float>(d); }
+    __device__ static inline C load(const T* p, int d, const Shift& sh) {
+        return __fmaf_rn(__half2float(*p), sh.ik, weight<float>(d));
+    }
+    // direction d1 if `second`, else d0 (both compile-time: the reference
+    // value is a select, not an indexed read)
+    __device__ static inline C load_pick(const T* p, int d0, int d1, bool second, const Shift& sh) {
+        return __fmaf_rn(__half2float(*p), sh.ik, second ? weight<float>(d1) : weight<float>(d0));
+    }
     // __float2half_rn: round to nearest even (not the round-toward-zero of the
     // packed v_cvt_pkrtz form, which would bias the deviations).
-    __device__ static inline void store(T* p, int d, C v) {
-        const __half h = __float2half_rn(v - weight<float>(d));
-        __builtin_nontemporal_store(*reinterpret_cast<const unsigned short*>(&h), reinterpret_cast<unsigned short*>(p));
+    __device__ static inline void store(T* p, int d, C v, const Shift& sh) {
+        const __half h = __float2half_rn((v - weight<float>(d)) * sh.k);
+        __builtin_nontemporal_store(__half_as_ushort(h), reinterpret_cast<unsigned short*>(p));
     }
-    __device__ static inline void load2(const T* p, int d, C& a, C& b) {
+    __device__ static inline void load2(const T* p, int d, C& a, C& b, const Shift& sh) {
         uint32_t v;
         __builtin_memcpy(&v, p, 4);
-        a = __half2float(__ushort_as_half((unsigned short)(v & 0xffffu))) + weight<float>(d);
-        b = __half2float(__ushort_as_half((unsigned short)(v >> 16))) + weight<float>(d);
+        a = __fmaf_rn(__half2float(__ushort_as_half((unsigned short)(v & 0xffffu))), sh.ik, weight<float>(d));
+        b = __fmaf_rn(__half2float(__ushort_as_half((unsigned short)(v >> 16))), sh.ik, weight<float>(d));
     }
-    __device__ static inline void store2(T* p, int d, C a, C b) {
-        const uint32_t lo = __half_as_ushort(__float2half_rn(a - weight<float>(d)));
-        const uint32_t hi = __half_as_ushort(__float2half_rn(b - weight<float>(d)));
+    __device__ static inline void store2(T* p, int d, C a, C b, const Shift& sh) {
+        const uint32_t lo = __half_as_ushort(__float2half_rn((a - weight<float>(d)) * sh.k));
+        const uint32_t hi = __half_as_ushort(__float2half_rn((b - weight<float>(d)) * sh.k));
         __builtin_nontemporal_store(lo | (hi << 16), reinterpret_cast<uint32_t*>(p));
     }
 };
@@ -99,17 +124,20 @@ struct StoreBF16S {
         u += 0x7fffu + ((u >> 16) & 1u);
         return uint16_t(u >> 16);
     }
-    __device__ static inline C load(const T* p, int d) { return bf16_to_f32(*p) + weight<float>(d); }
-    __device__ static inline void store(T* p, int d, C v) {
+    __device__ static inline C load(const T* p, int d, const Shift& sh) { return bf16_to_f32(*p) + weight<float>(d); }
+    __device__ static inline C load_pick(const T* p, int d0, int d1, bool second, const Shift& sh) {
+        return bf16_to_f32(*p) + (second ? weight<float>(d1) : weight<float>(d0));
+    }
+    __device__ static inline void store(T* p, int d, C v, const Shift& sh) {
         __builtin_nontemporal_store(f32_to_bf16_rne(v - weight<float>(d)), p);
     }
-    __device__ static inline void load2(const T* p, int d, C& a, C& b) {
+    __device__ static inline void load2(const T* p, int d, C& a, C& b, const Shift& sh) {
         uint32_t v;
         __builtin_memcpy(&v, p, 4);
         a = __uint_as_float(v << 16) + weight<float>(d);
         b = __uint_as_float(v & 0xffff0000u) + weight<float>(d);
     }
-    __device__ static inline void store2(T* p, int d, C a, C b) {
+    __device__ static inline void store2(T* p, int d, C a, C b, const Shift& sh) {
         const uint32_t lo = f32_to_bf16_rne(a - weight<float>(d));
         const uint32_t hi = f32_to_bf16_rne(b - weight<float>(d));
         __builtin_nontemporal_store(lo | (hi << 16), reinterpret_cast<uint32_t*>(p));

@@ -11,7 +11,7 @@ static void launch_bulk_t(const LaunchReq& r, const Phys<typename S::C>& p) {
     using T = typename S::T;
     const BulkShape bs = bulk_shape<S>(r.g);
     hipLaunchKernelGGL((k_bulk<S, COLL, FORCED, MODE>), dim3(bs.gx, bs.gy, r.g.nz), dim3(bs.bx, bs.by), 0, r.stream,
-                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, (const StepGeom*)r.rec, r.bg,
+                       (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, (const StepGeom*)r.rec,
                        (const uint8_t*)r.flags);
 }
 
@@ -25,7 +25,7 @@ static void launch_step_t(const LaunchReq& r, const Phys<typename S::C>& p) {
     const int64_t nbulk = (int64_t)bs.gx * bs.gy * r.g.nz;
     hipLaunchKernelGGL((k_step<S, COLL, FORCED, MODE>), dim3((unsigned)(r.grid_blocks + nbulk)), dim3(kBlock),
                        (unsigned)step_lds_bytes(r.nb), r.stream, (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, sg, bn, bo, r.flags,
-                       r.list, r.list_count, r.partials, r.write_flags, bs, r.bg, r.grid_blocks);
+                       r.list, r.list_count, r.partials, r.write_flags, bs, r.grid_blocks);
 }
 
 template <class S, int MODE>
@@ -62,10 +62,10 @@ int launch(const LaunchReq& r) {
             break;
         case K_EQUILIBRIUM:
             hipLaunchKernelGGL(k_equilibrium<S>, grid, block, 0, r.stream, (T*)r.fout, r.src_rho, r.src_u, r.rho0,
-                               r.u0[0], r.u0[1], r.u0[2], r.g.N, r.g.P);
+                               r.u0[0], r.u0[1], r.u0[2], r.g.N, r.g.P, r.g.sh);
             break;
         case K_FROM_DOUBLE:
-            hipLaunchKernelGGL(k_from_double<S>, grid, block, 0, r.stream, (T*)r.fout, r.src_f, r.g.N, r.g.P);
+            hipLaunchKernelGGL(k_from_double<S>, grid, block, 0, r.stream, (T*)r.fout, r.src_f, r.g.N, r.g.P, r.g.sh);
             break;
         default: return (int)hipErrorInvalidValue;
     }

@@ -47,6 +47,15 @@ struct Box {
     int ovl;   // 1: may share cells with an earlier fix-up box (cells there belong to the first)
 };
 
+// Scale of the 16-bit population formats: they store (f_d - w_d) k, the
+// deviation from the rest state w_d (rho0 = 1) times a power of two k that
+// keeps small deviations out of FP16's subnormal range (exact scaling, one
+// rounding as before).
+struct Shift {
+    float k;
+    float ik;  // 1 / k
+};
+
 struct Grid {
     int nx, ny, nz;
     int per;  // bit a: axis a periodic
@@ -54,7 +63,7 @@ struct Grid {
     int64_t P;  // population plane stride (elements): N plus a pad that staggers the planes' channel mapping
     int64_t nxy;
     int face_kind[6];
-    double face_u[6][3];
+    const double* face_u;  // device [6][3]: wall / inlet velocity of each face
     int wall_point;
     int refill;
     double rho0;
@@ -63,6 +72,7 @@ struct Grid {
     // written when the cell's f* is produced.  face_base[f] is the first slot
     // of face f (-1: periodic or zero-gradient face, no slots).
     int face_base[6];
+    Shift sh;        // 16-bit storage reference state
     void* side_in;   // per launch: moments of the face cells of fin's f*
     void* side_out;  // per launch: the same for fout
 };
@@ -71,6 +81,18 @@ template <typename C>
 struct Phys {
     C wp, wm;
     C g[3];
+};
+
+// What the bulk rows need of a step's geometry, in one compact block of the
+// step record: the mode, the body count and up to kBulkBoxes body regions
+// (read with one batch of scalar loads instead of a walk over the boxes).
+constexpr int kBulkBoxes = 8;
+struct BulkGeom {
+    int mode;     // 0: static (fix-up bit in the flags), 1: moving (analytic regions)
+    int nbodies;
+    int nbox;     // body regions listed below; -1: more than kBulkBoxes (read the step record)
+    int lo[3][kBulkBoxes];   // lo in [0, L) on periodic axes
+    int ext[3][kBulkBoxes];
 };
 
 // Geometry of one step for the fix-up blocks of k_step: domain edges and
@@ -85,19 +107,8 @@ struct StepGeom {
     int mode;          // 0: solidity from flags (static), 1: analytic (moving bodies)
     int list_blocks;   // k_fixup blocks over the static boundary-node list (mode 0)
     int pad;
+    BulkGeom bg;
     Box box[kMaxBoxes];
-};
-
-// What the bulk rows need of a step's geometry, passed by value with the
-// kernel arguments so a bulk wave reads it without a dependent load from the
-// step record: the mode, the body count and up to kBulkBoxes body regions.
-constexpr int kBulkBoxes = 8;
-struct BulkGeom {
-    int mode;     // 0: static (fix-up bit in the flags), 1: moving (analytic regions)
-    int nbodies;
-    int nbox;     // body regions listed below; -1: more than kBulkBoxes (read the step record)
-    int lo[3][kBulkBoxes];   // lo in [0, L) on periodic axes
-    int ext[3][kBulkBoxes];
 };
 
 enum { MODE_STEP = 0, MODE_GET = 1, MODE_COLLIDE_ONLY = 2, MODE_GET_RAW = 3 };
@@ -119,7 +130,6 @@ struct LaunchReq {
     double* gout;
     double* rho_out;
     double* u_out;
-    BulkGeom bg;
     // fix-up
     int grid_blocks;
     const char* rec;

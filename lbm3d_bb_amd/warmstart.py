@@ -184,6 +184,44 @@ def decode_output(y: np.ndarray, u_ref: float) -> tuple[np.ndarray, np.ndarray]:
     return rho, u
 
 
+def _grad(a: np.ndarray, axis: int, periodic: bool) -> np.ndarray:
+    """Central difference along an axis of a (nz, ny, nx) array: wraps on a
+    periodic axis, one-sided at the two ends otherwise."""
+    if periodic:
+        return 0.5 * (np.roll(a, -1, axis=axis) - np.roll(a, 1, axis=axis))
+    return np.gradient(a, axis=axis)
+
+
+def populations_from_fields(rho: np.ndarray, u: np.ndarray, shape, tau_plus: float,
+                            periodic=(False, False, False)) -> np.ndarray:
+    """f = f_eq(rho, u) + f_neq with the first-order Chapman-Enskog
+    non-equilibrium part f_neq_i = -3 tau+ w_i rho Q_i : grad(u),
+    Q_i = c_i c_i - I/3, gradients by central differences.  Starting from
+    f_eq alone leaves the viscous stress of the field to build up again (a
+    transient that the drag residual has to wait out); f_neq carries it.
+    shape = (nz, ny, nx); rho (N,), u (3, N); returns (19, N)."""
+    from .lattice import CX, CY, CZ, W, equilibrium
+
+    nz, ny, nx = shape
+    rho = np.asarray(rho, dtype=np.float64).reshape(-1)
+    u = np.asarray(u, dtype=np.float64).reshape(3, -1)
+    f = equilibrium(rho, u[0], u[1], u[2])
+    U = u.reshape(3, nz, ny, nx)
+    # grad[a][b] = d u_b / d x_a ; array axes: 0 = z, 1 = y, 2 = x
+    ax_of = {0: 2, 1: 1, 2: 0}
+    grad = [[_grad(U[b], ax_of[a], periodic[a]).reshape(-1) for b in range(3)] for a in range(3)]
+    C = np.stack([CX, CY, CZ], axis=1).astype(np.float64)
+    for d in range(19):
+        qs = np.zeros_like(rho)
+        for a in range(3):
+            for b in range(3):
+                q = C[d, a] * C[d, b] - (1.0 / 3.0 if a == b else 0.0)
+                if q != 0.0:
+                    qs += q * grad[a][b]
+        f[d] += -3.0 * tau_plus * W[d] * rho * qs
+    return f
+
+
 # --------------------------------------------------------------------------
 # Warm start with fallback, steady solves, data, training, evaluation
 # --------------------------------------------------------------------------
@@ -213,14 +251,17 @@ def predict(net: UNet, case) -> tuple[np.ndarray, np.ndarray]:
 
 
 def warm_start(solver, net: UNet | None, case) -> dict:
-    """Initialises ``solver`` at equilibrium of the network's (rho, u), or of
-    the uniform state when there is no network or its prediction is not
-    plausible.  Returns which start was taken and why."""
+    """Initialises ``solver`` from the network's (rho, u): equilibrium plus
+    the first-order non-equilibrium part of that field
+    (populations_from_fields), or at the uniform state when there is no
+    network or its prediction is not plausible.  Returns which start was
+    taken and why."""
     if net is not None:
         rho, u = predict(net, case)
         ok, why = plausible(rho, u, case.u_in)
         if ok:
-            solver.init_equilibrium(rho, u)
+            f = populations_from_fields(rho, u, (case.L,) * 3, case.tau_plus, periodic=(False, True, True))
+            solver.set_populations(f)
             return {"start": "net", "reason": why}
     else:
         why = "no network"
@@ -286,10 +327,32 @@ def generate_dataset(seeds, L: int, log=None, **kw) -> dict:
     return {"X": X[:k], "Y": Y[:k], "steps": steps, "converged": conv, "re": re, "seeds": np.asarray(seeds)}
 
 
+def augment(x: np.ndarray, y: np.ndarray, rng) -> tuple[np.ndarray, np.ndarray]:
+    """An exact symmetry of the pack flow applied to one sample (x (2, z, y, x),
+    y (4, z, y, x)): a periodic shift in y and z, reflections of y and z (the
+    velocity component flips sign) and the y <-> z exchange."""
+    sy, sz = (int(v) for v in rng.integers(0, x.shape[2], size=2))
+    x = np.roll(x, (sz, sy), axis=(1, 2))
+    y = np.roll(y, (sz, sy), axis=(1, 2))
+    if rng.random() < 0.5:  # y -> -y
+        x = x[:, :, ::-1, :]
+        y = y[:, :, ::-1, :].copy()
+        y[2] = -y[2]
+    if rng.random() < 0.5:  # z -> -z
+        x = x[:, ::-1, :, :]
+        y = y[:, ::-1, :, :].copy()
+        y[3] = -y[3]
+    if rng.random() < 0.5:  # y <-> z
+        x = np.swapaxes(x, 1, 2)
+        y = np.swapaxes(y, 1, 2)[[0, 1, 3, 2]]
+    return np.ascontiguousarray(x), np.ascontiguousarray(y)
+
+
 def train(net: UNet, X: np.ndarray, Y: np.ndarray, epochs: int, batch: int, lr: float = 1e-3, seed: int = 0,
-          log=None) -> list[float]:
+          log=None, augment_data: bool = True) -> list[float]:
     """Adam on the mean-squared error with a cosine learning-rate decay; the
-    samples are reshuffled every epoch.  Returns the mean loss per epoch."""
+    samples are reshuffled every epoch and, by default, each draw is a random
+    symmetry image of its sample (augment).  Returns the mean loss per epoch."""
     rng = np.random.default_rng(seed)
     n = X.shape[0]
     per_epoch = max(n // batch, 1)
@@ -301,8 +364,13 @@ def train(net: UNet, X: np.ndarray, Y: np.ndarray, epochs: int, batch: int, lr: 
         losses = []
         for k in range(per_epoch):
             idx = np.sort(order[k * batch:(k + 1) * batch])
+            xb, yb = X[idx], Y[idx]
+            if augment_data:
+                pairs = [augment(X[i], Y[i], rng) for i in idx]
+                xb = np.stack([p[0] for p in pairs])
+                yb = np.stack([p[1] for p in pairs])
             net.set_lr(lr * 0.5 * (1.0 + np.cos(np.pi * t / total)) + 1e-6)
-            losses.append(net.train_step(X[idx], Y[idx]))
+            losses.append(net.train_step(xb, yb))
             t += 1
         hist.append(float(np.mean(losses)))
         if log:

@@ -23,6 +23,7 @@ import time
 
 import numpy as np
 
+from .packs import U_IN, re_max
 from .warmstart import UNet, evaluate, generate_dataset, train
 
 HELD_OUT_BASE = 1_000_000
@@ -60,14 +61,17 @@ def main(argv=None) -> int:
     t_train = time.perf_counter() - t0
     ms, flops = net.last_timing()
     net.save(os.path.join(args.out, "unet_params.npy"))
+    used = int(data["X"].shape[0])
     del data
     t0 = time.perf_counter()
     ev = evaluate(net, range(HELD_OUT_BASE, HELD_OUT_BASE + args.eval), L, log=log, **kw)
     t_eval = time.perf_counter() - t0
     summary = {
-        "workload": f"{L}^3 sphere packs (4-32 spheres, r in [5,12]*{L}/128), Re log-uniform in [1,100], "
-                    "inlet u=0.05, zero-gradient outlet, periodic y/z, TRT 3/16, FP32",
-        "train_samples": args.train, "eval_samples": args.eval, "epochs": args.epochs, "batch": args.batch,
+        "workload": f"{L}^3 sphere packs (4-32 spheres, r in [5,12]*{L}/128), Re uniform in "
+                    f"[1,{re_max(L):.0f}], inlet u={U_IN}, zero-gradient outlet, periodic y/z, TRT 3/16, FP32; "
+                    f"drag residual {args.tol} over {args.every}-step window means, 3 in a row",
+        "train_samples": args.train, "train_samples_used": used, "eval_samples": args.eval, "epochs": args.epochs,
+        "batch": args.batch,
         "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
         "train_step_ms": ms, "train_step_tflops": flops / ms / 1e9 if ms > 0 else None,
         "seconds": {"generate": t_gen, "train": t_train, "evaluate": t_eval},

@@ -13,14 +13,14 @@ def trt_omegas(tau_p: float, lam: float = 3.0 / 16.0) -> tuple[float, float]:
 
 
 def periodic_box(n=24, storage="fp64", collision="BGK", tau=0.8, g=(1e-5, 0.0, 0.0)):
-    """Config-1 shape: periodic box, BGK tau=0.8, body force along x."""
+    """Periodic box, BGK tau=0.8 (or TRT), body force along x."""
     wp, wm = trt_omegas(tau)
     return Config(n, n, n, collision=collision, storage=storage, omega_plus=1.0 / tau if collision == "BGK" else wp,
                   omega_minus=wm, faces=[periodic()] * 6, g=g)
 
 
 def channel(nx=40, ny=20, nz=20, storage="fp32", u_in=0.05, outlet="zero_gradient", tau=0.56, refill=True):
-    """Config-2 shape: inlet velocity at x=0, outlet at x=Nx-1, no-slip y/z walls, TRT 3/16."""
+    """Channel: inlet velocity at x=0, outlet at x=Nx-1, no-slip y/z walls, TRT 3/16."""
     wp, wm = trt_omegas(tau)
     out = zero_gradient() if outlet == "zero_gradient" else pressure()
     return Config(nx, ny, nz, collision="TRT", storage=storage, omega_plus=wp, omega_minus=wm,

@@ -78,7 +78,7 @@ def test_initial_state_matches():
 
 @pytest.mark.parametrize("collision", ["BGK", "TRT"])
 def test_periodic_sphere_fp64(collision):
-    """Config-1 shape at 24^3: fixed sphere, body force, halfway bounce-back, FP64."""
+    """Periodic box with one fixed sphere (the 64^3 BGK sphere case at 24^3): body force, halfway bounce-back, FP64."""
     cfg = periodic_box(n=24, collision=collision)
     g, o = _pair(cfg, "fp64")
     for s in (g, o):
@@ -105,7 +105,7 @@ def test_multi_body_static_fp64():
 
 
 def test_channel_translating_sphere_fp64():
-    """Config-2 shape: inlet, zero-gradient outlet, no-slip walls, moving sphere, refill."""
+    """Channel (inlet, zero-gradient outlet, no-slip walls) with a translating sphere and refill."""
     cfg = channel(nx=40, ny=20, nz=20, storage="fp64")
     g, o = _pair(cfg, "fp64")
     law = ConstantMotion("sphere", (12.3, 10.1, 9.8), (4.0,), vel=(0.05, 0.0, 0.0))
@@ -223,3 +223,53 @@ def test_mass_conserved_periodic_no_body():
     assert abs(m1 - m0) / m0 < 1e-13
     # momentum grows by exactly sum(F) = M g per step
     assert abs((p1[0] - p0[0]) - 100 * m0 * 1e-4) < 1e-10 * m0
+
+
+def _mean_drag(F, last=100):
+    return np.asarray(F)[-last:, :, 0].mean(axis=0)
+
+
+def test_shifted_storage_channel_drag_vs_fp64():
+    """Channel shape (inlet, zero-gradient outlet, no-slip walls) with a
+    translating sphere: drag of the 16-bit storages against the FP64 oracle,
+    averaged over the last 100 of 1000 steps."""
+    law = ConstantMotion("sphere", (20.0, 16.0, 16.0), (5.0,), vel=(0.01, 0.0, 0.0))
+    o = OracleSolver(channel(nx=64, ny=32, nz=32, storage="fp64", tau=0.6), "fp64")
+    o.set_motion([law])
+    o.init_equilibrium()
+    ref = _mean_drag(o.step(1000))
+    o.close()
+    for storage, tol in (("fp16s", 5e-3), ("bf16s", 5e-2)):
+        g = Solver(channel(nx=64, ny=32, nz=32, storage=storage, tau=0.6))
+        g.set_motion([law])
+        g.init_equilibrium()
+        d = _mean_drag(g.step(1000))
+        g.close()
+        err = float(abs(d[0] - ref[0]) / abs(ref[0]))
+        print(storage, "channel drag error vs FP64", err)
+        assert err < tol, (storage, err)
+
+
+def test_shifted_storage_per_sphere_drag_vs_fp64():
+    """Periodic box with a seeded sphere pack driven by a body force (the
+    shape of the 64-sphere benchmark, 48^3 with 6 spheres): per-sphere drag
+    of FP32, FP16S and BF16S storage against the FP64 oracle."""
+    bodies = sphere_pack(6, 48, 3.0, 5.0, seed=0)
+
+    def cfg(storage):
+        return periodic_box(n=48, collision="TRT", storage=storage, tau=0.6, g=(2e-5, 0.0, 0.0))
+
+    o = OracleSolver(cfg("fp64"), "fp64")
+    o.set_bodies(bodies)
+    o.init_equilibrium()
+    ref = _mean_drag(o.step(1500))
+    o.close()
+    for storage, tol in (("fp32", 1e-4), ("fp16s", 5e-3), ("bf16s", 5e-2)):
+        g = Solver(cfg(storage))
+        g.set_bodies(bodies)
+        g.init_equilibrium()
+        d = _mean_drag(g.step(1500))
+        g.close()
+        err = float(np.abs(d - ref).max() / np.abs(ref).max())
+        print(storage, "per-sphere drag error vs FP64", err)
+        assert err < tol, (storage, err)
