@@ -44,11 +44,18 @@ def kernel_stats(directory: str) -> list[dict]:
 
 
 def counter_means(directory: str, counter: str) -> dict[str, tuple[float, int]]:
-    """Mean counter value per dispatch, by kernel name: {name: (mean, dispatches)}."""
+    """Mean counter value per dispatch, by kernel name: {name: (mean, dispatches)}.
+    The copy kernel is keyed by its largest grid only: the probe's copies are
+    the largest the process runs (smaller ones are the library's own
+    host/device transfers)."""
     rows = _rows(directory, "counter_collection.csv")
     acc: dict[str, list[float]] = defaultdict(list)
+    copy_grid = max((int(r["Grid_Size"]) for r in rows
+                     if r.get("Counter_Name") == counter and COPY_KERNEL in r["Kernel_Name"]), default=-1)
     for r in rows:
         if r.get("Counter_Name") != counter:
+            continue
+        if COPY_KERNEL in r["Kernel_Name"] and int(r["Grid_Size"]) != copy_grid:
             continue
         acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
