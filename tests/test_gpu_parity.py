@@ -273,3 +273,21 @@ def test_shifted_storage_per_sphere_drag_vs_fp64():
         err = float(np.abs(d - ref).max() / np.abs(ref).max())
         print(storage, "per-sphere drag error vs FP64", err)
         assert err < tol, (storage, err)
+
+
+def test_periodic_box_64_bgk_sphere_full_size_fp64():
+    """The reference-runnable case at its full size: 64^3 periodic box, BGK
+    tau = 0.8, FP64, body force g = 1e-6 along x, one fixed sphere of radius 8
+    centred at (32, 32, 32), 1000 steps; the drag every 10 steps and the final
+    state against the FP64 oracle."""
+    cfg = periodic_box(n=64, collision="BGK", storage="fp64", tau=0.8, g=(1e-6, 0.0, 0.0))
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies([sphere((32.0, 32.0, 32.0), 8.0)])
+        s.init_equilibrium()
+    Fg = g.step(1000)
+    Fo = o.step(1000)
+    assert np.abs(Fo[9::10, 0, 0]).min() > 0
+    err = force_err(Fg[9::10], Fo[9::10], force_scale(o))
+    assert err < TOL["fp64"], err
+    _compare_state(g, o, TOL["fp64"], "box64")
