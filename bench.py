@@ -46,7 +46,12 @@ _DT = {"fp64": "f64", "fp32": "f32", "fp16s": "f16 storage, f32 compute", "bf16s
 # force with 64 fixed spheres (radius uniform in [6, 10], centres by rejection
 # sampling from seed 0); channel1024_ellipsoids8: 1024x384x384 channel in
 # shifted FP16 with 8 ellipsoids (16, 8, 8) at random orientations, each
-# rotating at 1e-3 rad/step about a random axis.
+# rotating at 1e-3 rad/step about a random axis; refcase_*: the reference's
+# own input file (input/params.txt: 1201x1201x3, D3Q19 BGK at Re 185, velocity
+# inlet and y walls at U = 0.075, anti-bounce-back outlet, z periodic, a
+# cylinder R = 16.16 heaving with the reference's law) with its interpolated
+# bounce-back (IBB), the simple one (SBB), or the NACA 0012 airfoil of chord 120
+# (IBB).
 WORKLOADS = {
     "channel256": dict(case="channel", nx=256, ny=128, nz=128, storage="fp32"),
     "channel256_fp16s": dict(case="channel", nx=256, ny=128, nz=128, storage="fp16s"),
@@ -55,6 +60,9 @@ WORKLOADS = {
     "box512_spheres64_fp16s": dict(case="box", nx=512, ny=256, nz=256, storage="fp16s"),
     "box512_spheres64_fp64": dict(case="box", nx=512, ny=256, nz=256, storage="fp64"),
     "channel1024_ellipsoids8": dict(case="ellipsoids", nx=1024, ny=384, nz=384, storage="fp16s"),
+    "refcase_cyl_ibb": dict(case="refparams", shape="CYL", bc="IBB", nx=1201, ny=1201, nz=3, storage="fp32"),
+    "refcase_cyl_sbb": dict(case="refparams", shape="CYL", bc="SBB", nx=1201, ny=1201, nz=3, storage="fp32"),
+    "refcase_naca_ibb": dict(case="refparams", shape="AIR", bc="IBB", nx=1201, ny=1201, nz=3, storage="fp32"),
 }
 for _w in WORKLOADS.values():
     _w["dtype"] = _DT[_w["storage"]]
@@ -70,6 +78,18 @@ def make_case(w):
     from lbm3d_bb_amd.solver import Config, no_slip, periodic, velocity, zero_gradient
 
     walls = [velocity((0.05, 0, 0)), zero_gradient(), no_slip(), no_slip(), no_slip(), no_slip()]
+    if w["case"] == "refparams":
+        from lbm3d_bb_amd.params import LBMParams, config_from_params
+
+        p = LBMParams(Nx=w["nx"], Ny=w["ny"], Nz=w["nz"], h=1.0, R=16.16, x_c=600.5, y_c=600.5, theta0=0.0, q=19,
+                      Re=185.0, U_inf=0.075, V_inf=0.0, W_inf=0.0, rho_inf=1.0, dt=1.0, Lambda=0.1875,
+                      n_max=40000, n_save=10000, bc=w["bc"], coll="BGK", shape=w["shape"], naca_code="0012",
+                      chord=120.0)
+        cfg, laws = config_from_params(p, storage=w["storage"])
+        body = "cylinder R=16.16" if w["shape"] == "CYL" else "NACA 0012 airfoil, chord 120"
+        desc = (f"{cfg.nx}x{cfg.ny}x{cfg.nz} D3Q19 BGK, the reference's input/params.txt case: {body} heaving "
+                f"with the reference's law, {w['bc']} links, {w['storage']} populations, per-step MEA force")
+        return cfg, laws, None, desc
     if w["case"] == "channel":
         wp, wm = _trt(0.56)
         cfg = Config(w["nx"], w["ny"], w["nz"], collision="TRT", storage=w["storage"], omega_plus=wp, omega_minus=wm,
@@ -221,7 +241,8 @@ def main() -> int:
             "workload": f"{args.workload}: {desc}",
             "cells": N,
             "tau_plus": round(1.0 / cfg.omega_plus, 6),
-            "lambda": 0.1875,
+            "lambda": 0.1875 if cfg.collision == "TRT" else None,
+            "solid_bc": cfg.solid_bc,
             "step_hbm_frac_of_peak": round(bpc * N * args.steps / el / 1e9 / HBM_PEAK_GBS, 4),
             "step_kernel_ms": round(step_avg_s * 1e3, 5),
             "step_kernel_samples": prof["bulk_launches"],

@@ -84,6 +84,22 @@ typedef enum {
     LBM_WALL_REFERENCE = 1  /* x - c_d/2, as src/SolidBC.jl:195-196 samples it      */
 } lbm_wall_point;
 
+/* Solid boundary rule of the body links (solid_BC!'s bc argument,
+ * src/SolidBC.jl:129-153, "SBB" / "IBB").
+ *   SBB  simple (halfway) bounce-back with the moving-wall correction
+ *        f_opp(x) = f*_d(x) - 6 w_d rho (c_d . u_w)   (src/SolidBC.jl:187-208);
+ *   IBB  linear interpolated bounce-back (src/SolidBC.jl:253-350) with the wall
+ *        fraction q = phi(x) / (phi(x) - phi(x + c_d)) of the body's level set
+ *        on the link from the fluid node x to the solid node x + c_d, clamped to
+ *        [1e-3, 1 - 1e-3]:
+ *          q <  1/2: f_opp(x) = 2q f*_d(x) + (1 - 2q) f*_d(x - c_d) - corr
+ *          q >= 1/2: f_opp(x) = (f*_d(x) + (2q - 1) f*_opp(x) - corr) / (2q)
+ *        corr = 6 w_d rho (c_d . u_w(x + q c_d)).  A link whose level set does
+ *        not change sign, or (q < 1/2) whose second fluid node x - c_d is solid
+ *        or outside the domain, takes the SBB rule.  Needs analytic bodies
+ *        (lbm_set_bodies / lbm_set_motion), not lbm_set_flags. */
+typedef enum { LBM_SOLID_SBB = 0, LBM_SOLID_IBB = 1 } lbm_solid_bc;
+
 typedef struct {
     int32_t nx, ny, nz;
     int32_t collision;    /* lbm_collision */
@@ -91,7 +107,7 @@ typedef struct {
     int32_t wall_point;   /* lbm_wall_point */
     int32_t refill;       /* 0: none (reference); 1: fresh fluid nodes get f_eq of the
                              average of their fluid neighbours' rho, u */
-    int32_t pad_;
+    int32_t solid_bc;     /* lbm_solid_bc */
     double omega_plus;    /* omega_+ = 1/tau_+ (BGK: omega)        src/Parameters.jl:69-72 */
     double omega_minus;   /* omega_- = 1/tau_- (TRT only)          src/Parameters.jl:70-72 */
     double g[3];          /* body acceleration, Guo forcing with F = rho g (0 = off) */
@@ -100,28 +116,49 @@ typedef struct {
     lbm_face faces[6];    /* x-, x+, y-, y+, z-, z+ */
 } lbm_config;
 
-typedef enum { LBM_SPHERE = 0, LBM_ELLIPSOID = 1 } lbm_shape;
+/* Body shapes.  SPHERE and ELLIPSOID are 3-D; CYLINDER and NACA4 are the
+ * reference's planar shapes, extruded along z (src/Geometry.jl:14-39; the
+ * level sets of the interpolated bounce-back, phi_cylinder_lat and
+ * phi_airfoil_lat, src/Geometry.jl:368-396):
+ *   CYLINDER  semi[0] = R; (x - x_c)^2 + (y - y_c)^2 <= R^2 + 1e-6 R^2
+ *             (src/Geometry.jl:71-73); orientation ignored;
+ *   NACA4     semi[0] = chord; naca = {m, p, t, x_int, x_le_offset} from the
+ *             4-digit code (_parse_naca4, src/Geometry.jl:358-366): m = d1/100,
+ *             p = d2/10, t = d3d4/100; x_int is the trailing-edge squeeze
+ *             (1.0089304 in the reference) and x_le_offset the distance from
+ *             the pivot (the body centre) to the leading edge along the chord
+ *             (0.25 chord in the reference, src/Geometry.jl:30-35).  The body
+ *             frame is R^T (x - centre); R must be a rotation about z by the
+ *             pitch angle theta (_rot_to_body, src/Geometry.jl:321-323).
+ *             Inside test src/Geometry.jl:75-82, thickness and camber lines
+ *             src/Geometry.jl:325-356. */
+typedef enum { LBM_SPHERE = 0, LBM_ELLIPSOID = 1, LBM_CYLINDER = 2, LBM_NACA4 = 3 } lbm_shape;
 
 /* One rigid body.  The mask pose (centre, semi-axes, rotation) is evaluated in
- * FP32 as src/Geometry.jl:66-73 does; the half-step kinematics (centre, linear
- * and angular velocity at t_{n+1/2}, src/Motion.jl:62-63,96-100) feed the
- * moving-wall correction and the torque arm. */
+ * FP32 as src/Geometry.jl:66-82 does; the half-step pose and kinematics
+ * (centre, rotation, linear and angular velocity at t_{n+1/2},
+ * src/Motion.jl:62-63,96-116) feed the moving-wall correction, the torque arm
+ * and the interpolated bounce-back's wall fraction. */
 typedef struct {
     int32_t shape;            /* lbm_shape */
     int32_t pad_;
     float center[3];          /* mask pose at t_n, lattice units */
-    float semi[3];            /* sphere: semi[0] = R; ellipsoid: semi-axes a, b, c */
-    float rot[9];             /* row-major R with x_lattice = R x_body (ellipsoid) */
-    float pad2_;
+    float semi[3];            /* sphere, cylinder: semi[0] = R; ellipsoid: a, b, c; NACA4: semi[0] = chord */
+    float rot[9];             /* row-major R with x_lattice = R x_body (ellipsoid, NACA4) */
+    float naca[5];            /* NACA4: m, p, t, x_int, x_le_offset */
     double center_half[3];    /* centre at t_{n+1/2} */
     double vel[3];            /* U at t_{n+1/2} */
     double omega[3];          /* Omega at t_{n+1/2} (rad/step) */
+    double rot_half[9];       /* R at t_{n+1/2} (IBB wall fraction) */
 } lbm_body;
 
-/* Prescribed motion law evaluated by lbm_step() itself: constant linear
- * velocity and constant angular velocity about the body centre.  The pose for
- * step n (1-based) is taken at t = (n - 1 + offset) with offset 0 for the mask
- * and 1/2 for the kinematics (src/Motion.jl:26-36). */
+/* Prescribed motion law evaluated by lbm_step() itself: centre
+ * c(t) = center0 + vel t + amp sin(freq t + phase) and a constant angular
+ * velocity about the centre.  The pose for step n (1-based) is taken at
+ * t = (n - 1 + offset) with offset 0 for the mask and 1/2 for the kinematics
+ * (src/Motion.jl:26-36).  The reference's heaving cylinder
+ * (src/MainCore.jl:72-80) is amp = (0, -0.4 R, 0), freq = 2 pi 0.154 U / (2R),
+ * phase = 0. */
 typedef struct {
     int32_t shape;
     int32_t pad_;
@@ -130,6 +167,10 @@ typedef struct {
     double rot0[9];           /* orientation at t = 0, row-major */
     double vel[3];            /* constant linear velocity (cells/step) */
     double omega[3];          /* constant angular velocity (rad/step) */
+    double amp[3];            /* oscillation amplitude (cells) */
+    double freq;              /* oscillation angular frequency (rad/step) */
+    double phase;             /* oscillation phase (rad) */
+    double naca[5];           /* NACA4 parameters, as in lbm_body */
 } lbm_motion;
 
 typedef struct lbm_ctx lbm_ctx;

@@ -26,7 +26,10 @@ FACE_PERIODIC, FACE_VELOCITY, FACE_PRESSURE_ABB, FACE_ZERO_GRADIENT = 0, 1, 2, 3
 # lbm_wall_point
 WALL_PHYSICAL, WALL_REFERENCE = 0, 1
 # lbm_shape
-SPHERE, ELLIPSOID = 0, 1
+SPHERE, ELLIPSOID, CYLINDER, NACA4 = 0, 1, 2, 3
+SHAPES = {"sphere": SPHERE, "ellipsoid": ELLIPSOID, "cylinder": CYLINDER, "naca4": NACA4}
+# lbm_solid_bc
+SOLID_SBB, SOLID_IBB = 0, 1
 
 
 class LbmFace(ct.Structure):
@@ -37,7 +40,7 @@ class LbmConfig(ct.Structure):
     _fields_ = [
         ("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32),
         ("collision", ct.c_int32), ("storage", ct.c_int32), ("wall_point", ct.c_int32),
-        ("refill", ct.c_int32), ("pad_", ct.c_int32),
+        ("refill", ct.c_int32), ("solid_bc", ct.c_int32),
         ("omega_plus", ct.c_double), ("omega_minus", ct.c_double),
         ("g", ct.c_double * 3), ("rho0", ct.c_double), ("u0", ct.c_double * 3),
         ("faces", LbmFace * 6),
@@ -47,8 +50,9 @@ class LbmConfig(ct.Structure):
 class LbmBody(ct.Structure):
     _fields_ = [
         ("shape", ct.c_int32), ("pad_", ct.c_int32),
-        ("center", ct.c_float * 3), ("semi", ct.c_float * 3), ("rot", ct.c_float * 9), ("pad2_", ct.c_float),
+        ("center", ct.c_float * 3), ("semi", ct.c_float * 3), ("rot", ct.c_float * 9), ("naca", ct.c_float * 5),
         ("center_half", ct.c_double * 3), ("vel", ct.c_double * 3), ("omega", ct.c_double * 3),
+        ("rot_half", ct.c_double * 9),
     ]
 
 
@@ -56,7 +60,8 @@ class LbmMotion(ct.Structure):
     _fields_ = [
         ("shape", ct.c_int32), ("pad_", ct.c_int32),
         ("center0", ct.c_double * 3), ("semi", ct.c_double * 3), ("rot0", ct.c_double * 9),
-        ("vel", ct.c_double * 3), ("omega", ct.c_double * 3),
+        ("vel", ct.c_double * 3), ("omega", ct.c_double * 3), ("amp", ct.c_double * 3), ("freq", ct.c_double),
+        ("phase", ct.c_double), ("naca", ct.c_double * 5),
     ]
 
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,7 +66,7 @@ struct lbm_ctx {
     bool flags_only = false;      // geometry from lbm_set_flags (no analytic bodies)
     std::vector<DevBody> mask_bodies;  // pose that flags reflect
     std::vector<DevBody> step_bodies;  // pose + kinematics of the current step
-    std::vector<float> reach;          // bounding half-extent per body
+    std::vector<std::array<float, 3>> reach;  // bounding half-extents per body and axis
     bool have_flags_box = false;       // lbm_set_flags: bounding box of the solid cells
     float flags_lo[3] = {0, 0, 0}, flags_hi[3] = {0, 0, 0};
     bool geom_valid = false;
@@ -161,22 +162,30 @@ void rotation_at(const double om[3], double t, const double R0[9], double out[9]
 
 // motion_at (src/Motion.jl:26-36) for step n with dt = 1: mask pose at n-1,
 // kinematics at n-1/2.
+// The centre is c0 + v t + A sin(w t + phase), its velocity v + A w cos(w t + phase).
 lbm_body body_from_law(const lbm_motion& m, int64_t n) {
 #pragma clang fp contract(off)
     lbm_body b;
     std::memset(&b, 0, sizeof(b));
     b.shape = m.shape;
     const double t0 = (double)(n - 1), th = (double)(n - 1) + 0.5;
-    double R[9];
+    double R[9], Rh[9];
     rotation_at(m.omega, t0, m.rot0, R);
+    rotation_at(m.omega, th, m.rot0, Rh);
+    const double s0 = std::sin(m.freq * t0 + m.phase), sh = std::sin(m.freq * th + m.phase);
+    const double ch = std::cos(m.freq * th + m.phase);
     for (int k = 0; k < 3; ++k) {
-        b.center[k] = (float)(m.center0[k] + m.vel[k] * t0);
+        b.center[k] = (float)(m.center0[k] + m.vel[k] * t0 + m.amp[k] * s0);
         b.semi[k] = (float)m.semi[k];
-        b.center_half[k] = m.center0[k] + m.vel[k] * th;
-        b.vel[k] = m.vel[k];
+        b.center_half[k] = m.center0[k] + m.vel[k] * th + m.amp[k] * sh;
+        b.vel[k] = m.vel[k] + m.amp[k] * m.freq * ch;
         b.omega[k] = m.omega[k];
     }
-    for (int k = 0; k < 9; ++k) b.rot[k] = (float)R[k];
+    for (int k = 0; k < 9; ++k) {
+        b.rot[k] = (float)R[k];
+        b.rot_half[k] = Rh[k];
+    }
+    for (int k = 0; k < 5; ++k) b.naca[k] = (float)m.naca[k];
     return b;
 }
 
@@ -191,12 +200,17 @@ DevBody to_dev(const lbm_body& b) {
         d.v[k] = b.vel[k];
         d.om[k] = b.omega[k];
     }
-    for (int k = 0; k < 9; ++k) d.rot[k] = b.rot[k];
-    d.reach = b.shape == LBM_SPHERE ? b.semi[0] : std::max(b.semi[0], std::max(b.semi[1], b.semi[2]));
-    if (b.shape == LBM_SPHERE) {
+    for (int k = 0; k < 9; ++k) {
+        d.rot[k] = b.rot[k];
+        d.rh[k] = (float)b.rot_half[k];
+    }
+    for (int k = 0; k < 3; ++k) d.semi[k] = b.semi[k];
+    for (int k = 0; k < 5; ++k) d.naca[k] = b.naca[k];
+    d.reach = 0.0f;
+    if (b.shape == LBM_SPHERE || b.shape == LBM_CYLINDER) {
         const float R = b.semi[0];
         d.r2 = R * R;
-    } else {
+    } else if (b.shape == LBM_ELLIPSOID) {
         for (int k = 0; k < 3; ++k) {
             const float a = b.semi[k];
             d.ia2[k] = 1.0f / (a * a);
@@ -205,15 +219,49 @@ DevBody to_dev(const lbm_body& b) {
     return d;
 }
 
-float reach_of(const lbm_body& b) {
-    if (b.shape == LBM_SPHERE) return b.semi[0];
-    return std::max(b.semi[0], std::max(b.semi[1], b.semi[2]));
+// Bounding half-extents of a body about its centre, per axis.  The planar
+// shapes span the whole z extent (nz + 4 covers every z offset).
+std::array<float, 3> reach_of(int shape, const double semi[3], const double naca[5], int nz) {
+    if (shape == LBM_SPHERE) return {(float)semi[0], (float)semi[0], (float)semi[0]};
+    if (shape == LBM_ELLIPSOID) {
+        const float m = (float)std::max(semi[0], std::max(semi[1], semi[2]));
+        return {m, m, m};
+    }
+    const float zr = (float)nz + 4.0f;
+    if (shape == LBM_CYLINDER) return {(float)semi[0], (float)semi[0], zr};
+    // airfoil: farthest chord end from the pivot plus the thickness and camber
+    const double chord = semi[0], xle = naca[4];
+    const float r = (float)(std::max(std::fabs(xle), std::fabs(chord - xle)) + std::fabs(naca[2]) * chord +
+                            std::fabs(naca[0]) + 1.0);
+    return {r, r, zr};
 }
 
 bool same_mask_pose(const DevBody& a, const DevBody& b) {
     if (a.shape != b.shape || std::memcmp(a.c, b.c, sizeof(a.c)) != 0) return false;
-    if (a.shape == LBM_SPHERE) return a.r2 == b.r2;  // a sphere's mask ignores its orientation
+    // sphere and cylinder masks ignore the orientation
+    if (a.shape == LBM_SPHERE || a.shape == LBM_CYLINDER) return a.r2 == b.r2;
+    if (a.shape == LBM_NACA4)
+        return std::memcmp(a.semi, b.semi, sizeof(a.semi)) == 0 && std::memcmp(a.naca, b.naca, sizeof(a.naca)) == 0 &&
+               std::memcmp(a.rot, b.rot, sizeof(a.rot)) == 0;
     return std::memcmp(a.ia2, b.ia2, sizeof(a.ia2)) == 0 && std::memcmp(a.rot, b.rot, sizeof(a.rot)) == 0;
+}
+
+// Shape checks shared by lbm_set_bodies and lbm_set_motion: known shape,
+// positive size, and for the planar shapes a rotation about z only.
+const char* shape_error(int shape, const double semi[3], const double rot[9], const double omega[3]) {
+    if (shape < LBM_SPHERE || shape > LBM_NACA4) return "unknown body shape";
+    const int ns = shape == LBM_ELLIPSOID ? 3 : 1;
+    for (int k = 0; k < ns; ++k)
+        if (!(semi[k] > 0.0)) return "body size must be positive";
+    if (shape == LBM_NACA4) {
+        const double off[4] = {rot[2], rot[5], rot[6], rot[7]};
+        for (double v : off)
+            if (std::fabs(v) > 1e-6) return "a NACA4 body rotates about z only";
+        if (std::fabs(rot[8] - 1.0) > 1e-6) return "a NACA4 body rotates about z only";
+    }
+    if ((shape == LBM_NACA4 || shape == LBM_CYLINDER) && omega && (omega[0] != 0.0 || omega[1] != 0.0))
+        return "a planar body spins about z only";
+    return nullptr;
 }
 
 
@@ -307,6 +355,7 @@ LaunchReq base_req(const lbm_ctx* c, int kind, int mode) {
     r.kind = kind;
     r.mode = mode;
     r.collision = c->cfg.collision;
+    r.solid_bc = c->cfg.solid_bc;
     r.forced = c->forced ? 1 : 0;
     r.g = c->g;
     r.p = c->pd;
@@ -568,8 +617,8 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
         // body regions: union of the old and new bounding boxes, dilated
         for (int i = 0; i < c->nb; ++i) {
             float lo_f[3], hi_f[3];
-            const float rr = c->reach[i];
             for (int a = 0; a < 3; ++a) {
+                const float rr = c->reach[i][a];
                 lo_f[a] = std::min(bnew[i].c[a], bold[i].c[a]) - rr;
                 hi_f[a] = std::max(bnew[i].c[a], bold[i].c[a]) + rr;
             }
@@ -593,8 +642,8 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
             for (int i = 0; i < c->nb; ++i) {
                 float lo_f[3], hi_f[3];
                 for (int a = 0; a < 3; ++a) {
-                    lo_f[a] = bnew[i].c[a] - c->reach[i];
-                    hi_f[a] = bnew[i].c[a] + c->reach[i];
+                    lo_f[a] = bnew[i].c[a] - c->reach[i][a];
+                    hi_f[a] = bnew[i].c[a] + c->reach[i][a];
                 }
                 add_body_box(c, sg, lo_f, hi_f, true);
             }
@@ -705,6 +754,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     *out = nullptr;
     if (cfg->collision != LBM_BGK && cfg->collision != LBM_TRT) return LBM_ERR_ARG;
     if (cfg->storage < LBM_FP64 || cfg->storage > LBM_BF16S) return LBM_ERR_ARG;
+    if (cfg->solid_bc != LBM_SOLID_SBB && cfg->solid_bc != LBM_SOLID_IBB) return LBM_ERR_ARG;
     const int L[3] = {cfg->nx, cfg->ny, cfg->nz};
     for (int a = 0; a < 3; ++a) {
         const int k0 = cfg->faces[2 * a].kind, k1 = cfg->faces[2 * a + 1].kind;
@@ -857,11 +907,25 @@ static int set_body_count(lbm_ctx* c, int n) {
 int lbm_set_bodies(lbm_ctx* c, int32_t n, const lbm_body* bodies) {
     if (!c || (n > 0 && !bodies)) return LBM_ERR_ARG;
     for (int i = 0; i < n; ++i) {
-        if (bodies[i].shape != LBM_SPHERE && bodies[i].shape != LBM_ELLIPSOID)
-            return fail(c, LBM_ERR_ARG, "unknown body shape");
-        const int ns = bodies[i].shape == LBM_SPHERE ? 1 : 3;
-        for (int k = 0; k < ns; ++k)
-            if (!(bodies[i].semi[k] > 0.0f)) return fail(c, LBM_ERR_ARG, "body size must be positive");
+        const lbm_body& b = bodies[i];
+        const double semi[3] = {b.semi[0], b.semi[1], b.semi[2]};
+        double rot[9];
+        for (int k = 0; k < 9; ++k) rot[k] = b.rot[k];
+        if (const char* e = shape_error(b.shape, semi, rot, b.omega)) return fail(c, LBM_ERR_ARG, e);
+        if (b.shape == LBM_NACA4) {
+            for (int k = 0; k < 9; ++k) rot[k] = b.rot_half[k];
+            if (const char* e = shape_error(b.shape, semi, rot, b.omega)) return fail(c, LBM_ERR_ARG, e);
+        }
+        if (c->cfg.solid_bc == LBM_SOLID_IBB && (b.shape == LBM_ELLIPSOID || b.shape == LBM_NACA4)) {
+            // the wall fraction reads the half-step rotation: it must be one
+            for (int i2 = 0; i2 < 3; ++i2)
+                for (int j2 = 0; j2 < 3; ++j2) {
+                    double dot = 0.0;
+                    for (int k = 0; k < 3; ++k) dot += b.rot_half[3 * k + i2] * b.rot_half[3 * k + j2];
+                    if (std::fabs(dot - (i2 == j2 ? 1.0 : 0.0)) > 1e-6)
+                        return fail(c, LBM_ERR_ARG, "rot_half must be a rotation (interpolated bounce-back)");
+                }
+        }
     }
     if (c->flags_only) {
         c->flags_only = false;
@@ -874,19 +938,20 @@ int lbm_set_bodies(lbm_ctx* c, int32_t n, const lbm_body* bodies) {
     c->pending.assign(bodies, bodies + n);
     c->pending_set = true;
     c->reach.clear();
-    for (int i = 0; i < n; ++i) c->reach.push_back(reach_of(bodies[i]));
+    for (int i = 0; i < n; ++i) {
+        const double semi[3] = {bodies[i].semi[0], bodies[i].semi[1], bodies[i].semi[2]};
+        double naca[5];
+        for (int k = 0; k < 5; ++k) naca[k] = bodies[i].naca[k];
+        c->reach.push_back(reach_of(bodies[i].shape, semi, naca, c->g.nz));
+    }
     return LBM_OK;
 }
 
 int lbm_set_motion(lbm_ctx* c, int32_t n, const lbm_motion* laws) {
     if (!c || (n > 0 && !laws)) return LBM_ERR_ARG;
-    for (int i = 0; i < n; ++i) {
-        if (laws[i].shape != LBM_SPHERE && laws[i].shape != LBM_ELLIPSOID)
-            return fail(c, LBM_ERR_ARG, "unknown body shape");
-        const int ns = laws[i].shape == LBM_SPHERE ? 1 : 3;
-        for (int k = 0; k < ns; ++k)
-            if (!(laws[i].semi[k] > 0.0)) return fail(c, LBM_ERR_ARG, "body size must be positive");
-    }
+    for (int i = 0; i < n; ++i)
+        if (const char* e = shape_error(laws[i].shape, laws[i].semi, laws[i].rot0, laws[i].omega))
+            return fail(c, LBM_ERR_ARG, e);
     if (c->flags_only) {
         c->flags_only = false;
         c->geom_valid = false;
@@ -898,16 +963,14 @@ int lbm_set_motion(lbm_ctx* c, int32_t n, const lbm_motion* laws) {
     c->pending.clear();
     c->pending_set = false;
     c->reach.clear();
-    for (int i = 0; i < n; ++i) {
-        const lbm_motion& m = laws[i];
-        c->reach.push_back(m.shape == LBM_SPHERE ? (float)m.semi[0]
-                                                 : (float)std::max(m.semi[0], std::max(m.semi[1], m.semi[2])));
-    }
+    for (int i = 0; i < n; ++i) c->reach.push_back(reach_of(laws[i].shape, laws[i].semi, laws[i].naca, c->g.nz));
     return LBM_OK;
 }
 
 int lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
     if (!c || !flags) return LBM_ERR_ARG;
+    if (c->cfg.solid_bc == LBM_SOLID_IBB)
+        return fail(c, LBM_ERR_ARG, "the interpolated bounce-back needs analytic bodies, not explicit flags");
     int maxf = 0;
     int lo[3] = {c->g.nx, c->g.ny, c->g.nz}, hi[3] = {-1, -1, -1};
     for (int64_t i = 0; i < c->g.N; ++i) {
@@ -933,7 +996,7 @@ int lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
     c->pending_set = false;
     c->flags_only = true;
     // bodies without kinematics: at rest, centre at the origin
-    c->reach.assign(maxf, 0.0f);
+    c->reach.assign(maxf, std::array<float, 3>{0.0f, 0.0f, 0.0f});
     DevBody unknown{};
     unknown.reach = -1.0f;  // no analytic shape: every face cell checks its neighbours' flags
     c->step_bodies.assign(maxf, unknown);

@@ -38,18 +38,77 @@ __host__ __device__ inline float min_image(float d, bool periodic, int L) {
     return d;
 }
 
-// generate_mask_s_local!_gpu's test (src/Geometry.jl:66-73) for a sphere, and
-// its body-frame analogue for an ellipsoid.
+// NACA 4-digit lower and upper surface heights at the chordwise position x
+// from the leading edge (_naca_upper_lower_y, src/Geometry.jl:347-356): the
+// half thickness (_naca_thickness_y, :325-330) and the camber line with its
+// slope (_naca_camber_y_dydx, :332-345), in the reference's operation order.
+// The camber line is in chord fractions, not scaled by the chord, as the
+// reference evaluates it.  xi^4 is formed in FP64 and rounded once (Julia's
+// ^(::Float32, ::Integer)); cos(atan(s)) is 1 / sqrt(1 + s^2), which uses
+// only correctly rounded operations, so host and device agree bit for bit.
+template <typename T>
+__host__ __device__ inline void naca_lines(T x, T chord, T m, T p, T t, T x_int, T& yl, T& yu) {
+#pragma clang fp contract(off)
+    const T xbar = x / chord;
+    T xi = x_int * xbar;
+    xi = xi > T(0) ? xi : T(0);
+    const T rt = sqrt(xi);
+    const T x3 = xi * xi * xi;
+    const double xd = (double)xi;
+    const T x4 = (T)((xd * xd) * (xd * xd));
+    T poly = T(0.2969) * rt - T(0.1260) * xi;
+    poly = poly - T(0.3516) * xi * xi;
+    poly = poly + T(0.2843) * x3;
+    poly = poly - T(0.1015) * x4;
+    const T yt = T(5) * t * chord * poly;
+    T yc = T(0), dyb = T(0);
+    if (m == T(0) || p == T(0)) {
+    } else if (xbar < p) {
+        yc = m / (p * p) * (T(2) * p * xbar - xbar * xbar);
+        dyb = T(2) * m / (p * p) * (p - xbar);
+    } else {
+        const T den = T(1) - p;
+        const T den2 = den * den;
+        yc = m / den2 * ((T(1) - T(2) * p) + T(2) * p * xbar - xbar * xbar);
+        dyb = T(2) * m / den2 * (p - xbar);
+    }
+    const T sl = dyb / chord;
+    const T cs = T(1) / sqrt(T(1) + sl * sl);
+    yu = yc + yt * cs;
+    yl = yc - yt * cs;
+}
+
+// The airfoil's inside test (src/Geometry.jl:75-82) on the minimum-image
+// offset from the pivot.  Kept out of line: the solidity tests are inlined at
+// every neighbour of every fix-up cell, and the airfoil's divisions and square
+// roots there would multiply the kernel's code size.
+__device__ __noinline__ bool inside_naca(const DevBody& b, float dx, float dy) {
+#pragma clang fp contract(off)
+    const float xb = b.rot[0] * dx + b.rot[3] * dy;
+    const float yb = b.rot[1] * dx + b.rot[4] * dy;
+    const float xle = xb + b.naca[4];
+    const float chord = b.semi[0];
+    if (!(0.0f <= xle && xle <= chord)) return false;
+    float yl, yu;
+    naca_lines<float>(xle, chord, b.naca[0], b.naca[1], b.naca[2], b.naca[3], yl, yu);
+    const float tol = 1e-6f * chord;
+    return yl - tol <= yb && yb <= yu + tol;
+}
+
+// generate_mask_s_local!_gpu's tests (src/Geometry.jl:66-82): cylinder
+// (extruded along z) and NACA airfoil; the sphere is the cylinder's test with
+// the z term, the ellipsoid its body-frame analogue.
 __device__ inline bool inside_body(const DevBody& b, float px, float py, float pz, const Grid& g) {
 #pragma clang fp contract(off)
     float dx = min_image(px - b.c[0], g.per & 1, g.nx);
     float dy = min_image(py - b.c[1], g.per & 2, g.ny);
     float dz = min_image(pz - b.c[2], g.per & 4, g.nz);
-    if (b.shape == LBM_SPHERE) {
+    if (b.shape == LBM_SPHERE || b.shape == LBM_CYLINDER) {
         float s = dx * dx + dy * dy;
-        s = s + dz * dz;
+        if (b.shape == LBM_SPHERE) s = s + dz * dz;
         return s <= b.r2 + 1e-6f * b.r2;
     }
+    if (__builtin_expect(b.shape == LBM_NACA4, 0)) return inside_naca(b, dx, dy);
     float xb = b.rot[0] * dx + b.rot[3] * dy;
     xb = xb + b.rot[6] * dz;
     float yb = b.rot[1] * dx + b.rot[4] * dy;
@@ -59,6 +118,47 @@ __device__ inline bool inside_body(const DevBody& b, float px, float py, float p
     float q = xb * xb * b.ia2[0] + yb * yb * b.ia2[1];
     q = q + zb * zb * b.ia2[2];
     return q <= 1.0f + 1e-6f;
+}
+
+// Level set of body b at the point p, FP32 at the half-step pose (as the
+// reference evaluates it): negative inside, zero on the surface.  Cylinder:
+// phi_cylinder_lat (src/Geometry.jl:368-372); NACA4: phi_airfoil_lat
+// (src/Geometry.jl:375-396), the vertical distance to the nearer surface over
+// the chord and the distance to the leading / trailing edge point beyond it;
+// sphere |d| - R; ellipsoid sqrt(sum (x_i / a_i)^2) - 1.  The interpolated
+// bounce-back uses only the ratio phi(x) / (phi(x) - phi(x')).  Only correctly
+// rounded operations, so host and device agree bit for bit.
+__device__ __forceinline__ float body_phi(const DevBody& b, float px, float py, float pz, const Grid& g) {
+#pragma clang fp contract(off)
+    const float dx = min_image(px - (float)b.ch[0], g.per & 1, g.nx);
+    const float dy = min_image(py - (float)b.ch[1], g.per & 2, g.ny);
+    const float dz = min_image(pz - (float)b.ch[2], g.per & 4, g.nz);
+    if (b.shape == LBM_SPHERE || b.shape == LBM_CYLINDER) {
+        float s = dx * dx + dy * dy;
+        if (b.shape == LBM_SPHERE) s = s + dz * dz;
+        return sqrtf(s) - b.semi[0];
+    }
+    if (b.shape == LBM_ELLIPSOID) {
+        float e = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float r = b.rh[k] * dx + b.rh[3 + k] * dy;
+            r = r + b.rh[6 + k] * dz;
+            const float t = r / b.semi[k];
+            e = e + t * t;
+        }
+        return sqrtf(e) - 1.0f;
+    }
+    const float xb = b.rh[0] * dx + b.rh[3] * dy;
+    const float yb = b.rh[1] * dx + b.rh[4] * dy;
+    const float xle = xb + b.naca[4];
+    const float chord = b.semi[0];
+    if (xle < 0.0f) return sqrtf(xle * xle + yb * yb);
+    if (xle > chord) return sqrtf((xle - chord) * (xle - chord) + yb * yb);
+    float yl, yu;
+    naca_lines<float>(xle, chord, b.naca[0], b.naca[1], b.naca[2], b.naca[3], yl, yu);
+    if (yb >= yl && yb <= yu) return -fminf(yu - yb, yb - yl);
+    return yb > yu ? yb - yu : yl - yb;
 }
 
 // Flag of a cell: 0 fluid, k+1 inside body k (the first one wins).
@@ -876,11 +976,11 @@ __device__ __forceinline__ void plain_cell(const typename S::T* __restrict__ fin
 // ---------------------------------------------------------------------------
 enum { DIR_PULL = 0, DIR_FACE = 1, DIR_LINK = 2 };
 
-template <class S, int COLL, bool FORCED, int MODE>
+template <class S, int COLL, bool FORCED, int MODE, bool IBB = false>
 __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                              double* __restrict__ gout, double* __restrict__ rho_out, double* __restrict__ u_out,
                              const Grid& g, const Phys<typename S::C>& p, const SolidView& sv, int x, int y, int z,
-                             ForceAcc& acc, double* lds_overflow) {
+                             ForceAcc& acc, double* lds_overflow, float* lds_q = nullptr) {
     using C = typename S::C;
     const int64_t N = g.N;
     const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
@@ -963,6 +1063,40 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
         own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
     }
     __builtin_amdgcn_sched_barrier(0);
+    // interpolated bounce-back: wall fraction q and rule of every link from the
+    // body's level set, in a loop that is not unrolled (few values are live
+    // here); rule 1: q < 1/2 with a fluid second node x - c_dl; rule 2: q >= 1/2;
+    // rule 0: no sign change, or no second node (the simple bounce-back)
+    // q is kept in LDS (lds_q[d * kBlock + thread]) rather than registers
+    float* qv = lds_q + threadIdx.x;
+    uint64_t rules = 0;
+    if (IBB) {
+#pragma unroll 1
+        for (int d = 0; d < 18; ++d) {
+            if (((codes[d / 3] >> (10 * (d % 3))) & 3u) != DIR_LINK) continue;
+            const int dl = d ^ 1;  // opposite of a moving direction
+            const DevBody& B = sv.bnew[(codes[d / 3] >> (10 * (d % 3) + 2)) & 0xffu];
+            const int cx = c_at(0, dl), cy = c_at(1, dl), cz = c_at(2, dl);
+            const float pbx = (float)x + 0.5f, pby = (float)y + 0.5f, pbz = (float)z + 0.5f;
+            const float phb = body_phi(B, pbx, pby, pbz, g);
+            const float phs = body_phi(B, pbx + (float)cx, pby + (float)cy, pbz + (float)cz, g);
+            const float den = phb - phs;
+            if (!(phb > 0.0f && phs < 0.0f && fabsf(den) >= 1e-8f)) continue;
+            const float q = fminf(fmaxf(phb / den, kEpsQ), 1.0f - kEpsQ);
+            int rule = 2;
+            if (q < 0.5f) {
+                int s2[3] = {x - cx, y - cy, z - cz};
+                bool ok = true;
+                for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(s2[a], L[a], (g.per >> a) & 1);
+                rule = ok && solid_at(sv, s2[0], s2[1], s2[2], g) == 0 ? 1 : 0;
+            }
+            if (rule != 0) {
+                qv[d * kBlock] = q;
+                rules |= (uint64_t)rule << (2 * d);
+            }
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     // 1) faces (with precedence) and the plain pulls
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
@@ -1022,11 +1156,16 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             // boundary link (x, dl), dl = opp(d): the solid node is x + c_dl
             const int dl = kOpp[d];
             const DevBody& B = sv.bnew[aux];
+            const int rule = IBB ? (int)((rules >> (2 * d)) & 3u) : 0;
+            const double qw = IBB && rule != 0 ? (double)qv[d * kBlock] : 0.5;
+            // wall point x + h c_dl of the wall velocity: h = q (IBB) or 1/2 (SBB),
+            // mirrored to -h for the reference's sampling point
+            const double hw = rule == 0 ? hs : (g.wall_point == LBM_WALL_REFERENCE ? -qw : qw);
             double r[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const int cdl = a == 0 ? kCx[dl] : (a == 1 ? kCy[dl] : kCz[dl]);
-                r[a] = (double)q0[a] + 0.5 + hs * (double)cdl - B.ch[a];
+                r[a] = (double)q0[a] + 0.5 + hw * (double)cdl - B.ch[a];
                 if ((g.per >> a) & 1) {
                     const double h = 0.5 * L[a];
                     if (r[a] > h) r[a] -= L[a];
@@ -1038,17 +1177,38 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             const C uwz = C(B.v[2] + (B.om[0] * r[1] - B.om[1] * r[0]));
             const C cuw = C(kCx[dl]) * uwx + C(kCy[dl]) * uwy + C(kCz[dl]) * uwz;
             const C fin_dl = f[d];
-            const C val = fin_dl - C(2) * weight<C>(dl) * rho_prev * cuw * C(3);
+            const C corr = C(2) * weight<C>(dl) * rho_prev * cuw * C(3);
+            C val = fin_dl - corr;
+            if (IBB && rule != 0) {
+                // rule 1: f*_dl at the second fluid node x - c_dl; rule 2: the
+                // cell's own f*_d, which leaves towards that node.  The offsets are
+                // formed here (an opaque copy of the cell index), not hoisted ahead
+                // of the loop for every direction.
+                int xs = x, ys = y, zs = z;
+                asm volatile("" : "+v"(xs), "+v"(ys), "+v"(zs));
+                int s2[3] = {xs - kCx[dl], ys - kCy[dl], zs - kCz[dl]};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) wrap_coord(s2[a], L[a], (g.per >> a) & 1);
+                const int64_t cell2 = (int64_t)s2[0] + (int64_t)g.nx * s2[1] + g.nxy * s2[2];
+                const int64_t own = (int64_t)xs + (int64_t)g.nx * ys + g.nxy * zs;
+                const bool near = rule == 1;
+                const int64_t off = near ? (int64_t)dl * g.P + cell2 : (int64_t)d * g.P + own;
+                const C fx = S::load_pick(fin + off, dl, d, !near, g.sh);
+                const C q2 = C(2) * C(qw);
+                val = near ? q2 * fin_dl + (C(1) - q2) * fx - corr : (fin_dl + (q2 - C(1)) * fx - corr) / q2;
+            }
             f[d] = val;
             if (MODE == MODE_STEP) {
-                // momentum exchange of this link (src/MEA.jl:102-107), FP64, arm to x + c_dl/2
+                // momentum exchange of this link (src/MEA.jl:102-107), FP64, arm to the
+                // wall point x + c_dl/2 (SBB) or x + q c_dl (IBB)
                 const double t = (double)(fin_dl + val);
                 const double P[3] = {t * kCx[dl], t * kCy[dl], t * kCz[dl]};
+                const double ha = rule == 0 ? 0.5 : qw;
                 double ra[3];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     const int cdl = a == 0 ? kCx[dl] : (a == 1 ? kCy[dl] : kCz[dl]);
-                    ra[a] = (double)q0[a] + 0.5 + 0.5 * (double)cdl - B.ch[a];
+                    ra[a] = (double)q0[a] + 0.5 + ha * (double)cdl - B.ch[a];
                     if ((g.per >> a) & 1) {
                         const double h = 0.5 * L[a];
                         if (ra[a] > h) ra[a] -= L[a];
@@ -1065,6 +1225,8 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
                 }
             }
         }
+        // the interpolated rule's extra loads and FP64 sums: one direction at a time
+        if (IBB) __builtin_amdgcn_sched_barrier(0);
     }
     finish_face_cell<S, COLL, FORCED, MODE>(f, p, cell, g, fout, gout, rho_out, u_out, x, y, z, -1);
 }
@@ -1081,7 +1243,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
 // Cells in several regions go to the first.  Every block writes nbodies*6
 // partial force/torque sums (fixed order) to partials[blockIdx].
 // ---------------------------------------------------------------------------
-template <class S, int COLL, bool FORCED, int MODE>
+template <class S, int COLL, bool FORCED, int MODE, bool IBB = false>
 __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* __restrict__ fin,
                                             typename S::T* __restrict__ fout, double* __restrict__ gout,
                                             double* __restrict__ rho_out, double* __restrict__ u_out, const Grid& g,
@@ -1103,6 +1265,9 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
     // then the bodies (new poses, then old), when few enough
     const bool staged = nb <= kLdsBodies;
     DevBody* s_bodies = reinterpret_cast<DevBody*>(lds + (kWaves + 1) * n6);
+    // then, with the interpolated bounce-back, the wall fractions (18 per thread)
+    float* s_q = reinterpret_cast<float*>(reinterpret_cast<char*>(s_bodies) +
+                                          (staged ? 2 * (size_t)nb * sizeof(DevBody) : 0));
     if (staged) {
         const int nw = 2 * nb * (int)(sizeof(DevBody) / 4);
         const uint32_t* src_new = reinterpret_cast<const uint32_t*>(bnew);
@@ -1226,7 +1391,8 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
         if (plain)
             plain_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
         else
-            general_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, sv, x, y, z, acc, s_over);
+            general_cell<S, COLL, FORCED, MODE, IBB>(fin, fout, gout, rho_out, u_out, g, p, sv, x, y, z, acc, s_over,
+                                                     s_q);
     }
     if (MODE == MODE_STEP && nb > 0) {
         const int wave = tid >> 6;
@@ -1262,10 +1428,12 @@ inline BulkShape bulk_shape(const Grid& g) {
     return bs;
 }
 
-// dynamic LDS of a step launch: the fix-up blocks' force partials
-inline size_t step_lds_bytes(int nbodies) {
+// dynamic LDS of a step launch: the fix-up blocks' force partials, staged
+// bodies and (interpolated bounce-back) wall fractions
+inline size_t step_lds_bytes(int nbodies, bool ibb = false) {
     size_t b = (size_t)(kWaves + 1) * (size_t)nbodies * 6 * sizeof(double);
     if (nbodies <= kLdsBodies) b += 2 * (size_t)nbodies * sizeof(DevBody);
+    if (ibb) b += 18 * (size_t)kBlock * sizeof(float);
     return b;
 }
 
@@ -1275,7 +1443,7 @@ inline size_t step_lds_bytes(int nbodies) {
 #ifndef LBM_STEP_MIN_WAVES
 #define LBM_STEP_MIN_WAVES 1
 #endif
-template <class S, int COLL, bool FORCED, int MODE>
+template <class S, int COLL, bool FORCED, int MODE, bool IBB = false>
 __global__ __launch_bounds__(kBlock, LBM_STEP_MIN_WAVES) void k_step(const typename S::T* __restrict__ fin,
                                                  typename S::T* __restrict__ fout, double* __restrict__ gout,
                                                  double* __restrict__ rho_out, double* __restrict__ u_out, Grid g,
@@ -1288,7 +1456,7 @@ __global__ __launch_bounds__(kBlock, LBM_STEP_MIN_WAVES) void k_step(const typen
     extern __shared__ double lds_dyn[];
     const int blk = blockIdx.x;
     if (blk < nfix) {
-        fixup_block<S, COLL, FORCED, MODE>(blk, fin, fout, gout, rho_out, u_out, g, p, sgp, bnew, bold, flags, list,
+        fixup_block<S, COLL, FORCED, MODE, IBB>(blk, fin, fout, gout, rho_out, u_out, g, p, sgp, bnew, bold, flags, list,
                                            list_count, partials, write_flags, lds_dyn);
         return;
     }

@@ -15,7 +15,7 @@ static void launch_bulk_t(const LaunchReq& r, const Phys<typename S::C>& p) {
                        (const uint8_t*)r.flags);
 }
 
-template <class S, int MODE, int COLL, bool FORCED>
+template <class S, int MODE, int COLL, bool FORCED, bool IBB = false>
 static void launch_step_t(const LaunchReq& r, const Phys<typename S::C>& p) {
     using T = typename S::T;
     const StepGeom* sg = (const StepGeom*)r.rec;
@@ -23,10 +23,16 @@ static void launch_step_t(const LaunchReq& r, const Phys<typename S::C>& p) {
     const DevBody* bo = bn + r.nb;
     const BulkShape bs = bulk_shape<S>(r.g);
     const int64_t nbulk = (int64_t)bs.gx * bs.gy * r.g.nz;
-    hipLaunchKernelGGL((k_step<S, COLL, FORCED, MODE>), dim3((unsigned)(r.grid_blocks + nbulk)), dim3(kBlock),
-                       (unsigned)step_lds_bytes(r.nb), r.stream, (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, sg, bn, bo, r.flags,
+    hipLaunchKernelGGL((k_step<S, COLL, FORCED, MODE, IBB>), dim3((unsigned)(r.grid_blocks + nbulk)), dim3(kBlock),
+                       (unsigned)step_lds_bytes(r.nb, IBB), r.stream, (const T*)r.fin, (T*)r.fout, r.gout, r.rho_out, r.u_out, r.g, p, sg, bn, bo, r.flags,
                        r.list, r.list_count, r.partials, r.write_flags, bs, r.grid_blocks);
 }
+
+// Step and getter kernels with the interpolated bounce-back (separate
+// instantiations, so the simple bounce-back kernels keep their register
+// budget): defined in lbm_launch_ibb.h, instantiated by lbm_store_*_ibb.hip.
+template <class S>
+void launch_step_ibb(const LaunchReq& r, const Phys<typename S::C>& p);
 
 template <class S, int MODE>
 static void launch_mode(const LaunchReq& r, const Phys<typename S::C>& p) {
@@ -34,6 +40,8 @@ static void launch_mode(const LaunchReq& r, const Phys<typename S::C>& p) {
     if (r.kind == K_BULK) {
         if (trt) r.forced ? launch_bulk_t<S, MODE, LBM_TRT, true>(r, p) : launch_bulk_t<S, MODE, LBM_TRT, false>(r, p);
         else r.forced ? launch_bulk_t<S, MODE, LBM_BGK, true>(r, p) : launch_bulk_t<S, MODE, LBM_BGK, false>(r, p);
+    } else if ((MODE == MODE_STEP || MODE == MODE_GET) && r.solid_bc == LBM_SOLID_IBB) {
+        launch_step_ibb<S>(r, p);
     } else if (MODE == MODE_STEP || MODE == MODE_GET) {
         if (trt) r.forced ? launch_step_t<S, MODE, LBM_TRT, true>(r, p) : launch_step_t<S, MODE, LBM_TRT, false>(r, p);
         else r.forced ? launch_step_t<S, MODE, LBM_BGK, true>(r, p) : launch_step_t<S, MODE, LBM_BGK, false>(r, p);

@@ -15,6 +15,9 @@ constexpr int kScanBoxes = 32;                   // boxes a fix-up block scans w
 // Bodies the fix-up blocks stage in LDS (new and old pose) so the analytic
 // solidity tests and the link kinematics read LDS instead of global memory.
 constexpr int kLdsBodies = 16;
+// Interpolated bounce-back: the wall fraction q is clamped to [kEpsQ, 1 - kEpsQ]
+// (EPS_Q of src/SolidBC.jl:321, which the reference leaves undefined).
+constexpr float kEpsQ = 1e-3f;
 
 // Flag byte: low 7 bits 0 (fluid) or k+1 (solid of body k); bit 7 marks a fluid
 // boundary node of static geometry, finished by the fix-up blocks.
@@ -22,14 +25,18 @@ constexpr uint8_t kFlagBody = 0x7f;
 constexpr uint8_t kFlagFixup = 0x80;
 
 // Body as the kernels see it: FP32 mask pose (inside test) and FP64 half-step
-// kinematics (moving-wall velocity and torque arm).
+// pose and kinematics (moving-wall velocity, torque arm, wall fraction).
 struct DevBody {
     int shape;
     float c[3];
-    float r2;
-    float ia2[3];
+    float r2;      // sphere, cylinder: R * R
+    float ia2[3];  // ellipsoid: 1 / a^2, 1 / b^2, 1 / c^2
     float rot[9];
-    float reach;  // bounding half-extent of the mask (< 0: unknown, flags only)
+    float reach;   // 0: analytic shape; < 0: unknown (explicit flags)
+    float semi[3];
+    float naca[5];  // NACA4: m, p, t, x_int, x_le_offset
+    float rh[9];  // rotation at the half step
+    float pad_;
     double ch[3];
     double v[3];
     double om[3];
@@ -121,6 +128,7 @@ struct LaunchReq {
     int kind;
     int mode;
     int collision;
+    int solid_bc;  // lbm_solid_bc: picks the kernels with the interpolated rule
     int forced;
     Grid g;
     Phys<double> p;

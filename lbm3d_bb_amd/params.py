@@ -125,3 +125,68 @@ def read_params_input(path: str) -> LBMParams:
         theta0=float(np.float32(math.radians(v[7]))), q=int(v[9]), Re=v[10], U_inf=v[11], V_inf=v[12],
         W_inf=v[13], rho_inf=v[14], dt=v[15], Lambda=v[16], n_max=int(v[17]), n_save=int(v[18]),
         bc=bc, coll=coll, shape=shape, naca_code=naca, chord=v[8])
+
+
+def ref_area_lat(p: LBMParams) -> float:
+    """Reference area in lattice units (ref_area_lat, src/Utilities.jl:931-938):
+    diameter (cylinder) or chord (airfoil) in cells times the span Nz."""
+    f = np.float32
+    L = f(2.0) * f(p.R) if p.shape == "CYL" else f(p.chord)
+    return float((L / f(p.h)) * f(p.Nz))
+
+
+def aero_coefficients(forces: np.ndarray, p: LBMParams) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(CL, CD, CZ) from per-step body forces (..., 6) as compute_aero! scales them
+    (src/MEA.jl:57-59): fc = 1 / (0.5 rho_inf U_inf^2 A_ref)."""
+    fc = 1.0 / (0.5 * p.rho_inf * p.U_inf * p.U_inf * ref_area_lat(p))
+    F = np.asarray(forces)
+    return F[..., 1] * fc, F[..., 0] * fc, F[..., 2] * fc
+
+
+def config_from_params(p: LBMParams, storage: str = "fp32", wall_point: str = "physical",
+                       refill: bool = False):
+    """The reference's run set-up for one LBMParams, in lattice units
+    (main_simulation_HPC, src/MainCore.jl:57-80 and initialize_distributed):
+
+    * faces as compile_face_BC builds them (src/Utilities.jl:795-855): velocity
+      (U_inf, V_inf, W_inf) on the inlet x = 1 and on both y walls,
+      anti-bounce-back outlet at x = Nx, z periodic;
+    * collision Coll with omega+- from Re, U_inf and Lambda (src/Parameters.jl:60-72);
+    * solid boundary BC (SBB | IBB);
+    * the body: the z-extruded cylinder of radius R, or the NACA airfoil of
+      chord Chord pitched by theta0 about its quarter-chord pivot
+      (init_solid_mask_state, src/Geometry.jl:26-39), centred at (x_c, y_c);
+    * its heave law y(t) = y_c - A sin(w t), A = 0.2 (2R), w = 2 pi 0.154 U_inf / (2R)
+      (src/MainCore.jl:72-80; the airfoil heaves with the same law, R included).
+
+    The start is the uniform equilibrium of (rho_inf, U_inf, V_inf, W_inf); the
+    reference's random 20 % perturbation of the start velocity
+    (src/Initialization.jl:70) is not reproduced.  wall_point="reference" samples
+    the moving-wall velocity where the reference does (src/SolidBC.jl:195-196).
+
+    Returns (Config, [ConstantMotion]) for Solver(config).set_motion(laws).
+    """
+    from .bodies import ConstantMotion, X_INT, naca_params, rotation_z
+    from .solver import Config, periodic, pressure, velocity
+
+    h = float(p.h)
+    U = (float(p.U_inf), float(p.V_inf), float(p.W_inf))
+    coll = p.coll.upper()
+    if coll not in ("BGK", "TRT"):
+        raise ValueError(f"Unknown collision method: {p.coll}")
+    cfg = Config(p.Nx, p.Ny, p.Nz, collision=coll, storage=storage, omega_plus=p.omega_p,
+                 omega_minus=p.omega_m if coll == "TRT" else p.omega_p,
+                 faces=[velocity(U), pressure(), velocity(U), velocity(U), periodic(), periodic()],
+                 rho0=float(p.rho_inf), u0=U, wall_point=wall_point, refill=refill, solid_bc=p.bc.upper())
+    A = 0.2 * 2.0 * float(p.R)
+    w = 2.0 * math.pi * 0.154 * float(p.U_inf) / (2.0 * float(p.R))
+    centre = (float(p.x_c) / h, float(p.y_c) / h, 0.5 * p.Nz)
+    amp = (0.0, -A / h, 0.0)
+    freq = w * float(p.dt)
+    if p.shape == "CYL":
+        law = ConstantMotion("cylinder", centre, (float(p.R) / h,), amp=amp, freq=freq)
+    else:
+        chord = float(p.chord) / h
+        law = ConstantMotion("naca4", centre, (chord,), amp=amp, freq=freq, rot0=rotation_z(float(p.theta0)),
+                             naca=naca_params(chord, p.naca_code or "0012", 0.25, X_INT))
+    return cfg, [law]

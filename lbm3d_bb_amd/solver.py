@@ -63,6 +63,7 @@ class Config:
     u0: Sequence[float] = (0.0, 0.0, 0.0)
     wall_point: str = "physical"  # physical | reference
     refill: bool = False
+    solid_bc: str = "SBB"  # SBB | IBB (solid_BC!'s bc, src/SolidBC.jl:129-153)
 
     @property
     def ncells(self) -> int:
@@ -78,6 +79,10 @@ class Config:
         c.storage = _ffi.STORAGE[self.storage.lower()]
         c.wall_point = _ffi.WALL_REFERENCE if self.wall_point == "reference" else _ffi.WALL_PHYSICAL
         c.refill = 1 if self.refill else 0
+        bc = self.solid_bc.upper()
+        if bc not in ("SBB", "IBB"):
+            raise ValueError(f"Unknown BC method: {self.solid_bc}")
+        c.solid_bc = _ffi.SOLID_SBB if bc == "SBB" else _ffi.SOLID_IBB
         c.omega_plus = float(self.omega_plus)
         c.omega_minus = float(self.omega_minus)
         for k in range(3):

@@ -94,6 +94,19 @@ int orc_inside_test(const lbm_body* b, float px, float py, float pz, const int32
     return orc_inside(&ob, px, py, pz, p, l);
 }
 
+/* The FP32 level set alone (interpolated bounce-back wall fractions). */
+float orc_phi_test(const lbm_body* b, float px, float py, float pz, const int32_t per[3], const int32_t L[3]) {
+    orc_body ob;
+    orc_prepare_body(b, &ob);
+    int p[3] = {per[0], per[1], per[2]}, l[3] = {L[0], L[1], L[2]};
+    return orc_phi(&ob, px, py, pz, p, l);
+}
+
+/* The FP32 NACA 4-digit lower / upper surface at chordwise position x. */
+void orc_naca_test(float x, float chord, float m, float p, float t, float x_int, float* yl, float* yu) {
+    orc_naca_lines(x, chord, m, p, t, x_int, yl, yu);
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     extern int omp_get_max_threads(void);

@@ -61,6 +61,10 @@ def load_oracle() -> ct.CDLL:
         lib.orc_inside_test.argtypes = [ct.POINTER(_ffi.LbmBody), ct.c_float, ct.c_float, ct.c_float,
                                         ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32)]
         lib.orc_inside_test.restype = ct.c_int
+        lib.orc_phi_test.argtypes = lib.orc_inside_test.argtypes
+        lib.orc_phi_test.restype = ct.c_float
+        lib.orc_naca_test.argtypes = [ct.c_float] * 6 + [ct.POINTER(ct.c_float)] * 2
+        lib.orc_naca_test.restype = None
         lib.orc_num_threads.restype = ct.c_int
         _LIB = lib
     return _LIB
@@ -94,3 +98,26 @@ def body_at(law, n: int) -> _ffi.LbmBody:
     out = _ffi.LbmBody()
     load_oracle().orc_body_at(ct.byref(law.to_c()), n, ct.byref(out))
     return out
+
+
+def _int3(v):
+    return (ct.c_int32 * 3)(*[int(a) for a in v])
+
+
+def inside(body, p, per=(0, 0, 0), L=(1 << 20,) * 3) -> bool:
+    """The oracle's FP32 inside test of one body at point p."""
+    b = body.to_c()
+    return bool(load_oracle().orc_inside_test(ct.byref(b), *[float(a) for a in p], _int3(per), _int3(L)))
+
+
+def phi(body, p, per=(0, 0, 0), L=(1 << 20,) * 3) -> float:
+    """The oracle's FP32 level set of one body at point p (half-step pose)."""
+    b = body.to_c()
+    return float(load_oracle().orc_phi_test(ct.byref(b), *[float(a) for a in p], _int3(per), _int3(L)))
+
+
+def naca_lines(x, chord, m, p, t, x_int=1.0089304):
+    """The oracle's FP32 NACA 4-digit (lower, upper) surface at chordwise x."""
+    yl, yu = ct.c_float(), ct.c_float()
+    load_oracle().orc_naca_test(x, chord, m, p, t, x_int, ct.byref(yl), ct.byref(yu))
+    return yl.value, yu.value

@@ -29,10 +29,12 @@ static const int OWCLS[19] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
 typedef struct {
     int shape;
     float c[3];
-    float r2;          /* sphere R*R */
+    float r2;          /* sphere, cylinder R*R */
     float ia2[3];      /* ellipsoid 1/(a*a), 1/(b*b), 1/(c*c) */
     float rot[9];
-    float reach;       /* bounding half-extent */
+    float semi[3];
+    float naca[5];     /* m, p, t, x_int, x_le_offset */
+    float rh[9];       /* half-step rotation, FP32 */
     double ch[3], v[3], om[3];
 } orc_body;
 
@@ -45,20 +47,60 @@ static void orc_prepare_body(const lbm_body* b, orc_body* o) {
         o->v[k] = b->vel[k];
         o->om[k] = b->omega[k];
     }
-    for (int k = 0; k < 9; ++k) o->rot[k] = b->rot[k];
-    if (b->shape == LBM_SPHERE) {
+    for (int k = 0; k < 9; ++k) {
+        o->rot[k] = b->rot[k];
+        o->rh[k] = (float)b->rot_half[k];
+    }
+    for (int k = 0; k < 3; ++k) o->semi[k] = b->semi[k];
+    for (int k = 0; k < 5; ++k) o->naca[k] = b->naca[k];
+    if (b->shape == LBM_SPHERE || b->shape == LBM_CYLINDER) {
         float R = b->semi[0];
         o->r2 = R * R;
-        o->reach = R;
-    } else {
-        float m = 0.0f;
+    } else if (b->shape == LBM_ELLIPSOID) {
         for (int k = 0; k < 3; ++k) {
             float a = b->semi[k];
             o->ia2[k] = 1.0f / (a * a);
-            if (a > m) m = a;
         }
-        o->reach = m;
     }
+}
+
+/* NACA 4-digit surfaces at chordwise position x from the leading edge, FP32,
+ * in the reference's operation order: _naca_thickness_y
+ * (src/Geometry.jl:325-330), _naca_camber_y_dydx (:332-345) and
+ * _naca_upper_lower_y (:347-356).  The camber line stays in chord fractions
+ * (not multiplied by the chord), as there.  xi^4 is evaluated in FP64 and
+ * rounded once (Julia's ^(::Float32, ::Integer) does that), and
+ * cos(atan(s)) = 1/sqrt(1 + s^2) exactly. */
+static void orc_naca_lines(float x, float chord, float m, float p, float t, float x_int, float* yl, float* yu) {
+    float xbar = x / chord;
+    float xi = x_int * xbar;
+    if (!(xi > 0.0f)) xi = 0.0f;
+    float rt = sqrtf(xi);
+    float x3 = xi * xi * xi;
+    double xd = (double)xi;
+    float x4 = (float)((xd * xd) * (xd * xd));
+    float poly = 0.2969f * rt - 0.1260f * xi;
+    poly = poly - 0.3516f * xi * xi;
+    poly = poly + 0.2843f * x3;
+    poly = poly - 0.1015f * x4;
+    float yt = 5.0f * t * chord * poly;
+    float yc = 0.0f, dyb = 0.0f;
+    if (m == 0.0f || p == 0.0f) {
+        yc = 0.0f;
+        dyb = 0.0f;
+    } else if (xbar < p) {
+        yc = m / (p * p) * (2.0f * p * xbar - xbar * xbar);
+        dyb = 2.0f * m / (p * p) * (p - xbar);
+    } else {
+        float den = 1.0f - p;
+        float den2 = den * den;
+        yc = m / den2 * ((1.0f - 2.0f * p) + 2.0f * p * xbar - xbar * xbar);
+        dyb = 2.0f * m / den2 * (p - xbar);
+    }
+    float sl = dyb / chord;
+    float cs = 1.0f / sqrtf(1.0f + sl * sl);
+    *yu = yc + yt * cs;
+    *yl = yc - yt * cs;
 }
 
 /* Minimum-image displacement along a periodic axis of length L (FP32). */
@@ -72,7 +114,9 @@ static float orc_min_image(float d, int periodic, int L) {
 }
 
 /* Inside test in FP32 with the reference's tolerance R^2 + 1e-6 R^2
- * (src/Geometry.jl:73), evaluated left to right with no contraction. */
+ * (src/Geometry.jl:73; the airfoil src/Geometry.jl:75-82 with its 1e-6 chord
+ * tolerance), evaluated left to right with no contraction.  The cylinder
+ * ignores z (extrusion); the sphere adds the z term. */
 static int orc_inside(const orc_body* b, float px, float py, float pz, const int per[3],
                       const int L[3]) {
     float dx = orc_min_image(px - b->c[0], per[0], L[0]);
@@ -82,6 +126,20 @@ static int orc_inside(const orc_body* b, float px, float py, float pz, const int
         float s = dx * dx + dy * dy;
         s = s + dz * dz;
         return s <= b->r2 + 1e-6f * b->r2;
+    } else if (b->shape == LBM_CYLINDER) {
+        float s = dx * dx + dy * dy;
+        return s <= b->r2 + 1e-6f * b->r2;
+    } else if (b->shape == LBM_NACA4) {
+        /* _rot_to_body (src/Geometry.jl:321-323): rot = [[c,-s,0],[s,c,0],[0,0,1]] */
+        float xb = b->rot[0] * dx + b->rot[3] * dy;
+        float yb = b->rot[1] * dx + b->rot[4] * dy;
+        float xle = xb + b->naca[4];
+        float chord = b->semi[0];
+        if (!(0.0f <= xle && xle <= chord)) return 0;
+        float yl, yu;
+        orc_naca_lines(xle, chord, b->naca[0], b->naca[1], b->naca[2], b->naca[3], &yl, &yu);
+        float tol = 1e-6f * chord;
+        return yl - tol <= yb && yb <= yu + tol;
     } else {
         float xb = b->rot[0] * dx + b->rot[3] * dy;
         xb = xb + b->rot[6] * dz;
@@ -93,6 +151,45 @@ static int orc_inside(const orc_body* b, float px, float py, float pz, const int
         q = q + zb * zb * b->ia2[2];
         return q <= 1.0f + 1e-6f;
     }
+}
+
+/* Level set at p (FP32, half-step pose), negative inside: phi_cylinder_lat
+ * (src/Geometry.jl:368-372) and phi_airfoil_lat (src/Geometry.jl:375-396) with
+ * the half-step centre and rotation (src/Motion.jl:104-116); sphere |d| - R;
+ * ellipsoid sqrt(sum (x_i/a_i)^2) - 1.  hypot(a, b) is sqrt(a^2 + b^2). */
+static float orc_phi(const orc_body* b, float px, float py, float pz, const int per[3], const int L[3]) {
+    float dx = orc_min_image(px - (float)b->ch[0], per[0], L[0]);
+    float dy = orc_min_image(py - (float)b->ch[1], per[1], L[1]);
+    float dz = orc_min_image(pz - (float)b->ch[2], per[2], L[2]);
+    if (b->shape == LBM_SPHERE) {
+        float s = dx * dx + dy * dy;
+        s = s + dz * dz;
+        return sqrtf(s) - b->semi[0];
+    }
+    if (b->shape == LBM_CYLINDER) {
+        float s = dx * dx + dy * dy;
+        return sqrtf(s) - b->semi[0];
+    }
+    if (b->shape == LBM_ELLIPSOID) {
+        float e = 0.0f;
+        for (int k = 0; k < 3; ++k) {
+            float r = b->rh[k] * dx + b->rh[3 + k] * dy;
+            r = r + b->rh[6 + k] * dz;
+            float t = r / b->semi[k];
+            e = e + t * t;
+        }
+        return sqrtf(e) - 1.0f;
+    }
+    float xb = b->rh[0] * dx + b->rh[3] * dy;
+    float yb = b->rh[1] * dx + b->rh[4] * dy;
+    float xle = xb + b->naca[4];
+    float chord = b->semi[0];
+    if (xle < 0.0f) return sqrtf(xle * xle + yb * yb);
+    if (xle > chord) return sqrtf((xle - chord) * (xle - chord) + yb * yb);
+    float yl, yu;
+    orc_naca_lines(xle, chord, b->naca[0], b->naca[1], b->naca[2], b->naca[3], &yl, &yu);
+    if (yb >= yl && yb <= yu) return -fminf(yu - yb, yb - yl);
+    return yb > yu ? yb - yu : yl - yb;
 }
 
 /* Rodrigues rotation about unit axis k by angle th, composed as Rot * R0. */
@@ -121,21 +218,30 @@ static void orc_rotation_at(const double om[3], double t, const double R0[9], do
 }
 
 /* motion_at for step n (1-based): mask pose at t = n-1, kinematics at t = n-1/2
- * (src/Motion.jl:26-36 with dt = 1). */
+ * (src/Motion.jl:26-36 with dt = 1).  Centre c0 + v t + A sin(w t + phase),
+ * velocity v + A w cos(w t + phase) (the reference's heave law,
+ * src/MainCore.jl:72-80, is A = (0, -0.4 R, 0), phase 0). */
 static void orc_body_from_law(const lbm_motion* m, int64_t n, lbm_body* b) {
     memset(b, 0, sizeof(*b));
     b->shape = m->shape;
     double t0 = (double)(n - 1), th = (double)(n - 1) + 0.5;
-    double R[9];
+    double R[9], Rh[9];
     orc_rotation_at(m->omega, t0, m->rot0, R);
+    orc_rotation_at(m->omega, th, m->rot0, Rh);
+    double s0 = sin(m->freq * t0 + m->phase), sh = sin(m->freq * th + m->phase);
+    double chh = cos(m->freq * th + m->phase);
     for (int k = 0; k < 3; ++k) {
-        b->center[k] = (float)(m->center0[k] + m->vel[k] * t0);
+        b->center[k] = (float)(m->center0[k] + m->vel[k] * t0 + m->amp[k] * s0);
         b->semi[k] = (float)m->semi[k];
-        b->center_half[k] = m->center0[k] + m->vel[k] * th;
-        b->vel[k] = m->vel[k];
+        b->center_half[k] = m->center0[k] + m->vel[k] * th + m->amp[k] * sh;
+        b->vel[k] = m->vel[k] + m->amp[k] * m->freq * chh;
         b->omega[k] = m->omega[k];
     }
-    for (int k = 0; k < 9; ++k) b->rot[k] = (float)R[k];
+    for (int k = 0; k < 9; ++k) {
+        b->rot[k] = (float)R[k];
+        b->rot_half[k] = Rh[k];
+    }
+    for (int k = 0; k < 5; ++k) b->naca[k] = (float)m->naca[k];
 }
 
 #endif /* ORACLE_GEOM_H */

@@ -299,10 +299,11 @@ static void SFX(orc_stream)(SFX(orc)* o) {
 
 /* Wall velocity u_w = U + Omega x (x_w - X) at the half step
  * (uw_at_wall_lat_planar, src/Geometry.jl:412-419, generalised to 3-D);
- * x_w = x + c_d/2 (physical) or x - c_d/2 (src/SolidBC.jl:195-196). */
+ * x_w = x + h c_d (physical) or x - h c_d (src/SolidBC.jl:195-196), h = 1/2 on a
+ * simple bounce-back link and q on an interpolated one (src/SolidBC.jl:324). */
 static void SFX(orc_wall_velocity)(const SFX(orc)* o, const orc_body* b, int x, int y, int z,
-                                   int d, int mode, REAL uw[3]) {
-    REAL sgn = mode == LBM_WALL_REFERENCE ? (REAL)-0.5 : (REAL)0.5;
+                                   int d, int mode, REAL h, REAL uw[3]) {
+    REAL sgn = mode == LBM_WALL_REFERENCE ? -h : h;
     REAL p[3] = {(REAL)x + (REAL)0.5 + sgn * (REAL)OCX[d], (REAL)y + (REAL)0.5 + sgn * (REAL)OCY[d],
                  (REAL)z + (REAL)0.5 + sgn * (REAL)OCZ[d]};
     REAL r[3];
@@ -320,7 +321,39 @@ static void SFX(orc_wall_velocity)(const SFX(orc)* o, const orc_body* b, int x, 
     uw[2] = (REAL)b->v[2] + (om[0] * r[1] - om[1] * r[0]);
 }
 
-/* SBB! over the CSR (src/SolidBC.jl:187-208). */
+/* Rule of the link (cell c, direction d into the solid) under the interpolated
+ * bounce-back (IBB!, src/SolidBC.jl:253-350, with the solid-side point taken
+ * at x + c_d and the second fluid node at x - c_d): 0 = simple bounce-back
+ * (no sign change of the level set, or q < 1/2 without a fluid second node),
+ * 1 = q < 1/2, 2 = q >= 1/2.  q = phi(x) / (phi(x) - phi(x + c_d)) in FP32,
+ * clamped to [1e-3, 1 - 1e-3] (EPS_Q, undefined in the reference). */
+static int SFX(orc_link_rule)(const SFX(orc)* o, int x, int y, int z, int d, const orc_body* b, float* q_out,
+                              int64_t* second) {
+    *q_out = 0.5f;
+    *second = -1;
+    if (o->cfg.solid_bc != LBM_SOLID_IBB) return 0;
+    float pb[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+    float phb = orc_phi(b, pb[0], pb[1], pb[2], o->per, o->L);
+    float phs = orc_phi(b, pb[0] + (float)OCX[d], pb[1] + (float)OCY[d], pb[2] + (float)OCZ[d], o->per, o->L);
+    float den = phb - phs;
+    if (!(phb > 0.0f && phs < 0.0f && fabsf(den) >= 1e-8f)) return 0;
+    float q = phb / den;
+    if (q < 1e-3f) q = 1e-3f;
+    if (q > 1.0f - 1e-3f) q = 1.0f - 1e-3f;
+    if (q >= 0.5f) {
+        *q_out = q;
+        return 2;
+    }
+    int64_t s2 = SFX(orc_nbr)(o, x, y, z, OOPP[d]);
+    if (s2 < 0 || o->flags[s2] != 0) return 0;
+    *q_out = q;
+    *second = s2;
+    return 1;
+}
+
+/* solid_BC! over the CSR: SBB! (src/SolidBC.jl:187-208) or IBB!
+ * (src/SolidBC.jl:253-350) with the moving-wall correction of Lallemand & Luo
+ * (2003), corr / (2q) on the q >= 1/2 branch. */
 static void SFX(orc_sbb)(SFX(orc)* o) {
     int64_t N = o->N;
     int nx = o->nx, ny = o->ny;
@@ -332,11 +365,25 @@ static void SFX(orc_sbb)(SFX(orc)* o) {
         int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
         int64_t s = SFX(orc_nbr)(o, x, y, z, d);
         const orc_body* b = &o->ob[o->flags[s] - 1];
+        float qf;
+        int64_t c2;
+        int rule = SFX(orc_link_rule)(o, x, y, z, d, b, &qf, &c2);
         REAL uw[3];
-        SFX(orc_wall_velocity)(o, b, x, y, z, d, o->cfg.wall_point, uw);
+        SFX(orc_wall_velocity)(o, b, x, y, z, d, o->cfg.wall_point, rule == 0 ? (REAL)0.5 : (REAL)qf, uw);
         REAL cuw = (REAL)OCX[d] * uw[0] + (REAL)OCY[d] * uw[1] + (REAL)OCZ[d] * uw[2];
-        o->f_old[(int64_t)OOPP[d] * N + c] =
-            o->f_new[(int64_t)d * N + c] - (REAL)2 * o->w[d] * o->rho[c] * cuw * (REAL)3;
+        REAL corr = (REAL)2 * o->w[d] * o->rho[c] * cuw * (REAL)3;
+        REAL fd = o->f_new[(int64_t)d * N + c];
+        REAL val;
+        if (rule == 1) {
+            REAL q2 = (REAL)2 * (REAL)qf;
+            val = q2 * fd + ((REAL)1 - q2) * o->f_new[(int64_t)d * N + c2] - corr;
+        } else if (rule == 2) {
+            REAL q2 = (REAL)2 * (REAL)qf;
+            val = (fd + (q2 - (REAL)1) * o->f_new[(int64_t)OOPP[d] * N + c] - corr) / q2;
+        } else {
+            val = fd - corr;
+        }
+        o->f_old[(int64_t)OOPP[d] * N + c] = val;
     }
 }
 
@@ -460,8 +507,12 @@ static void SFX(orc_mea)(SFX(orc)* o, double* out) {
         REAL tmp = o->f_new[(int64_t)d * N + c] + o->f_old[(int64_t)OOPP[d] * N + c];
         double t = (double)tmp;
         double P[3] = {t * OCX[d], t * OCY[d], t * OCZ[d]};
-        double r[3] = {(double)x + 0.5 + 0.5 * OCX[d] - b->ch[0], (double)y + 0.5 + 0.5 * OCY[d] - b->ch[1],
-                       (double)z + 0.5 + 0.5 * OCZ[d] - b->ch[2]};
+        /* arm to the wall point x + h c_d: h = 1/2 (SBB) or q (IBB) */
+        float qf;
+        int64_t c2;
+        double h = SFX(orc_link_rule)(o, x, y, z, d, b, &qf, &c2) == 0 ? 0.5 : (double)qf;
+        double r[3] = {(double)x + 0.5 + h * OCX[d] - b->ch[0], (double)y + 0.5 + h * OCY[d] - b->ch[1],
+                       (double)z + 0.5 + h * OCZ[d] - b->ch[2]};
         for (int a = 0; a < 3; ++a) {
             if (o->per[a]) {
                 double h = 0.5 * o->L[a];

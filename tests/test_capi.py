@@ -9,6 +9,7 @@ import re
 from lbm3d_bb_amd import _ffi
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(REPO, "include")
 
 
 def _declared():
@@ -34,13 +35,34 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def test_struct_layout_matches_header_sizes():
-    # lbm_face: 8 + 24; lbm_config: 8 ints (32) + 2 doubles + g[3] + rho0 + u0[3] (72) + 6 faces (192)
-    assert ct.sizeof(_ffi.LbmFace) == 32
-    assert ct.sizeof(_ffi.LbmConfig) == 32 + 72 + 6 * 32
-    # lbm_body: 8 + 3f + 3f + 9f + 1f (64) + 9 doubles (72)
-    assert ct.sizeof(_ffi.LbmBody) == 8 + 64 + 72
-    assert ct.sizeof(_ffi.LbmMotion) == 8 + 8 * (3 + 3 + 9 + 3 + 3)
+def test_struct_layout_matches_header(tmp_path):
+    """Every ctypes field offset and struct size equals the C compiler's for
+    include/lbm_hip.h (gcc on a generated probe)."""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"lbm_face": _ffi.LbmFace, "lbm_config": _ffi.LbmConfig, "lbm_body": _ffi.LbmBody,
+               "lbm_motion": _ffi.LbmMotion}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "lbm_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", INCLUDE, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == ct.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
 def test_missing_library_fails_loudly(tmp_path):

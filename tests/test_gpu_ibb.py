@@ -1,0 +1,190 @@
+"""The planar shapes and the interpolated bounce-back: HIP path vs the oracle (GPU).
+
+Same bars as the other parity tests: flags and links bit-exact; populations,
+moments and per-body force/torque within 1e-12 (FP64) or 1e-5 (FP32),
+normwise (forces relative to the momentum-exchange scale).  The reference's
+own input file (tests/golden/params_reference_default.txt: 1201 x 1201 x 3,
+heaving cylinder, IBB, BGK) runs at full size.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from lbm3d_bb_amd.bodies import ConstantMotion, cylinder, ellipsoid, heave_motion, naca4, naca_params, rotation_matrix
+from lbm3d_bb_amd.bodies import rotation_z, sphere
+from lbm3d_bb_amd.lattice import C as LC, W as LW
+from lbm3d_bb_amd.params import config_from_params, read_params_input
+from lbm3d_bb_amd.solver import Config, Solver, periodic
+from oracle.oracle import OracleSolver
+
+from cases import channel, force_err, force_scale, periodic_box, reference_channel, rel_err
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+def _pair(cfg, prec):
+    return Solver(cfg), OracleSolver(cfg, prec)
+
+
+def _compare_state(g, o, tol, what=""):
+    rg, ug = g.macros()
+    ro, uo = o.macros()
+    assert rel_err(rg, ro) < tol, f"{what} rho {rel_err(rg, ro)}"
+    assert rel_err(ug, uo) < tol * 1e2 or np.abs(ug - uo).max() < tol, f"{what} u {rel_err(ug, uo)}"
+    fg, fo = g.populations(), o.populations()
+    assert rel_err(fg, fo) < tol, f"{what} f {rel_err(fg, fo)}"
+
+
+def _same_geometry(g, o):
+    assert np.array_equal(g.flags(), o.flags())
+    cg, dg = g.links()
+    co, do = o.links()
+    assert len(co) > 0
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+
+
+def test_flags_and_links_bit_exact_cylinder_and_airfoil():
+    cfg = reference_channel(nx=96, ny=64, nz=4, storage="fp32")
+    for bodies in ([cylinder((30.3, 31.7, 2.0), 9.37)],
+                   [naca4((40.2, 32.9, 2.0), 48.0, "2412", theta=0.17)],
+                   [cylinder((20.0, 16.5, 2.0), 5.0), naca4((60.0, 40.0, 2.0), 30.0, "0015", theta=-0.4)]):
+        g, o = _pair(cfg, "fp32")
+        for s in (g, o):
+            s.set_bodies(bodies)
+            s.update_geometry()
+        _same_geometry(g, o)
+
+
+@pytest.mark.parametrize("collision", ["BGK", "TRT"])
+def test_ibb_static_cylinder_reference_channel_fp64(collision):
+    cfg = reference_channel(nx=64, ny=40, nz=4, storage="fp64")
+    cfg.collision = collision
+    cfg.solid_bc = "IBB"
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies([cylinder((20.3, 19.6, 2.0), 6.37)])
+        s.init_equilibrium()
+    Fg, Fo = g.step(60), o.step(60)
+    assert np.abs(Fo).max() > 0
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
+    _compare_state(g, o, TOL["fp64"], collision)
+
+
+def test_ibb_pitched_airfoil_and_3d_bodies_fp64():
+    cfg = periodic_box(n=40, collision="TRT", g=(2e-5, 5e-6, 0.0))
+    cfg.solid_bc = "IBB"
+    bodies = [naca4((14.2, 20.3, 20.0), 20.0, "2412", theta=0.14),
+              sphere((31.3, 9.8, 10.4), 4.6),
+              ellipsoid((30.1, 29.7, 28.2), (7.3, 4.1, 3.3), rot=rotation_matrix((0.2, 0.7, -0.4), 0.9))]
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies(bodies)
+        s.init_equilibrium()
+    Fg, Fo = g.step(50), o.step(50)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
+    _compare_state(g, o, TOL["fp64"], "mixed")
+
+
+def test_ibb_heaving_cylinder_scaled_reference_case_fp64():
+    """The reference's set-up (faces, BGK, IBB, heave law) on a small lattice."""
+    p = read_params_input(os.path.join(HERE, "golden", "params_reference_default.txt"))
+    p.Nx, p.Ny, p.Nz, p.R, p.x_c, p.y_c, p.Re = 120, 80, 3, 6.3, 40.5, 40.0, 20.0
+    cfg, laws = config_from_params(p, storage="fp64", refill=True)
+    # heave faster than the reference so that the mask moves within the run
+    laws[0].freq = 0.05
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_motion(laws)
+        s.init_equilibrium()
+    Fg, Fo = g.step(40), o.step(40)
+    _same_geometry(g, o)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
+    _compare_state(g, o, TOL["fp64"], "heave")
+
+
+@pytest.mark.parametrize("wall_point", ["physical", "reference"])
+def test_ibb_moving_airfoil_and_spinning_ellipsoid_fp64(wall_point):
+    cfg = channel(nx=64, ny=40, nz=36, storage="fp64")
+    cfg.solid_bc = "IBB"
+    cfg.wall_point = wall_point
+    laws = [ConstantMotion("naca4", (20.3, 20.1, 18.0), (24.0,), vel=(0.02, 0.01, 0.0), omega=(0.0, 0.0, 0.004),
+                           rot0=rotation_z(0.1), naca=naca_params(24.0, "4412")),
+            ConstantMotion("ellipsoid", (45.2, 19.7, 17.6), (6.0, 3.5, 3.0), omega=(0.01, 0.02, 0.03),
+                           rot0=rotation_matrix((1, 1, 0), 0.3))]
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_motion(laws)
+        s.init_equilibrium()
+    Fg, Fo = g.step(30), o.step(30)
+    _same_geometry(g, o)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
+    _compare_state(g, o, TOL["fp64"], wall_point)
+
+
+def test_ibb_cylinder_channel_fp32():
+    cfg = channel(nx=80, ny=48, nz=8, storage="fp32")
+    cfg.solid_bc = "IBB"
+    g, o = _pair(cfg, "fp32")
+    for s in (g, o):
+        s.set_motion([heave_motion("cylinder", (24.3, 24.2, 4.0), (7.4,), 7.4, 0.05)])
+        s.init_equilibrium()
+    Fg, Fo = g.step(50), o.step(50)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp32"], force_err(Fg, Fo, force_scale(o))
+    _compare_state(g, o, TOL["fp32"], "fp32")
+
+
+def test_ibb_uniform_flow_moving_with_bodies_stays_at_equilibrium():
+    """Exact steady solution: every population stays f_eq(1, U) when the walls move at U."""
+    U = (0.05, 0.02, -0.01)
+    cfg = Config(48, 40, 32, collision="TRT", storage="fp64", omega_plus=1 / 0.7, omega_minus=1 / 1.1,
+                 faces=[periodic()] * 6, u0=U, refill=True, solid_bc="IBB")
+    laws = [ConstantMotion("cylinder", (12.3, 11.8, 8.0), (5.3,), vel=U),
+            ConstantMotion("naca4", (30.2, 28.9, 8.0), (18.0,), vel=U, rot0=rotation_z(0.3),
+                           naca=naca_params(18.0, "2412")),
+            ConstantMotion("ellipsoid", (34.0, 12.0, 20.0), (6.0, 4.0, 3.0), vel=U,
+                           rot0=rotation_matrix((1, 2, 3), 0.5))]
+    g = Solver(cfg)
+    g.set_motion(laws)
+    g.init_equilibrium()
+    g.step(12)
+    cu = LC @ np.asarray(U)
+    feq = LW * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * np.dot(U, U))
+    assert np.abs(g.populations() - feq[:, None]).max() < 1e-13
+
+
+@pytest.mark.parametrize("storage", ["fp16s", "bf16s"])
+def test_ibb_16bit_storage_drag(storage):
+    """16-bit storage with the interpolated rule: drag of a cylinder in a channel
+    after 1500 steps within 1 % of the FP64 run."""
+    def drag(st):
+        cfg = channel(nx=96, ny=48, nz=4, storage=st, u_in=0.05, tau=0.6)
+        cfg.faces[4] = cfg.faces[5] = periodic()
+        cfg.solid_bc = "IBB"
+        s = Solver(cfg)
+        s.set_bodies([cylinder((30.3, 24.4, 2.0), 6.3)])
+        s.init_equilibrium()
+        F = s.step(1500)
+        return F[-200:, 0, 0].mean()
+
+    d64, d16 = drag("fp64"), drag(storage)
+    assert abs(d16 / d64 - 1) < 0.01, (d16, d64)
+
+
+def test_reference_default_case_full_size_fp32():
+    """The reference's own input file, at its full 1201 x 1201 x 3 size, 3 steps, FP32."""
+    p = read_params_input(os.path.join(HERE, "golden", "params_reference_default.txt"))
+    cfg, laws = config_from_params(p, storage="fp32")
+    g, o = _pair(cfg, "fp32")
+    for s in (g, o):
+        s.set_motion(laws)
+        s.init_equilibrium()
+    Fg, Fo = g.step(3), o.step(3)
+    _same_geometry(g, o)
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp32"], force_err(Fg, Fo, force_scale(o))
+    _compare_state(g, o, TOL["fp32"], "reference default")
