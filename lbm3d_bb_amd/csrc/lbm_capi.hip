@@ -189,7 +189,9 @@ lbm_body body_from_law(const lbm_motion& m, int64_t n) {
     return b;
 }
 
-DevBody to_dev(const lbm_body& b) {
+std::array<float, 3> reach_of(int shape, const double semi[3], const double naca[5], int nz);
+
+DevBody to_dev(const lbm_body& b, int nz) {
 #pragma clang fp contract(off)
     DevBody d;
     std::memset(&d, 0, sizeof(d));
@@ -207,6 +209,15 @@ DevBody to_dev(const lbm_body& b) {
     for (int k = 0; k < 3; ++k) d.semi[k] = b.semi[k];
     for (int k = 0; k < 5; ++k) d.naca[k] = b.naca[k];
     d.reach = 0.0f;
+    {
+        // bounding half-extents, widened by the neighbour offset and rounding
+        const double semi[3] = {b.semi[0], b.semi[1], b.semi[2]};
+        double naca[5];
+        for (int k = 0; k < 5; ++k) naca[k] = b.naca[k];
+        const std::array<float, 3> r = reach_of(b.shape, semi, naca, nz);
+        for (int k = 0; k < 3; ++k) d.ext[k] = r[k] * 1.0001f + 1.01f;
+        if (b.shape == LBM_CYLINDER || b.shape == LBM_NACA4) d.ext[2] = 3.0e38f;
+    }
     if (b.shape == LBM_SPHERE || b.shape == LBM_CYLINDER) {
         const float R = b.semi[0];
         d.r2 = R * R;
@@ -286,14 +297,18 @@ void index_box(const lbm_ctx* c, float lo_f, float hi_f, int axis, int& lo, int&
     ext = std::max(h - l + 1, 0);
 }
 
-// Domain edges: the lines where two non-periodic faces meet (corner cells are
-// shared by several lines; the first one owns them).
+// Domain edges: the lines where two non-periodic faces meet.  With three
+// non-periodic axes the corner cells lie on three lines: the lines of the
+// first axis pair keep them and the later lines stop one cell short, so the
+// edge boxes share no cell.
 void add_edge_boxes(const lbm_ctx* c, StepGeom& sg) {
     const int L[3] = {c->g.nx, c->g.ny, c->g.nz};
+    int pairs = 0;
     for (int a = 0; a < 3; ++a)
         for (int b = a + 1; b < 3; ++b) {
             if (((c->g.per >> a) & 1) || ((c->g.per >> b) & 1)) continue;
             const int o = 3 - a - b;
+            const bool trim = pairs++ > 0 && !((c->g.per >> o) & 1);
             for (int sa = 0; sa < 2; ++sa)
                 for (int sb = 0; sb < 2; ++sb) {
                     Box e{};
@@ -301,8 +316,8 @@ void add_edge_boxes(const lbm_ctx* c, StepGeom& sg) {
                     e.ext[a] = 1;
                     e.lo[b] = sb ? L[b] - 1 : 0;
                     e.ext[b] = 1;
-                    e.lo[o] = 0;
-                    e.ext[o] = L[o];
+                    e.lo[o] = trim ? 1 : 0;
+                    e.ext[o] = trim ? L[o] - 2 : L[o];
                     e.kind = 2;
                     e.face = -1;
                     sg.box[sg.nbox++] = e;
@@ -558,9 +573,9 @@ int full_geometry(lbm_ctx* c, const std::vector<DevBody>& bodies) {
 void bodies_for_step(lbm_ctx* c, int64_t n, std::vector<DevBody>& out) {
     out.clear();
     if (c->have_laws) {
-        for (const auto& m : c->laws) out.push_back(to_dev(body_from_law(m, n)));
+        for (const auto& m : c->laws) out.push_back(to_dev(body_from_law(m, n), c->g.nz));
     } else if (c->pending_set) {
-        for (const auto& b : c->pending) out.push_back(to_dev(b));
+        for (const auto& b : c->pending) out.push_back(to_dev(b, c->g.nz));
     } else {
         out = c->step_bodies;
     }

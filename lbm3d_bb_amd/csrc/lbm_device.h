@@ -681,6 +681,91 @@ __device__ __forceinline__ void face_row(const typename S::T* __restrict__ fin, 
     }
 }
 
+// One pair of cells on a row of a single y or z velocity / anti-bounce-back
+// face (16-bit storage): the face_pull of both cells with one pair load per
+// direction, as interior_row loads them.  Directions leaving the face take
+// the rule from the cells' own f*[opp(d)] (an aligned pair) and their face
+// slots; the others pull, with the x shift of interior_row (periodic x: the
+// element across the wrap is one extra load, selected by value).  The x-face
+// cells of such a row are domain edges and inactive, so no x rule applies.
+template <class S, int COLL, bool FORCED, int MODE, int AXIS, int SIDE>
+__device__ __forceinline__ void face_row_pair(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                              double* __restrict__ gout, double* __restrict__ rho_out,
+                                              double* __restrict__ u_out, const Grid& g, const Phys<typename S::C>& p,
+                                              const bool (&act_in)[2], const uint8_t* fixup_flags, int x0, int y,
+                                              int z) {
+    using C = typename S::C;
+    uint32_t fl = 0;
+    if (fixup_flags) fl = *reinterpret_cast<const uint16_t*>(fixup_flags);
+    const int nx = g.nx, nxy = (int)g.nxy;
+    const int row = nx * y + nxy * z;
+    const int fidx = 2 * AXIS + SIDE;
+    const int kind = g.face_kind[fidx];
+    const bool px = g.per & 1;
+    const bool left = x0 == 0, right = x0 + 2 == nx;
+    // periodic wraps of the axis along the face (zero when not periodic)
+    const int wym = (AXIS != 1 && y == 0) ? nxy : 0, wyp = (AXIS != 1 && y == g.ny - 1) ? -nxy : 0;
+    const int wzm = (AXIS != 2 && z == 0) ? (int)g.N : 0, wzp = (AXIS != 2 && z == g.nz - 1) ? -(int)g.N : 0;
+    const int wxm = left ? (px ? nx : 1) : 0, wxp = right ? (px ? -nx : -1) : 0;
+    const C uf[3] = {C(face_u_at(g, fidx, 0)), C(face_u_at(g, fidx, 1)), C(face_u_at(g, fidx, 2))};
+    const int slot0 = face_slot(g, fidx, x0, y, z);
+    C rho_prev[2], up[2][3], uu2[2];
+    side_read<C>(g, slot0, rho_prev[0], up[0], uu2[0]);
+    side_read<C>(g, slot0 + 1, rho_prev[1], up[1], uu2[1]);
+    C f[2][Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        const int ca = AXIS == 1 ? kCy[d] : kCz[d];
+        const bool out = SIDE == 0 ? (ca > 0) : (ca < 0);
+        if (out) {
+            C a, b;
+            S::load2(fin + (int64_t)kOpp[d] * g.P + row + x0, kOpp[d], a, b, g.sh);
+            f[0][d] = a;
+            f[1][d] = b;
+            continue;
+        }
+        int srow = row - (kCy[d] * nx + kCz[d] * nxy);
+        if (kCy[d] > 0) srow += wym;
+        if (kCy[d] < 0) srow += wyp;
+        if (kCz[d] > 0) srow += wzm;
+        if (kCz[d] < 0) srow += wzp;
+        const typename S::T* plane = fin + (int64_t)d * g.P + srow;
+        if (kCx[d] == 0) {
+            S::load2(plane + x0, d, f[0][d], f[1][d], g.sh);
+        } else if (kCx[d] > 0) {
+            C a, b;
+            S::load2(left ? plane + x0 : plane + x0 - 1, d, a, b, g.sh);
+            f[1][d] = left ? a : b;
+            const C w = S::load(plane + x0 - 1 + wxm, d, g.sh);
+            f[0][d] = px ? w : a;
+        } else {
+            C a, b;
+            S::load2(right ? plane + x0 : plane + x0 + 1, d, a, b, g.sh);
+            f[0][d] = right ? b : a;
+            const C w = S::load(plane + x0 + 2 + wxp, d, g.sh);
+            f[1][d] = px ? w : b;
+        }
+    }
+    // the rules (ALU only: every load above is issued)
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        const int ca = AXIS == 1 ? kCy[d] : kCz[d];
+        const bool out = SIDE == 0 ? (ca > 0) : (ca < 0);
+        if (!out) continue;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) f[v][d] = face_value<C>(kind, uf, d, f[v][d], rho_prev[v], up[v], uu2[v]);
+    }
+    bool act[2];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) act[v] = act_in[v] && !((fl >> (8 * v)) & kFlagFixup);
+    finish_group<S, COLL, FORCED, MODE, 2>(f, act, p, row + x0, g, fout, gout, rho_out, u_out);
+    if (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY) {
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+            if (act[v]) side_write<S, FORCED>(g, f[v], p, slot0 + v);
+    }
+}
+
 // x treatment of an interior row, a wave-uniform choice between branch-free
 // copies of the pull: periodic; zero-gradient faces only (clamps are value
 // selects); at least one velocity / anti-bounce-back face (its rule starts
@@ -868,7 +953,26 @@ __device__ __forceinline__ void bulk_group(const typename S::T* __restrict__ fin
     for (int v = 0; v < V; ++v) any = any || act[v];
     if (!any) return;
     if (nyz == 1) {
-        // a row on one y or z face: the whole wave takes the face path
+        // a row on one y or z face: the whole wave takes the face path (16-bit
+        // pairs on a velocity / anti-bounce-back face: the pair-load copy)
+        if constexpr (V == 2) {
+            const int fidx = ylo ? 2 : (yhi ? 3 : (zlo ? 4 : 5));
+            if (rule_face(g, fidx)) {
+                if (ylo)
+                    face_row_pair<S, COLL, FORCED, MODE, 1, 0>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags,
+                                                               x0, y, z);
+                else if (yhi)
+                    face_row_pair<S, COLL, FORCED, MODE, 1, 1>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags,
+                                                               x0, y, z);
+                else if (zlo)
+                    face_row_pair<S, COLL, FORCED, MODE, 2, 0>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags,
+                                                               x0, y, z);
+                else
+                    face_row_pair<S, COLL, FORCED, MODE, 2, 1>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags,
+                                                               x0, y, z);
+                return;
+            }
+        }
         if (ylo) face_row<S, COLL, FORCED, MODE, 1, 0>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
         else if (yhi)
             face_row<S, COLL, FORCED, MODE, 1, 1>(fin, fout, gout, rho_out, u_out, g, p, act, fixup_flags, x0, y, z);
@@ -976,19 +1080,134 @@ __device__ __forceinline__ void plain_cell(const typename S::T* __restrict__ fin
 // ---------------------------------------------------------------------------
 enum { DIR_PULL = 0, DIR_FACE = 1, DIR_LINK = 2 };
 
+// Link codes of the 18 pull sources, 10 bits per direction (kind, body).
+struct LinkCodes {
+    uint32_t w[(Q + 2) / 3];
+};
+
+// Solidity of a cell and of its 18 neighbours against the new pose of
+// analytic bodies, with inside_body's FP32 operations at every neighbour:
+// per body, the three offsets of each axis (neighbour coordinate + 1/2 minus
+// the centre, minimum image) are formed once and combined per neighbour, and a
+// body out of reach of every neighbour is skipped.  Sets the link codes of the
+// in-domain pull sources x - c_d (the first body containing a source wins, as
+// in flag_of) and returns the cell's own body + 1 (0: fluid).
+__device__ inline int link_codes_moving(const SolidView& sv, int x, int y, int z, const Grid& g, LinkCodes& lc) {
+#pragma clang fp contract(off)
+    const int q[3] = {x, y, z};
+    const int L[3] = {g.nx, g.ny, g.nz};
+    int qq[3][3];
+    uint32_t valid = 0u;  // bit 3a + k: neighbour offset k - 1 along axis a is in the domain
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            qq[a][k] = q[a] + k - 1;
+            if (wrap_coord(qq[a][k], L[a], (g.per >> a) & 1)) valid |= 1u << (3 * a + k);
+        }
+    uint32_t taken = 0u;  // bit e: neighbour x + c_e already inside an earlier body
+    int self = 0;
+    for (int b = 0; b < sv.nb; ++b) {
+        const DevBody& B = sv.bnew[b];
+        float dv[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dv[a][k] = min_image((float)qq[a][k] + 0.5f - B.c[a], (g.per >> a) & 1, L[a]);
+        if (fabsf(dv[0][1]) > B.ext[0] || fabsf(dv[1][1]) > B.ext[1] || fabsf(dv[2][1]) > B.ext[2]) continue;
+        uint32_t hit = 0u;
+        if (B.shape == LBM_SPHERE || B.shape == LBM_CYLINDER) {
+            float s2[3][3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) s2[a][k] = dv[a][k] * dv[a][k];
+            const float thr = B.r2 + 1e-6f * B.r2;
+            const bool sph = B.shape == LBM_SPHERE;
+#pragma unroll 1
+            for (int e = 0; e < 19; ++e) {
+                float t = s2[0][c_at(0, e) + 1] + s2[1][c_at(1, e) + 1];
+                if (sph) t = t + s2[2][c_at(2, e) + 1];
+                hit |= (t <= thr ? 1u : 0u) << e;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else if (B.shape == LBM_ELLIPSOID) {
+#pragma unroll 1
+            for (int e = 0; e < 19; ++e) {
+                const float dx = dv[0][c_at(0, e) + 1], dy = dv[1][c_at(1, e) + 1], dz = dv[2][c_at(2, e) + 1];
+                float xb = B.rot[0] * dx + B.rot[3] * dy;
+                xb = xb + B.rot[6] * dz;
+                float yb = B.rot[1] * dx + B.rot[4] * dy;
+                yb = yb + B.rot[7] * dz;
+                float zb = B.rot[2] * dx + B.rot[5] * dy;
+                zb = zb + B.rot[8] * dz;
+                float qe = xb * xb * B.ia2[0] + yb * yb * B.ia2[1];
+                qe = qe + zb * zb * B.ia2[2];
+                hit |= (qe <= 1.0f + 1e-6f ? 1u : 0u) << e;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+#pragma unroll 1
+            for (int e = 0; e < 19; ++e)
+                if (inside_naca(B, dv[0][c_at(0, e) + 1], dv[1][c_at(1, e) + 1])) hit |= 1u << e;
+        }
+        hit &= ~taken;
+        taken |= hit;
+        if ((hit >> 18) & 1u) self = b + 1;
+        // pull direction d reads the neighbour e = opp(d)
+#pragma unroll
+        for (int d = 0; d < 18; ++d) {
+            const int e = kOpp[d];
+            if ((hit >> e) & 1u) lc.w[d / 3] |= (uint32_t)(DIR_LINK | (b << 2)) << (10 * (d % 3));
+        }
+    }
+    // drop the sources outside a non-periodic face
+#pragma unroll
+    for (int d = 0; d < 18; ++d) {
+        const int e = kOpp[d];
+        const bool ok = ((valid >> (kCx[e] + 1)) & (valid >> (3 + kCy[e] + 1)) & (valid >> (6 + kCz[e] + 1)) & 1u) != 0u;
+        if (!ok) lc.w[d / 3] &= ~(0x3ffu << (10 * (d % 3)));
+    }
+    return self;
+}
+
+// The same from the flags (static bodies): the link codes of the in-domain
+// pull sources and the cell's own body + 1.
+__device__ inline int link_codes_flags(const SolidView& sv, int x, int y, int z, const Grid& g, LinkCodes& lc) {
+#pragma unroll
+    for (int d = 0; d < 18; ++d) {
+        int sx = x - kCx[d], sy = y - kCy[d], sz = z - kCz[d];
+        const bool okx = wrap_coord(sx, g.nx, g.per & 1), oky = wrap_coord(sy, g.ny, g.per & 2),
+                   okz = wrap_coord(sz, g.nz, g.per & 4);
+        const bool ok = okx && oky && okz;
+        // outside a non-periodic face: read the cell itself and drop it
+        const int sbr = solid_at(sv, ok ? sx : x, ok ? sy : y, ok ? sz : z, g);
+        const int sb = ok ? sbr : 0;
+        if (sb != 0) lc.w[d / 3] |= (uint32_t)(DIR_LINK | ((sb - 1) << 2)) << (10 * (d % 3));
+    }
+    return solid_at(sv, x, y, z, g);
+}
+
 template <class S, int COLL, bool FORCED, int MODE, bool IBB = false>
 __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                              double* __restrict__ gout, double* __restrict__ rho_out, double* __restrict__ u_out,
                              const Grid& g, const Phys<typename S::C>& p, const SolidView& sv, int x, int y, int z,
-                             ForceAcc& acc, double* lds_overflow, float* lds_q = nullptr) {
+                             ForceAcc& acc, double* lds_overflow, const double* lds_face_u, int pre_self,
+                             const LinkCodes pre, float* lds_q = nullptr) {
     using C = typename S::C;
     const int64_t N = g.N;
     const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
     const int L[3] = {g.nx, g.ny, g.nz};
     const int q0[3] = {x, y, z};
+    // link codes of the pull sources: from the caller (moving bodies: the
+    // analytic solidity it computed), or below from the flags (static bodies)
+    uint32_t codes[(Q + 2) / 3];
+#pragma unroll
+    for (int i = 0; i < (Q + 2) / 3; ++i) codes[i] = pre.w[i];
     // solid cells are streamed like fluid ones (no links), as in the reference
-    const bool self_solid = sv.nb > 0 && solid_at(sv, x, y, z, g) != 0;
-    const bool fresh = sv.nb > 0 && !self_solid && g.refill && fresh_at(sv, x, y, z, g);
+    const bool self_solid = sv.nb > 0 && pre_self != 0;
+    const bool fresh = sv.nb > 0 && !self_solid && g.refill && sv.mode == 1 &&
+                       flag_of(sv.bold, sv.nb, x, y, z, g) != 0;
     C f[Q];
     if (fresh) {
         // equilibrium of the mean previous-step rho, u of fluid neighbours
@@ -1040,26 +1259,26 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
     // solidity of every in-domain pull source x - c_d, in one memory round
     // trip; the body index of a link source is packed 10 bits per direction
     C rho_prev, up[3], uu2;
-    uint32_t codes[(Q + 2) / 3];
+    if (self_solid) {
 #pragma unroll
-    for (int i = 0; i < (Q + 2) / 3; ++i) codes[i] = 0u;
-    {
+        for (int i = 0; i < (Q + 2) / 3; ++i) codes[i] = 0u;
+    }
+    // a cell on a velocity or anti-bounce-back face reads the moments its face
+    // slot received when its f* was produced (one load, which then overlaps the
+    // pulls below); any other cell forms them from its own 19 populations
+    int slot = -1;
+#pragma unroll
+    for (int a = 2; a >= 0; --a) {
+        if ((g.per >> a) & 1) continue;
+        if (q0[a] == 0 && rule_face(g, 2 * a)) slot = face_slot(g, 2 * a, x, y, z);
+        if (q0[a] == L[a] - 1 && rule_face(g, 2 * a + 1)) slot = face_slot(g, 2 * a + 1, x, y, z);
+    }
+    if (slot >= 0) {
+        side_read<C>(g, slot, rho_prev, up, uu2);
+    } else {
         C own[Q];
 #pragma unroll
         for (int d = 0; d < Q; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
-        if (!self_solid && sv.nb > 0) {
-#pragma unroll
-            for (int d = 0; d < 18; ++d) {
-                int sx = x - kCx[d], sy = y - kCy[d], sz = z - kCz[d];
-                const bool okx = wrap_coord(sx, g.nx, g.per & 1), oky = wrap_coord(sy, g.ny, g.per & 2),
-                           okz = wrap_coord(sz, g.nz, g.per & 4);
-                const bool ok = okx && oky && okz;
-                // outside a non-periodic face: read the cell itself and drop it
-                const int sbr = solid_at(sv, ok ? sx : x, ok ? sy : y, ok ? sz : z, g);
-                const int sb = ok ? sbr : 0;
-                if (sb != 0) codes[d / 3] |= (uint32_t)(DIR_LINK | ((sb - 1) << 2)) << (10 * (d % 3));
-            }
-        }
         own_moments<S, FORCED>(own, p, rho_prev, up, uu2);
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -1150,7 +1369,8 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
         const uint32_t code = (codes[d / 3] >> (10 * (d % 3))) & 0x3ffu;
         const int kind_d = (int)(code & 3u), aux = (int)(code >> 2);
         if (kind_d == DIR_FACE) {
-            const C uf[3] = {C(face_u_at(g, aux, 0)), C(face_u_at(g, aux, 1)), C(face_u_at(g, aux, 2))};
+            // face velocities from LDS: a global load here would wait once per face direction
+            const C uf[3] = {C(lds_face_u[3 * aux]), C(lds_face_u[3 * aux + 1]), C(lds_face_u[3 * aux + 2])};
             f[d] = face_value<C>(face_kind_at(g, aux), uf, d, f[d], rho_prev, up, uu2);
         } else if (kind_d == DIR_LINK) {
             // boundary link (x, dl), dl = opp(d): the solid node is x + c_dl
@@ -1265,9 +1485,12 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
     // then the bodies (new poses, then old), when few enough
     const bool staged = nb <= kLdsBodies;
     DevBody* s_bodies = reinterpret_cast<DevBody*>(lds + (kWaves + 1) * n6);
-    // then, with the interpolated bounce-back, the wall fractions (18 per thread)
-    float* s_q = reinterpret_cast<float*>(reinterpret_cast<char*>(s_bodies) +
-                                          (staged ? 2 * (size_t)nb * sizeof(DevBody) : 0));
+    // then the faces' wall / inlet velocities (6 x 3), then, with the interpolated
+    // bounce-back, the wall fractions (18 per thread)
+    double* s_face_u = reinterpret_cast<double*>(reinterpret_cast<char*>(s_bodies) +
+                                                 (staged ? 2 * (size_t)nb * sizeof(DevBody) : 0));
+    float* s_q = reinterpret_cast<float*>(s_face_u + 18);
+    if (tid < 18) s_face_u[tid] = g.face_u[tid];
     if (staged) {
         const int nw = 2 * nb * (int)(sizeof(DevBody) / 4);
         const uint32_t* src_new = reinterpret_cast<const uint32_t*>(bnew);
@@ -1291,6 +1514,9 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
 
     bool work = false, plain = false;
     int x = 0, y = 0, z = 0;
+    // moving bodies: the cell's own body and its link codes (general_cell's format)
+    int pre_self = 0;
+    LinkCodes pre = {{0u, 0u, 0u, 0u, 0u, 0u, 0u}};
     if (blk < sg.nfix_blocks) {
         // the box of this block: the number of boxes starting at or before
         // it, from independent loads of the first kScanBoxes start blocks
@@ -1336,11 +1562,15 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
                     for (int m = sg.first_fix_box; m < bi && !dup; ++m) dup = in_box(sg.box[m], x, y, z, g);
                 }
             }
-            if (!dup && sv.mode == 1 && write_flags) {
-                // moving bodies: the region's flags follow the new pose (the
-                // static fix-up bits are rebuilt when the bodies stop)
-                const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
-                flags[cell] = (uint8_t)flag_of(sv.bnew, nb, x, y, z, g);
+            if (!dup && sv.mode == 1) {
+                // moving bodies: solidity of the cell and its neighbours, once
+                pre_self = link_codes_moving(sv, x, y, z, g, pre);
+                if (write_flags) {
+                    // the region's flags follow the new pose (the static fix-up
+                    // bits are rebuilt when the bodies stop)
+                    const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
+                    flags[cell] = (uint8_t)pre_self;
+                }
             }
             if (b.kind == 2) {
                 work = !dup;  // domain edge: every cell
@@ -1356,18 +1586,10 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
                         // a fluid cell with a solid neighbour, or a fresh one, takes
                         // the general path
                         bool gen = false;
-                        if (flag_of(sv.bnew, nb, x, y, z, g) == 0) {
-                            gen = g.refill && fresh_at(sv, x, y, z, g);
-                            const int L[3] = {g.nx, g.ny, g.nz};
+                        if (pre_self == 0) {
+                            gen = g.refill && flag_of(sv.bold, nb, x, y, z, g) != 0;
 #pragma unroll
-                            for (int d = 0; d < 18; ++d) {
-                                int q[3] = {x + kCx[d], y + kCy[d], z + kCz[d]};
-                                bool ok = true;
-#pragma unroll
-                                for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(q[a], L[a], (g.per >> a) & 1);
-                                const int sb = flag_of(sv.bnew, nb, ok ? q[0] : x, ok ? q[1] : y, ok ? q[2] : z, g);
-                                gen = gen || (ok && sb != 0);
-                            }
+                            for (int i = 0; i < 7; ++i) gen = gen || pre.w[i] != 0u;
                         }
                         plain = !gen;
                     }
@@ -1387,12 +1609,13 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
             work = open_face_count(x, y, z, g) == 0;  // face cells belong to the regions
         }
     }
+    if (work && !plain && sv.mode == 0 && nb > 0) pre_self = link_codes_flags(sv, x, y, z, g, pre);
     if (work && (MODE == MODE_STEP || MODE == MODE_GET)) {
         if (plain)
             plain_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
         else
             general_cell<S, COLL, FORCED, MODE, IBB>(fin, fout, gout, rho_out, u_out, g, p, sv, x, y, z, acc, s_over,
-                                                     s_q);
+                                                     s_face_u, pre_self, pre, s_q);
     }
     if (MODE == MODE_STEP && nb > 0) {
         const int wave = tid >> 6;
@@ -1429,10 +1652,11 @@ inline BulkShape bulk_shape(const Grid& g) {
 }
 
 // dynamic LDS of a step launch: the fix-up blocks' force partials, staged
-// bodies and (interpolated bounce-back) wall fractions
+// bodies, face velocities and (interpolated bounce-back) wall fractions
 inline size_t step_lds_bytes(int nbodies, bool ibb = false) {
     size_t b = (size_t)(kWaves + 1) * (size_t)nbodies * 6 * sizeof(double);
     if (nbodies <= kLdsBodies) b += 2 * (size_t)nbodies * sizeof(DevBody);
+    b += 18 * sizeof(double);
     if (ibb) b += 18 * (size_t)kBlock * sizeof(float);
     return b;
 }

@@ -35,8 +35,8 @@ struct DevBody {
     float reach;   // 0: analytic shape; < 0: unknown (explicit flags)
     float semi[3];
     float naca[5];  // NACA4: m, p, t, x_int, x_le_offset
-    float rh[9];  // rotation at the half step
-    float pad_;
+    float rh[9];   // rotation at the half step
+    float ext[3];  // per axis: beyond this offset of the cell centre no neighbour is inside
     double ch[3];
     double v[3];
     double om[3];
