@@ -78,6 +78,8 @@ int orc_step(orc_handle* h, int32_t nsteps, double* force_out) { return DISPATCH
 int orc_get_macros(orc_handle* h, double* rho, double* u) { return DISPATCH(h, orc_get_macros, rho, u); }
 int orc_get_populations(orc_handle* h, double* f) { return DISPATCH(h, orc_get_populations, f); }
 int orc_conservation(orc_handle* h, double* mass, double* mom) { return DISPATCH(h, orc_conservation, mass, mom); }
+/* lbm_sync's counterpart: the oracle runs synchronously. */
+int orc_sync(orc_handle* h) { return h ? LBM_OK : LBM_ERR_ARG; }
 
 /* Pose of a motion law at step n (1-based), exposed so tests can pin it. */
 int orc_body_at(const lbm_motion* m, int64_t n, lbm_body* out) {

@@ -34,6 +34,7 @@ _SIGS = {
     "get_populations": [_P, _D],
     "set_populations": [_P, _D],
     "conservation": [_P, _D, _D],
+    "sync": [_P],
 }
 
 _LIB = None

@@ -188,3 +188,41 @@ def test_reference_default_case_full_size_fp32():
     _same_geometry(g, o)
     assert force_err(Fg, Fo, force_scale(o)) < TOL["fp32"], force_err(Fg, Fo, force_scale(o))
     _compare_state(g, o, TOL["fp32"], "reference default")
+
+
+def test_run_reference_driver_outputs_gpu(tmp_path):
+    """The reference's loop on the GPU (scaled input file, IBB, heave): its HDF5
+    outputs hold the oracle's coefficients and snapshots within the FP32 bar."""
+    h5io = pytest.importorskip("lbm3d_bb_amd.h5io")
+    try:
+        h5io.lib()
+    except ImportError as e:
+        pytest.skip(f"HDF5 C library not available: {e}")
+    from lbm3d_bb_amd.params import aero_coefficients
+    from lbm3d_bb_amd.run_reference import run
+
+    p = read_params_input(os.path.join(HERE, "golden", "params_reference_default.txt"))
+    p.Nx, p.Ny, p.Nz, p.R, p.x_c, p.y_c, p.Re = 96, 64, 3, 7.3, 32.5, 32.2, 40.0
+    out = str(tmp_path / "output")
+    run("", out, n_max=50, n_save=20, params=p, log=lambda m: None)
+    cfg, laws = config_from_params(p)
+    o = OracleSolver(cfg, "fp32")
+    o.set_motion(laws)
+    o.init_equilibrium()
+    F, snaps = [], []
+    for k in (20, 20, 10):
+        F.append(o.step(k))
+        if len(F) < 3:
+            snaps.append(o.macros())
+    F = np.concatenate(F)[:, 0, :]
+    cl, cd, _ = aero_coefficients(F, p)
+    scale = force_scale(o) / (0.5 * p.U_inf ** 2 * p.Nz * 2 * p.R)  # coefficient of the momentum-exchange scale
+    with h5io.H5File(os.path.join(out, "aero_coeffs.h5")) as f:
+        assert np.abs(f.read("CD") - cd).max() / scale < TOL["fp32"]
+        assert np.abs(f.read("CL") - cl).max() / scale < TOL["fp32"]
+    with h5io.H5File(os.path.join(out, "final_fields.h5")) as f:
+        assert list(f.read("steps")) == [20, 40, 0]
+        rho, u = f.read("rho"), f.read("u")
+        for s, (r, uu) in enumerate(snaps):
+            assert rel_err(rho[s].ravel(), r) < TOL["fp32"]
+            assert np.abs(u[s].ravel() - uu[0]).max() < TOL["fp32"]
