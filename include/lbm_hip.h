@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LBM_Q 19
+#define LBM_Q 19  /* populations per cell of the default lattice; a D3Q27 context holds 27 */
 #define LBM_MAX_BODIES 127
 
 typedef enum {
@@ -108,6 +108,8 @@ typedef struct {
     int32_t refill;       /* 0: none (reference); 1: fresh fluid nodes get f_eq of the
                              average of their fluid neighbours' rho, u */
     int32_t solid_bc;     /* lbm_solid_bc */
+    int32_t q;            /* lattice: 19 (D3Q19) or 27 (D3Q27), src/Utilities.jl:945-993; 0 means 19 */
+    int32_t pad_;
     double omega_plus;    /* omega_+ = 1/tau_+ (BGK: omega)        src/Parameters.jl:69-72 */
     double omega_minus;   /* omega_- = 1/tau_- (TRT only)          src/Parameters.jl:70-72 */
     double g[3];          /* body acceleration, Guo forcing with F = rho g (0 = off) */

@@ -40,7 +40,7 @@ class LbmConfig(ct.Structure):
     _fields_ = [
         ("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32),
         ("collision", ct.c_int32), ("storage", ct.c_int32), ("wall_point", ct.c_int32),
-        ("refill", ct.c_int32), ("solid_bc", ct.c_int32),
+        ("refill", ct.c_int32), ("solid_bc", ct.c_int32), ("q", ct.c_int32), ("pad_", ct.c_int32),
         ("omega_plus", ct.c_double), ("omega_minus", ct.c_double),
         ("g", ct.c_double * 3), ("rho0", ct.c_double), ("u0", ct.c_double * 3),
         ("faces", LbmFace * 6),

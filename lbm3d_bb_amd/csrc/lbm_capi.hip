@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "lbm_q27.h"
 #include "lbm_types.h"
 
 using namespace lbm;
@@ -43,8 +44,16 @@ struct lbm_ctx {
     double* d_face_u = nullptr;              // [6][3] face velocities
     size_t side_bytes = 0;
     int cur = 0;               // buffer holding f* (pending stream)
-    size_t fbytes = 0;         // 19 planes of P elements
+    size_t fbytes = 0;         // q planes of P elements
     uint8_t* flags = nullptr;
+    int q = 19;                // lattice: D3Q19 or D3Q27
+
+    // D3Q27: flags before the last pose change, and the last step's bodies
+    // (inside the device step records) for the getters' replay
+    uint8_t* flags_old = nullptr;
+    const DevBody* q27_bnew = nullptr;
+    const DevBody* q27_bold = nullptr;
+    int q27_refill = 0;
 
     // static boundary-node list (solidity from flags)
     int32_t* list = nullptr;
@@ -508,18 +517,25 @@ int ensure_partials(lbm_ctx* c, int64_t blocks) {
 }
 
 // Ordered compaction of the boundary nodes (and optionally the links) from flags.
+// Per-block count buffers of at least n entries.
+int ensure_counts(lbm_ctx* c, int64_t n) {
+    if (n <= c->cnt_cap) return LBM_OK;
+    dev_free(c, c->cnt_a, (size_t)c->cnt_cap * 4);
+    dev_free(c, c->cnt_b, (size_t)c->cnt_cap * 4);
+    c->cnt_a = c->cnt_b = nullptr;
+    c->cnt_cap = 0;
+    int rc = dev_alloc(c, (void**)&c->cnt_a, (size_t)n * 4);
+    if (rc) return rc;
+    rc = dev_alloc(c, (void**)&c->cnt_b, (size_t)n * 4);
+    if (rc) return rc;
+    c->cnt_cap = n;
+    return LBM_OK;
+}
+
 int build_list(lbm_ctx* c, int32_t* d_link_cell, int8_t* d_link_dir, int64_t* nlinks_out) {
     const int64_t nblk = nblocks_of(c->g.N);
-    if (nblk + 1 > c->cnt_cap) {
-        dev_free(c, c->cnt_a, (size_t)c->cnt_cap * 4);
-        dev_free(c, c->cnt_b, (size_t)c->cnt_cap * 4);
-        c->cnt_a = c->cnt_b = nullptr;
-        c->cnt_cap = nblk + 1;
-        int rc = dev_alloc(c, (void**)&c->cnt_a, (size_t)c->cnt_cap * 4);
-        if (rc) return rc;
-        rc = dev_alloc(c, (void**)&c->cnt_b, (size_t)c->cnt_cap * 4);
-        if (rc) return rc;
-    }
+    int rc0 = ensure_counts(c, nblk + 1);
+    if (rc0) return rc0;
     GEOCHK(c, launch_bb_count(c->flags, c->g, c->cnt_a, c->cnt_b, c->stream));
     GEOCHK(c, launch_scan2(c->cnt_a, c->cnt_b, (int)nblk, c->stream));
     int32_t tot[2];
@@ -563,6 +579,7 @@ int full_geometry(lbm_ctx* c, const std::vector<DevBody>& bodies) {
     dev_free(c, d_b, std::max<size_t>(bodies.size(), 1) * sizeof(DevBody));
     c->mask_bodies = bodies;
     c->geom_valid = true;
+    if (c->q == 27) return LBM_OK;  // the D3Q27 step finds its links from the flags
     rc = build_list(c, nullptr, nullptr, nullptr);
     if (rc) return rc;
     c->list_valid = true;
@@ -689,7 +706,121 @@ hipEvent_t next_event(lbm_ctx* c) {
     return c->ev_pool[c->ev_used++];
 }
 
+q27::StepArgs q27_args(const lbm_ctx* c, int mode, const void* fin, void* fout) {
+    q27::StepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.mode = mode;
+    a.storage = c->cfg.storage;
+    a.trt = c->cfg.collision == LBM_TRT;
+    a.forced = c->forced ? 1 : 0;
+    a.g = c->g;
+    a.wp = c->pd.wp;
+    a.wm = c->pd.wm;
+    for (int k = 0; k < 3; ++k) a.grav[k] = c->pd.g[k];
+    a.stream = c->stream;
+    a.fin = fin;
+    a.fout = fout;
+    a.fl_new = c->flags;
+    a.fl_old = c->flags;
+    a.nb = c->nb;
+    a.face_u = c->d_face_u;
+    return a;
+}
+
+int q27_step_launch(lbm_ctx* c, const q27::StepArgs& a) {
+    const int e = q27_launch_step(a);
+    if (e) return fail(c, LBM_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString((hipError_t)e));
+    return LBM_OK;
+}
+
+// D3Q27 steps: per step the bodies of the new pose (and of the old one when it
+// changed) go into the step record; a pose change rebuilds the flags (the old
+// ones kept for the refill) before the step's single launch.
+int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
+    const int par = c->parity;
+    if (c->rec_ev_used[par]) HIPCHK(c, hipEventSynchronize(c->rec_ev[par]));
+    const int nb = c->nb;
+    std::vector<char> moved((size_t)nsteps, 0);
+    std::vector<DevBody> bnew;
+    int nprep = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        bool moving = false;
+        if (c->flags_only) {
+            bnew = c->step_bodies;
+        } else {
+            bodies_for_step(c, c->step + 1 + s, bnew);
+            if (nb > 0 && !c->geom_valid) {
+                if (s > 0) break;  // the full build runs before any launch of the chunk
+                int rc = full_geometry(c, bnew);
+                if (rc) return rc;
+            } else {
+                for (int i = 0; i < nb; ++i)
+                    if (!same_mask_pose(bnew[i], c->mask_bodies[i])) moving = true;
+            }
+        }
+        DevBody* rb = (DevBody*)(c->h_rec[par] + (size_t)s * c->rec_bytes);
+        for (int i = 0; i < nb; ++i) {
+            rb[i] = bnew[i];
+            rb[nb + i] = moving ? c->mask_bodies[i] : bnew[i];
+        }
+        if (moving) c->mask_bodies = bnew;
+        c->step_bodies = bnew;
+        moved[(size_t)s] = moving ? 1 : 0;
+        ++nprep;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_rec, c->h_rec[par], (size_t)nprep * c->rec_bytes, hipMemcpyHostToDevice, c->stream));
+    const int64_t nblk = nblocks_of(c->g.N);
+    const int64_t stride = nblk * std::max(nb, 1) * 6;
+    int rc = ensure_partials(c, nblk * std::max(nprep, 1));
+    if (rc) return rc;
+    const int n6 = nb * 6;
+    for (int s = 0; s < nprep; ++s) {
+        const DevBody* dn = (const DevBody*)(c->d_rec + (size_t)s * c->rec_bytes);
+        const bool mv = moved[(size_t)s] != 0;
+        if (mv) {
+            HIPCHK(c, hipMemcpyAsync(c->flags_old, c->flags, (size_t)c->g.N, hipMemcpyDeviceToDevice, c->stream));
+            GEOCHK(c, launch_flags_full(c->flags, dn, nb, c->g, c->stream));
+        }
+        q27::StepArgs a = q27_args(c, MODE_STEP, c->fbuf[c->cur], c->fbuf[1 - c->cur]);
+        a.fl_old = mv ? c->flags_old : c->flags;
+        a.bnew = dn;
+        a.bold = mv ? dn + nb : nullptr;
+        a.refill = (c->cfg.refill && mv) ? 1 : 0;
+        a.partials = c->d_partials + (size_t)s * stride;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool sample = c->profiling && ((c->step + s) % kProfileStride == 0);
+        if (sample) {
+            e0 = next_event(c);
+            e1 = next_event(c);
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+        }
+        rc = q27_step_launch(c, a);
+        if (rc) return rc;
+        if (sample) HIPCHK(c, hipEventRecord(e1, c->stream));
+        c->cur = 1 - c->cur;
+        c->q27_bnew = a.bnew;
+        c->q27_bold = a.bold;
+        c->q27_refill = a.refill;
+        c->last_rec_valid = true;
+        c->raw_state = false;
+    }
+    if (n6 > 0 && nprep > 0)
+        GEOCHK(c, q27_launch_force_sum(c->d_partials, nblk, stride, n6, nprep, c->d_forces, c->stream));
+    if (force_out && n6 > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->h_forces[par], c->d_forces, (size_t)nprep * n6 * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipEventRecord(c->rec_ev[par], c->stream));
+    c->rec_ev_used[par] = true;
+    c->parity = 1 - par;
+    c->step += nprep;
+    c->pending_set = false;
+    *done = nprep;
+    return LBM_OK;
+}
+
 int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
+    if (c->q == 27) return run_chunk27(c, nsteps, force_out, done);
     // the pinned buffers of this parity are free once their last copy finished
     const int par = c->parity;
     if (c->rec_ev_used[par]) HIPCHK(c, hipEventSynchronize(c->rec_ev[par]));
@@ -748,6 +879,7 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
 }
 
 int collide_only(lbm_ctx* c) {
+    if (c->q == 27) return q27_step_launch(c, q27_args(c, MODE_COLLIDE_ONLY, c->fbuf[1 - c->cur], c->fbuf[c->cur]));
     return launch_bulk(c, MODE_COLLIDE_ONLY, c->fbuf[1 - c->cur], c->fbuf[c->cur], nullptr, nullptr, nullptr, nullptr);
 }
 
@@ -755,6 +887,18 @@ int collide_only(lbm_ctx* c) {
 int get_state(lbm_ctx* c, double* d_f, double* d_rho, double* d_u) {
     if (!c->initialised) return fail(c, LBM_ERR_STATE, "not initialised");
     const void* prev = c->fbuf[1 - c->cur];
+    if (c->q == 27) {
+        if (!c->raw_state && !c->last_rec_valid) return fail(c, LBM_ERR_STATE, "no completed step to read");
+        q27::StepArgs a = q27_args(c, c->raw_state ? MODE_GET_RAW : MODE_GET, prev, nullptr);
+        a.gout = d_f;
+        a.rho_out = d_rho;
+        a.u_out = d_u;
+        a.bnew = c->q27_bnew;
+        a.bold = c->q27_bold;
+        a.fl_old = c->q27_bold ? c->flags_old : c->flags;
+        a.refill = c->q27_refill;
+        return q27_step_launch(c, a);
+    }
     if (c->raw_state) return launch_bulk(c, MODE_GET_RAW, prev, nullptr, d_f, d_rho, d_u, nullptr);
     if (!c->last_rec_valid) return fail(c, LBM_ERR_STATE, "no completed step to read");
     return launch_step(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0, nullptr);
@@ -764,12 +908,51 @@ int get_state(lbm_ctx* c, double* d_f, double* d_rho, double* d_u) {
 
 extern "C" {
 
+// D3Q27 links from the flags: per-block counts, their scan, then the fill.
+static int get_links27(lbm_ctx* c, int32_t* cell, int8_t* dir, int64_t cap, int64_t* nnz) {
+    const int64_t nblk = nblocks_of(c->g.N);
+    int rc = ensure_counts(c, nblk + 1);
+    if (rc) return rc;
+    GEOCHK(c, q27_launch_link_count(c->flags, c->g, c->cnt_a, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->cnt_b, 0, (size_t)(nblk + 1) * 4, c->stream));
+    GEOCHK(c, launch_scan2(c->cnt_a, c->cnt_b, (int)nblk, c->stream));
+    int32_t tot = 0;
+    HIPCHK(c, hipMemcpyAsync(&tot, c->cnt_a + nblk, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t need = tot;
+    if (nnz) *nnz = need;
+    if (!cell && !dir) return LBM_OK;
+    if (need > cap) return fail(c, LBM_ERR_CAPACITY, "link buffer too small");
+    int32_t* d_cell = nullptr;
+    int8_t* d_dir = nullptr;
+    rc = dev_alloc(c, (void**)&d_cell, (size_t)std::max<int64_t>(need, 1) * 4);
+    if (rc == LBM_OK) rc = dev_alloc(c, (void**)&d_dir, (size_t)std::max<int64_t>(need, 1));
+    if (rc == LBM_OK) {
+        const int e = q27_launch_link_fill(c->flags, c->g, c->cnt_a, d_cell, d_dir, c->stream);
+        if (e) rc = fail(c, LBM_ERR_HIP, std::string("link fill: ") + hipGetErrorString((hipError_t)e));
+    }
+    if (rc == LBM_OK && need) {
+        if (cell && hipMemcpyAsync(cell, d_cell, (size_t)need * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = fail(c, LBM_ERR_HIP, "link copy");
+        if (dir && hipMemcpyAsync(dir, d_dir, (size_t)need, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = fail(c, LBM_ERR_HIP, "link copy");
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && rc == LBM_OK) rc = fail(c, LBM_ERR_HIP, "link sync");
+    dev_free(c, d_cell, (size_t)std::max<int64_t>(need, 1) * 4);
+    dev_free(c, d_dir, (size_t)std::max<int64_t>(need, 1));
+    return rc;
+}
+
 int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     if (!cfg || !out) return LBM_ERR_ARG;
     *out = nullptr;
     if (cfg->collision != LBM_BGK && cfg->collision != LBM_TRT) return LBM_ERR_ARG;
     if (cfg->storage < LBM_FP64 || cfg->storage > LBM_BF16S) return LBM_ERR_ARG;
     if (cfg->solid_bc != LBM_SOLID_SBB && cfg->solid_bc != LBM_SOLID_IBB) return LBM_ERR_ARG;
+    const int q = cfg->q == 0 ? 19 : cfg->q;
+    if (q != 19 && q != 27) return LBM_ERR_ARG;
+    // D3Q27: FP64 / FP32 storage and simple bounce-back
+    if (q == 27 && (cfg->storage > LBM_FP32 || cfg->solid_bc != LBM_SOLID_SBB)) return LBM_ERR_ARG;
     const int L[3] = {cfg->nx, cfg->ny, cfg->nz};
     for (int a = 0; a < 3; ++a) {
         const int k0 = cfg->faces[2 * a].kind, k1 = cfg->faces[2 * a + 1].kind;
@@ -786,6 +969,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
 
     lbm_ctx* c = new lbm_ctx();
     c->cfg = *cfg;
+    c->cfg.q = q;
+    c->q = q;
     c->g.nx = cfg->nx;
     c->g.ny = cfg->ny;
     c->g.nz = cfg->nz;
@@ -838,7 +1023,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     // (N * 4 B is exactly 2^24 B for a 256x128x128 lattice).
     const int64_t align = 256 / bytes;
     c->g.P = (N + align - 1) / align * align + 2304 / bytes;
-    c->fbytes = (size_t)19 * c->g.P * bytes;
+    c->fbytes = (size_t)q * c->g.P * bytes;
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return LBM_ERR_HIP;
@@ -863,6 +1048,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         c->g.face_u = c->d_face_u;
     }
     if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->flags, (size_t)N);
+    if (rc == LBM_OK && q == 27) rc = dev_alloc(c, (void**)&c->flags_old, (size_t)N);
     if (rc == LBM_OK) rc = dev_alloc(c, (void**)&c->list_count, 16);
     if (rc == LBM_OK) {
         if (hipMemsetAsync(c->flags, 0, (size_t)N, c->stream) != hipSuccess ||
@@ -891,7 +1077,7 @@ int lbm_destroy(lbm_ctx* c) {
         if (c->h_forces[i]) (void)hipHostFree(c->h_forces[i]);
         if (c->rec_ev[i]) (void)hipEventDestroy(c->rec_ev[i]);
     }
-    void* ptrs[] = {c->flags, c->list, c->list_count, c->cnt_a, c->cnt_b, c->d_rec, c->d_partials, c->d_forces,
+    void* ptrs[] = {c->flags, c->flags_old, c->list, c->list_count, c->cnt_a, c->cnt_b, c->d_rec, c->d_partials, c->d_forces,
                     c->d_face_u};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1019,10 +1205,11 @@ int lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
     HIPCHK(c, hipMemcpyAsync(c->flags, flags, (size_t)c->g.N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->geom_valid = true;
+    c->last_rec_valid = false;
+    if (c->q == 27) return LBM_OK;
     rc = build_list(c, nullptr, nullptr, nullptr);
     if (rc) return rc;
     c->list_valid = true;
-    c->last_rec_valid = false;
     return LBM_OK;
 }
 
@@ -1048,6 +1235,7 @@ int lbm_get_flags(lbm_ctx* c, uint8_t* flags) {
 
 int lbm_get_links(lbm_ctx* c, int32_t* cell, int8_t* dir, int64_t cap, int64_t* nnz) {
     if (!c) return LBM_ERR_ARG;
+    if (c->q == 27) return get_links27(c, cell, dir, cap, nnz);
     // count first
     int64_t need = 0;
     int rc = build_list(c, nullptr, nullptr, &need);
@@ -1085,7 +1273,11 @@ int lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* u) {
     if (rc) return rc;
     if (rho) HIPCHK(c, hipMemcpyAsync(d_rho, rho, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
     if (u) HIPCHK(c, hipMemcpyAsync(d_u, u, (size_t)3 * N * 8, hipMemcpyHostToDevice, c->stream));
-    {
+    if (c->q == 27) {
+        const int e = q27_launch_equilibrium(c->cfg.storage, c->g, c->stream, c->fbuf[1 - c->cur], d_rho, d_u,
+                                             c->cfg.rho0, c->cfg.u0);
+        if (e) rc = fail(c, LBM_ERR_HIP, std::string("equilibrium: ") + hipGetErrorString((hipError_t)e));
+    } else {
         LaunchReq r = base_req(c, K_EQUILIBRIUM, 0);
         r.fout = c->fbuf[1 - c->cur];
         r.src_rho = d_rho;
@@ -1110,10 +1302,14 @@ int lbm_set_populations(lbm_ctx* c, const double* f) {
     if (!c || !f) return LBM_ERR_ARG;
     const int64_t N = c->g.N;
     double* d = nullptr;
-    int rc = dev_alloc(c, (void**)&d, (size_t)19 * N * 8);
+    const size_t bytes = (size_t)c->q * N * 8;
+    int rc = dev_alloc(c, (void**)&d, bytes);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(d, f, (size_t)19 * N * 8, hipMemcpyHostToDevice, c->stream));
-    {
+    HIPCHK(c, hipMemcpyAsync(d, f, bytes, hipMemcpyHostToDevice, c->stream));
+    if (c->q == 27) {
+        const int e = q27_launch_from_double(c->cfg.storage, c->g, c->stream, c->fbuf[1 - c->cur], d);
+        if (e) rc = fail(c, LBM_ERR_HIP, std::string("populations: ") + hipGetErrorString((hipError_t)e));
+    } else {
         LaunchReq r = base_req(c, K_FROM_DOUBLE, 0);
         r.fout = c->fbuf[1 - c->cur];
         r.src_f = d;
@@ -1121,7 +1317,7 @@ int lbm_set_populations(lbm_ctx* c, const double* f) {
     }
     if (rc == LBM_OK) rc = collide_only(c);
     if (rc == LBM_OK) HIPCHK(c, hipStreamSynchronize(c->stream));
-    dev_free(c, d, (size_t)19 * N * 8);
+    dev_free(c, d, bytes);
     if (rc) return rc;
     c->step = 0;
     c->initialised = true;
@@ -1198,13 +1394,14 @@ int lbm_get_populations(lbm_ctx* c, double* f) {
     if (!c || !f) return LBM_ERR_ARG;
     const int64_t N = c->g.N;
     double* d = nullptr;
-    int rc = dev_alloc(c, (void**)&d, (size_t)19 * N * 8);
+    const size_t bytes = (size_t)c->q * N * 8;
+    int rc = dev_alloc(c, (void**)&d, bytes);
     if (rc == LBM_OK) rc = get_state(c, d, nullptr, nullptr);
     if (rc == LBM_OK) {
-        HIPCHK(c, hipMemcpyAsync(f, d, (size_t)19 * N * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(f, d, bytes, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    dev_free(c, d, (size_t)19 * N * 8);
+    dev_free(c, d, bytes);
     return rc;
 }
 

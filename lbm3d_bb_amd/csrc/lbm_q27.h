@@ -1,0 +1,51 @@
+// lbm_q27.h -- the D3Q27 kernels (lbm_q27.hip) as the host code launches them.
+#pragma once
+
+#include "lbm_types.h"
+
+namespace lbm {
+namespace q27 {
+
+constexpr int kQ = 27;
+
+// One launch of k27_step: a time step (MODE_STEP), the getters' replay of the
+// last step (MODE_GET) or of the start state (MODE_GET_RAW), or the collision
+// of the start state (MODE_COLLIDE_ONLY).
+struct StepArgs {
+    int mode;
+    int storage;  // LBM_FP64 or LBM_FP32
+    int trt;
+    int forced;
+    Grid g;
+    double wp, wm;
+    double grav[3];
+    hipStream_t stream;
+    const void* fin;
+    void* fout;
+    double* gout;     // MODE_GET*: f_old, 27 planes of N
+    double* rho_out;  // MODE_GET*: N
+    double* u_out;    // MODE_GET*: 3 x N
+    const uint8_t* fl_new;
+    const uint8_t* fl_old;  // flags before this step's pose change (== fl_new when the pose did not change)
+    const DevBody* bnew;    // nb bodies of the step's pose
+    const DevBody* bold;    // nb bodies of the previous pose, or null when the pose did not change
+    int nb;
+    double* partials;  // MODE_STEP: [blocks][nb][6]
+    int refill;
+    const double* face_u;  // [6][3]
+};
+
+}  // namespace q27
+
+int q27_launch_step(const q27::StepArgs& a);
+// out[s][k] = sum over blocks b of partials[s * stride + b * n6 + k], s < nsteps
+int q27_launch_force_sum(const double* partials, int64_t nblocks, int64_t stride, int n6, int nsteps, double* out,
+                         hipStream_t st);
+int q27_launch_equilibrium(int storage, const Grid& g, hipStream_t st, void* f, const double* rho, const double* u,
+                           double rho0, const double u0[3]);
+int q27_launch_from_double(int storage, const Grid& g, hipStream_t st, void* f, const double* src);
+int q27_launch_link_count(const uint8_t* flags, const Grid& g, int32_t* counts, hipStream_t st);
+int q27_launch_link_fill(const uint8_t* flags, const Grid& g, const int32_t* block_off, int32_t* cells, int8_t* dirs,
+                         hipStream_t st);
+
+}  // namespace lbm

@@ -64,6 +64,7 @@ class Config:
     wall_point: str = "physical"  # physical | reference
     refill: bool = False
     solid_bc: str = "SBB"  # SBB | IBB (solid_BC!'s bc, src/SolidBC.jl:129-153)
+    q: int = 19            # lattice: 19 (D3Q19) | 27 (D3Q27), params.txt's q
 
     @property
     def ncells(self) -> int:
@@ -83,6 +84,9 @@ class Config:
         if bc not in ("SBB", "IBB"):
             raise ValueError(f"Unknown BC method: {self.solid_bc}")
         c.solid_bc = _ffi.SOLID_SBB if bc == "SBB" else _ffi.SOLID_IBB
+        if self.q not in (19, 27):
+            raise ValueError(f"Invalid q value: {self.q}")
+        c.q = int(self.q)
         c.omega_plus = float(self.omega_plus)
         c.omega_minus = float(self.omega_minus)
         for k in range(3):
@@ -219,12 +223,13 @@ class Solver:
         return rho, u.reshape(3, self.N)
 
     def populations(self) -> np.ndarray:
-        f = np.empty(19 * self.N)
+        q = self.config.q
+        f = np.empty(q * self.N)
         self._call("get_populations", _dp(f))
-        return f.reshape(19, self.N)
+        return f.reshape(q, self.N)
 
     def set_populations(self, f: np.ndarray) -> None:
-        a = np.ascontiguousarray(f, dtype=np.float64).reshape(19 * self.N)
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(self.config.q * self.N)
         self._call("set_populations", _dp(a))
 
     def conservation(self) -> tuple[float, np.ndarray]:

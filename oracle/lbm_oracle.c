@@ -38,6 +38,7 @@ typedef struct {
 int orc_create(const lbm_config* cfg, int prec, orc_handle** out) {
     if (!cfg || !out || (prec != 0 && prec != 1)) return LBM_ERR_ARG;
     if (cfg->nx < 2 || cfg->ny < 2 || cfg->nz < 2) return LBM_ERR_ARG;
+    if (cfg->q != 0 && cfg->q != 19 && cfg->q != 27) return LBM_ERR_ARG;
     for (int a = 0; a < 3; ++a) {
         int lo = cfg->faces[2 * a].kind == LBM_FACE_PERIODIC, hi = cfg->faces[2 * a + 1].kind == LBM_FACE_PERIODIC;
         if (lo != hi) return LBM_ERR_ARG;

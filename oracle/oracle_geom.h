@@ -25,6 +25,22 @@ static const int OOPP[19] = {1, 0, 3, 2, 5, 4, 7, 6, 9, 8, 11, 10, 13, 12, 15, 1
 /* weight class: 0 -> 1/18, 1 -> 1/36, 2 -> 1/3 (src/Utilities.jl:954-958). */
 static const int OWCLS[19] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2};
 
+/* D3Q27 in the reference order (src/Utilities.jl:962-976), 0-based: the D3Q19
+ * moving directions, then the eight corners, then rest. */
+static const int OCX27[27] = {1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0,
+                              1, -1, 1, -1, 1, -1, -1, 1, 0};
+static const int OCY27[27] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1,
+                              1, -1, 1, -1, -1, 1, 1, -1, 0};
+static const int OCZ27[27] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1,
+                              1, -1, -1, 1, 1, -1, 1, -1, 0};
+static const int OOPP27[27] = {1, 0, 3, 2, 5, 4, 7, 6, 9, 8, 11, 10, 13, 12, 15, 14, 17, 16,
+                               19, 18, 21, 20, 23, 22, 25, 24, 26};
+/* D3Q19 rest direction in the 27-entry layout of the oracle's tables */
+static const int OCX19[27] = {1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0};
+static const int OCY19[27] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 0};
+static const int OCZ19[27] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 0};
+static const int OOPP19[27] = {1, 0, 3, 2, 5, 4, 7, 6, 9, 8, 11, 10, 13, 12, 15, 14, 17, 16, 18};
+
 /* Per-body data prepared on the host once per pose (FP32 mask, FP64 kinematics). */
 typedef struct {
     int shape;

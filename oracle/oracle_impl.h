@@ -25,7 +25,9 @@ typedef struct {
     int nx, ny, nz;
     int64_t N;
     int per[3], L[3];
-    REAL w[19];
+    int q;                           /* 19 or 27 */
+    const int *cx, *cy, *cz, *opp;   /* lattice tables (src/Utilities.jl:945-993) */
+    REAL w[27];
     REAL *f_old, *f_new;
     REAL *rho, *ux, *uy, *uz;
     uint8_t *flags, *fresh;
@@ -44,9 +46,13 @@ typedef struct {
     int initialised;
 } SFX(orc);
 
-static REAL SFX(orc_wgt)(int d) {
+/* D3Q19 weights 1/18, 1/36, 1/3 (src/Utilities.jl:954-958); D3Q27 weights
+ * 2/27, 1/54, 1/216, 8/27 by the number of non-zero components (:967-973). */
+static REAL SFX(orc_wgt)(int q, int d) {
     const REAL one = (REAL)1;
-    return OWCLS[d] == 0 ? one / (REAL)18 : (OWCLS[d] == 1 ? one / (REAL)36 : one / (REAL)3);
+    if (q == 19) return OWCLS[d] == 0 ? one / (REAL)18 : (OWCLS[d] == 1 ? one / (REAL)36 : one / (REAL)3);
+    int n = (OCX27[d] != 0) + (OCY27[d] != 0) + (OCZ27[d] != 0);
+    return n == 0 ? (REAL)8 / (REAL)27 : (n == 1 ? (REAL)2 / (REAL)27 : (n == 2 ? one / (REAL)54 : one / (REAL)216));
 }
 
 static void* SFX(orc_create)(const lbm_config* cfg) {
@@ -61,10 +67,15 @@ static void* SFX(orc_create)(const lbm_config* cfg) {
     o->L[1] = cfg->ny;
     o->L[2] = cfg->nz;
     for (int a = 0; a < 3; ++a) o->per[a] = cfg->faces[2 * a].kind == LBM_FACE_PERIODIC;
-    for (int d = 0; d < 19; ++d) o->w[d] = SFX(orc_wgt)(d);
+    o->q = cfg->q == 27 ? 27 : 19;
+    o->cx = o->q == 27 ? OCX27 : OCX19;
+    o->cy = o->q == 27 ? OCY27 : OCY19;
+    o->cz = o->q == 27 ? OCZ27 : OCZ19;
+    o->opp = o->q == 27 ? OOPP27 : OOPP19;
+    for (int d = 0; d < o->q; ++d) o->w[d] = SFX(orc_wgt)(o->q, d);
     int64_t N = o->N;
-    o->f_old = (REAL*)calloc((size_t)(19 * N), sizeof(REAL));
-    o->f_new = (REAL*)calloc((size_t)(19 * N), sizeof(REAL));
+    o->f_old = (REAL*)calloc((size_t)(o->q * N), sizeof(REAL));
+    o->f_new = (REAL*)calloc((size_t)(o->q * N), sizeof(REAL));
     o->rho = (REAL*)calloc((size_t)N, sizeof(REAL));
     o->ux = (REAL*)calloc((size_t)N, sizeof(REAL));
     o->uy = (REAL*)calloc((size_t)N, sizeof(REAL));
@@ -100,7 +111,7 @@ static void SFX(orc_destroy)(void* p) {
 /* Neighbour of cell (x,y,z) along c_d: wraps on periodic axes, -1 outside
  * (nbr_lli, src/Utilities.jl:556-572; the wrap is an extension). */
 static int64_t SFX(orc_nbr)(const SFX(orc)* o, int x, int y, int z, int d) {
-    int q[3] = {x + OCX[d], y + OCY[d], z + OCZ[d]};
+    int q[3] = {x + o->cx[d], y + o->cy[d], z + o->cz[d]};
     for (int a = 0; a < 3; ++a) {
         if (q[a] < 0 || q[a] >= o->L[a]) {
             if (!o->per[a]) return -1;
@@ -150,7 +161,7 @@ static int SFX(orc_build_geometry)(SFX(orc)* o, int keep_flags) {
         int cnt = 0;
         if (o->flags[c] == 0) {
             int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
-            for (int d = 0; d < 19; ++d) {
+            for (int d = 0; d < o->q; ++d) {
                 int64_t b = SFX(orc_nbr)(o, x, y, z, d);
                 if (b >= 0 && o->flags[b] != 0) ++cnt;
             }
@@ -172,7 +183,7 @@ static int SFX(orc_build_geometry)(SFX(orc)* o, int keep_flags) {
         int64_t i = o->bb_ptr[c];
         if (i == o->bb_ptr[c + 1]) continue;
         int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
-        for (int d = 0; d < 19; ++d) {
+        for (int d = 0; d < o->q; ++d) {
             int64_t b = SFX(orc_nbr)(o, x, y, z, d);
             if (b >= 0 && o->flags[b] != 0) {
                 o->bb_cell[i] = (int32_t)c;
@@ -188,8 +199,8 @@ static int SFX(orc_build_geometry)(SFX(orc)* o, int keep_flags) {
 
 /* Equilibrium in the reference's BGK form w rho (1 + 3cu + 4.5cu^2 - 1.5u^2)
  * (src/Initialization.jl:37-42). */
-static REAL SFX(orc_feq)(REAL w, REAL rho, REAL ux, REAL uy, REAL uz, int d) {
-    REAL cu = (REAL)OCX[d] * ux + (REAL)OCY[d] * uy + (REAL)OCZ[d] * uz;
+static REAL SFX(orc_feq)(const SFX(orc)* o, REAL w, REAL rho, REAL ux, REAL uy, REAL uz, int d) {
+    REAL cu = (REAL)o->cx[d] * ux + (REAL)o->cy[d] * uy + (REAL)o->cz[d] * uz;
     REAL u2 = ux * ux + uy * uy + uz * uz;
     REAL cu2 = cu * cu;
     return w * rho * ((REAL)1 + cu * (REAL)3 + (REAL)0.5 * cu2 * (REAL)9 - (REAL)0.5 * u2 * (REAL)3);
@@ -204,12 +215,12 @@ static void SFX(orc_macros)(SFX(orc)* o) {
 #pragma omp parallel for schedule(static)
     for (int64_t c = 0; c < N; ++c) {
         REAL r = 0, mx = 0, my = 0, mz = 0;
-        for (int d = 0; d < 19; ++d) {
+        for (int d = 0; d < o->q; ++d) {
             REAL f = o->f_old[(int64_t)d * N + c];
             r += f;
-            mx += f * (REAL)OCX[d];
-            my += f * (REAL)OCY[d];
-            mz += f * (REAL)OCZ[d];
+            mx += f * (REAL)o->cx[d];
+            my += f * (REAL)o->cy[d];
+            mz += f * (REAL)o->cz[d];
         }
         o->rho[c] = r;
         o->ux[c] = mx / r;
@@ -240,20 +251,20 @@ static void SFX(orc_collide)(SFX(orc)* o) {
         REAL u2 = ux * ux + uy * uy + uz * uz;
         REAL Fx = rho * gx, Fy = rho * gy, Fz = rho * gz;
         REAL uF = ux * Fx + uy * Fy + uz * Fz;
-        for (int d = 0; d < 19; ++d) {
+        for (int d = 0; d < o->q; ++d) {
             REAL w = o->w[d];
             REAL fi = o->f_old[(int64_t)d * N + c];
-            REAL cu = (REAL)OCX[d] * ux + (REAL)OCY[d] * uy + (REAL)OCZ[d] * uz;
+            REAL cu = (REAL)o->cx[d] * ux + (REAL)o->cy[d] * uy + (REAL)o->cz[d] * uz;
             REAL cu2 = cu * cu;
             REAL out;
             REAL Sp = 0, Sm = 0;
             if (forced) {
-                REAL cF = (REAL)OCX[d] * Fx + (REAL)OCY[d] * Fy + (REAL)OCZ[d] * Fz;
+                REAL cF = (REAL)o->cx[d] * Fx + (REAL)o->cy[d] * Fy + (REAL)o->cz[d] * Fz;
                 Sp = w * ((REAL)9 * cu * cF - (REAL)3 * uF);
                 Sm = w * ((REAL)3 * cF);
             }
             if (trt) {
-                REAL fo = o->f_old[(int64_t)OOPP[d] * N + c];
+                REAL fo = o->f_old[(int64_t)o->opp[d] * N + c];
                 REAL feq_p = w * rho * ((REAL)1 + (REAL)0.5 * cu2 * (REAL)9 - (REAL)0.5 * u2 * (REAL)3);
                 REAL feq_m = w * rho * (cu * (REAL)3);
                 REAL fp = (REAL)0.5 * (fi + fo);
@@ -281,8 +292,8 @@ static void SFX(orc_stream)(SFX(orc)* o) {
 #pragma omp parallel for schedule(static)
     for (int64_t c = 0; c < N; ++c) {
         int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
-        for (int d = 0; d < 19; ++d) {
-            int s[3] = {x - OCX[d], y - OCY[d], z - OCZ[d]};
+        for (int d = 0; d < o->q; ++d) {
+            int s[3] = {x - o->cx[d], y - o->cy[d], z - o->cz[d]};
             int out = 0;
             for (int a = 0; a < 3; ++a) {
                 if (s[a] < 0 || s[a] >= o->L[a]) {
@@ -304,8 +315,8 @@ static void SFX(orc_stream)(SFX(orc)* o) {
 static void SFX(orc_wall_velocity)(const SFX(orc)* o, const orc_body* b, int x, int y, int z,
                                    int d, int mode, REAL h, REAL uw[3]) {
     REAL sgn = mode == LBM_WALL_REFERENCE ? -h : h;
-    REAL p[3] = {(REAL)x + (REAL)0.5 + sgn * (REAL)OCX[d], (REAL)y + (REAL)0.5 + sgn * (REAL)OCY[d],
-                 (REAL)z + (REAL)0.5 + sgn * (REAL)OCZ[d]};
+    REAL p[3] = {(REAL)x + (REAL)0.5 + sgn * (REAL)o->cx[d], (REAL)y + (REAL)0.5 + sgn * (REAL)o->cy[d],
+                 (REAL)z + (REAL)0.5 + sgn * (REAL)o->cz[d]};
     REAL r[3];
     for (int a = 0; a < 3; ++a) {
         r[a] = p[a] - (REAL)b->ch[a];
@@ -334,7 +345,7 @@ static int SFX(orc_link_rule)(const SFX(orc)* o, int x, int y, int z, int d, con
     if (o->cfg.solid_bc != LBM_SOLID_IBB) return 0;
     float pb[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
     float phb = orc_phi(b, pb[0], pb[1], pb[2], o->per, o->L);
-    float phs = orc_phi(b, pb[0] + (float)OCX[d], pb[1] + (float)OCY[d], pb[2] + (float)OCZ[d], o->per, o->L);
+    float phs = orc_phi(b, pb[0] + (float)o->cx[d], pb[1] + (float)o->cy[d], pb[2] + (float)o->cz[d], o->per, o->L);
     float den = phb - phs;
     if (!(phb > 0.0f && phs < 0.0f && fabsf(den) >= 1e-8f)) return 0;
     float q = phb / den;
@@ -344,7 +355,7 @@ static int SFX(orc_link_rule)(const SFX(orc)* o, int x, int y, int z, int d, con
         *q_out = q;
         return 2;
     }
-    int64_t s2 = SFX(orc_nbr)(o, x, y, z, OOPP[d]);
+    int64_t s2 = SFX(orc_nbr)(o, x, y, z, o->opp[d]);
     if (s2 < 0 || o->flags[s2] != 0) return 0;
     *q_out = q;
     *second = s2;
@@ -370,7 +381,7 @@ static void SFX(orc_sbb)(SFX(orc)* o) {
         int rule = SFX(orc_link_rule)(o, x, y, z, d, b, &qf, &c2);
         REAL uw[3];
         SFX(orc_wall_velocity)(o, b, x, y, z, d, o->cfg.wall_point, rule == 0 ? (REAL)0.5 : (REAL)qf, uw);
-        REAL cuw = (REAL)OCX[d] * uw[0] + (REAL)OCY[d] * uw[1] + (REAL)OCZ[d] * uw[2];
+        REAL cuw = (REAL)o->cx[d] * uw[0] + (REAL)o->cy[d] * uw[1] + (REAL)o->cz[d] * uw[2];
         REAL corr = (REAL)2 * o->w[d] * o->rho[c] * cuw * (REAL)3;
         REAL fd = o->f_new[(int64_t)d * N + c];
         REAL val;
@@ -379,11 +390,11 @@ static void SFX(orc_sbb)(SFX(orc)* o) {
             val = q2 * fd + ((REAL)1 - q2) * o->f_new[(int64_t)d * N + c2] - corr;
         } else if (rule == 2) {
             REAL q2 = (REAL)2 * (REAL)qf;
-            val = (fd + (q2 - (REAL)1) * o->f_new[(int64_t)OOPP[d] * N + c] - corr) / q2;
+            val = (fd + (q2 - (REAL)1) * o->f_new[(int64_t)o->opp[d] * N + c] - corr) / q2;
         } else {
             val = fd - corr;
         }
-        o->f_old[(int64_t)OOPP[d] * N + c] = val;
+        o->f_old[(int64_t)o->opp[d] * N + c] = val;
     }
 }
 
@@ -403,8 +414,8 @@ static void SFX(orc_faces)(SFX(orc)* o) {
         for (int a = 0; a < 3; ++a)
             if (!o->per[a] && (q[a] == 0 || q[a] == o->L[a] - 1)) onface = 1;
         if (!onface) continue;
-        for (int d = 0; d < 19; ++d) {
-            int cd[3] = {OCX[d], OCY[d], OCZ[d]};
+        for (int d = 0; d < o->q; ++d) {
+            int cd[3] = {o->cx[d], o->cy[d], o->cz[d]};
             int s[3];
             int face_of_axis[3] = {-1, -1, -1};
             int any = 0;
@@ -434,12 +445,12 @@ static void SFX(orc_faces)(SFX(orc)* o) {
             if (kind == LBM_FACE_VELOCITY) {
                 const double* U = o->cfg.faces[chosen].u;
                 REAL cu = (REAL)cd[0] * (-(REAL)U[0]) + (REAL)cd[1] * (-(REAL)U[1]) + (REAL)cd[2] * (-(REAL)U[2]);
-                val = o->f_new[(int64_t)OOPP[d] * N + c] - (REAL)2 * w * o->rho[c] * cu * (REAL)3;
+                val = o->f_new[(int64_t)o->opp[d] * N + c] - (REAL)2 * w * o->rho[c] * cu * (REAL)3;
             } else if (kind == LBM_FACE_PRESSURE_ABB) {
                 REAL uw = o->ux[c], vw = o->uy[c], ww = o->uz[c];
                 REAL cu = (REAL)cd[0] * uw + (REAL)cd[1] * vw + (REAL)cd[2] * ww;
                 REAL cu2 = cu * cu, uu2 = uw * uw + vw * vw + ww * ww;
-                val = -o->f_new[(int64_t)OOPP[d] * N + c] +
+                val = -o->f_new[(int64_t)o->opp[d] * N + c] +
                       (REAL)2 * w * o->rho[c] * ((REAL)1 + (REAL)0.5 * cu2 * (REAL)9 - (REAL)0.5 * uu2 * (REAL)3);
             } else {
                 /* every out axis is zero-gradient: clamp onto the face */
@@ -465,7 +476,7 @@ static void SFX(orc_refill)(SFX(orc)* o) {
         int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
         REAL sr = 0, su = 0, sv = 0, sw = 0;
         int cnt = 0;
-        for (int d = 0; d < 18; ++d) {
+        for (int d = 0; d < o->q - 1; ++d) {
             int64_t b = SFX(orc_nbr)(o, x, y, z, d);
             if (b < 0 || o->flags[b] != 0 || o->fresh[b]) continue;
             sr += o->rho[b];
@@ -485,7 +496,7 @@ static void SFX(orc_refill)(SFX(orc)* o) {
             r = (REAL)o->cfg.rho0;
             u = v = ww = 0;
         }
-        for (int d = 0; d < 19; ++d) o->f_old[(int64_t)d * N + c] = SFX(orc_feq)(o->w[d], r, u, v, ww, d);
+        for (int d = 0; d < o->q; ++d) o->f_old[(int64_t)d * N + c] = SFX(orc_feq)(o, o->w[d], r, u, v, ww, d);
     }
 }
 
@@ -504,15 +515,15 @@ static void SFX(orc_mea)(SFX(orc)* o, double* out) {
         int64_t s = SFX(orc_nbr)(o, x, y, z, d);
         int bi = o->flags[s] - 1;
         const orc_body* b = &o->ob[bi];
-        REAL tmp = o->f_new[(int64_t)d * N + c] + o->f_old[(int64_t)OOPP[d] * N + c];
+        REAL tmp = o->f_new[(int64_t)d * N + c] + o->f_old[(int64_t)o->opp[d] * N + c];
         double t = (double)tmp;
-        double P[3] = {t * OCX[d], t * OCY[d], t * OCZ[d]};
+        double P[3] = {t * o->cx[d], t * o->cy[d], t * o->cz[d]};
         /* arm to the wall point x + h c_d: h = 1/2 (SBB) or q (IBB) */
         float qf;
         int64_t c2;
         double h = SFX(orc_link_rule)(o, x, y, z, d, b, &qf, &c2) == 0 ? 0.5 : (double)qf;
-        double r[3] = {(double)x + 0.5 + h * OCX[d] - b->ch[0], (double)y + 0.5 + h * OCY[d] - b->ch[1],
-                       (double)z + 0.5 + h * OCZ[d] - b->ch[2]};
+        double r[3] = {(double)x + 0.5 + h * o->cx[d] - b->ch[0], (double)y + 0.5 + h * o->cy[d] - b->ch[1],
+                       (double)z + 0.5 + h * o->cz[d] - b->ch[2]};
         for (int a = 0; a < 3; ++a) {
             if (o->per[a]) {
                 double h = 0.5 * o->L[a];
@@ -593,7 +604,7 @@ static int SFX(orc_init)(void* p, const double* rho, const double* u) {
         REAL ux = u ? (REAL)u[c] : (REAL)o->cfg.u0[0];
         REAL uy = u ? (REAL)u[N + c] : (REAL)o->cfg.u0[1];
         REAL uz = u ? (REAL)u[2 * N + c] : (REAL)o->cfg.u0[2];
-        for (int d = 0; d < 19; ++d) o->f_old[(int64_t)d * N + c] = SFX(orc_feq)(o->w[d], r, ux, uy, uz, d);
+        for (int d = 0; d < o->q; ++d) o->f_old[(int64_t)d * N + c] = SFX(orc_feq)(o, o->w[d], r, ux, uy, uz, d);
     }
     SFX(orc_macros)(o);
     o->step = 0;
@@ -603,7 +614,7 @@ static int SFX(orc_init)(void* p, const double* rho, const double* u) {
 
 static int SFX(orc_set_populations)(void* p, const double* f) {
     SFX(orc)* o = (SFX(orc)*)p;
-    for (int64_t i = 0; i < 19 * o->N; ++i) o->f_old[i] = (REAL)f[i];
+    for (int64_t i = 0; i < (int64_t)o->q * o->N; ++i) o->f_old[i] = (REAL)f[i];
     SFX(orc_macros)(o);
     o->step = 0;
     o->initialised = 1;
@@ -654,7 +665,7 @@ static int SFX(orc_get_macros)(void* p, double* rho, double* u) {
 
 static int SFX(orc_get_populations)(void* p, double* f) {
     SFX(orc)* o = (SFX(orc)*)p;
-    for (int64_t i = 0; i < 19 * o->N; ++i) f[i] = (double)o->f_old[i];
+    for (int64_t i = 0; i < (int64_t)o->q * o->N; ++i) f[i] = (double)o->f_old[i];
     return LBM_OK;
 }
 

@@ -1,0 +1,208 @@
+"""D3Q27 (src/Utilities.jl:962-976): HIP path vs the oracle (GPU).
+
+Same bars as the D3Q19 parity tests: flags and the 26-direction link list
+bit-exact; populations, moments and per-body force/torque within 1e-12 (FP64)
+or 1e-5 (FP32), normwise (forces relative to the momentum-exchange scale
+sum over links of 2 w_d rho0).  The oracle's D3Q27 step is pinned separately
+(tests/test_d3q27_oracle.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from lbm3d_bb_amd.bodies import ConstantMotion, ellipsoid, rotation_matrix, sphere
+from lbm3d_bb_amd.lattice import C27, W27
+from lbm3d_bb_amd.solver import Solver
+from oracle.oracle import OracleSolver
+
+from cases import channel, force_err, periodic_box, reference_channel, rel_err, sphere_pack
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+def _q27(cfg):
+    cfg.q = 27
+    return cfg
+
+
+def _pair(cfg, prec):
+    return Solver(cfg), OracleSolver(cfg, prec)
+
+
+def _scale27(o, rho0=1.0):
+    _, dirs = o.links()
+    return float(2.0 * rho0 * W27[dirs.astype(np.int64)].sum())
+
+
+def _compare_state(g, o, tol, what=""):
+    rg, ug = g.macros()
+    ro, uo = o.macros()
+    assert rel_err(rg, ro) < tol, f"{what} rho {rel_err(rg, ro)}"
+    assert rel_err(ug, uo) < tol * 1e2 or np.abs(ug - uo).max() < tol, f"{what} u {rel_err(ug, uo)}"
+    fg, fo = g.populations(), o.populations()
+    assert fg.shape == (27, g.N)
+    assert rel_err(fg, fo) < tol, f"{what} f {rel_err(fg, fo)}"
+
+
+def test_d3q27_flags_and_links_bit_exact():
+    cfg = _q27(channel(nx=40, ny=26, nz=22, storage="fp32"))
+    bodies = [sphere((12.3, 13.1, 10.8), 5.2), ellipsoid((27.0, 12.0, 11.0), (6.0, 4.0, 3.0),
+                                                         rot=rotation_matrix((1, 2, 0), 0.5))]
+    g, o = _pair(cfg, "fp32")
+    for s in (g, o):
+        s.set_bodies(bodies)
+        s.update_geometry()
+    assert np.array_equal(g.flags(), o.flags())
+    cg, dg = g.links()
+    co, do = o.links()
+    assert len(co) > 0 and int(do.max()) > 17  # edge and corner links present
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+    # every link points at a solid node from a fluid node
+    fl = o.flags()
+    x, y, z = cg % 40, (cg // 40) % 26, cg // (40 * 26)
+    c = C27[dg.astype(np.int64)]
+    assert (fl[cg] == 0).all()
+    assert (fl[(x + c[:, 0]) + 40 * ((y + c[:, 1]) + 26 * (z + c[:, 2]))] != 0).all()
+
+
+def test_d3q27_start_state_and_raw_getters():
+    cfg = _q27(periodic_box(n=12, collision="TRT"))
+    g, o = _pair(cfg, "fp64")
+    rng = np.random.default_rng(3)
+    rho = 1.0 + 0.02 * rng.standard_normal(cfg.ncells)
+    u = 0.03 * rng.standard_normal((3, cfg.ncells))
+    for s in (g, o):
+        s.init_equilibrium(rho, u)
+    _compare_state(g, o, 1e-15, "init")
+    f = o.populations() + 1e-4 * rng.standard_normal((27, cfg.ncells))
+    g.set_populations(f)
+    assert np.array_equal(g.populations(), f)
+
+
+@pytest.mark.parametrize("collision", ["BGK", "TRT"])
+def test_d3q27_periodic_sphere_forced_fp64(collision):
+    cfg = _q27(periodic_box(n=24, collision=collision, g=(2e-5, -1e-5, 5e-6)))
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies([sphere((11.7, 12.2, 12.9), 5.3)])
+        s.init_equilibrium()
+    Fg, Fo = g.step(40), o.step(40)
+    assert np.abs(Fo).max() > 0
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], collision)
+
+
+def test_d3q27_multi_body_wrapping_fp64():
+    """Several spheres, some across the periodic boundaries, over more than one chunk of steps."""
+    cfg = _q27(periodic_box(n=28, collision="TRT"))
+    bodies = sphere_pack(6, 28, 2.5, 3.5, seed=4)
+    assert any(min(b.center[a], 28 - b.center[a]) < b.semi[0] for b in bodies for a in range(3))
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies(bodies)
+        s.init_equilibrium()
+    Fg, Fo = g.step(70), o.step(70)
+    assert Fg.shape == (70, len(bodies), 6)
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], "multi")
+
+
+def test_d3q27_channel_translating_sphere_refill_fp64():
+    """Velocity inlet, zero-gradient outlet, no-slip walls (edges and corners of
+    the 27 directions), a translating sphere with fresh-node refill."""
+    cfg = _q27(channel(nx=40, ny=20, nz=20, storage="fp64"))
+    g, o = _pair(cfg, "fp64")
+    law = ConstantMotion("sphere", (12.3, 10.1, 9.8), (4.0,), vel=(0.05, 0.0, 0.0))
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg, Fo = g.step(50), o.step(50)
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], "channel")
+    assert np.array_equal(g.flags(), o.flags())
+
+
+def test_d3q27_reference_face_set_moving_wall_fp64():
+    """The reference's faces (velocity inlet and y-walls at U_inf, anti-bounce-back
+    outlet, z periodic), its wall sampling point, a sphere moving in y and spinning."""
+    cfg = _q27(reference_channel(nx=40, ny=24, nz=8, storage="fp64"))
+    g, o = _pair(cfg, "fp64")
+    law = ConstantMotion("sphere", (14.0, 12.0, 4.0), (3.5,), vel=(0.0, 0.02, 0.0), omega=(0.0, 0.0, 0.01))
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg, Fo = g.step(40), o.step(40)
+    assert np.abs(Fo[:, :, 5]).max() > 0
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], "refchannel")
+
+
+def test_d3q27_rotating_ellipsoid_refill_fp64():
+    cfg = _q27(channel(nx=36, ny=28, nz=28, storage="fp64", u_in=0.02))
+    g, o = _pair(cfg, "fp64")
+    law = ConstantMotion("ellipsoid", (17.0, 14.0, 14.0), (8.0, 4.0, 4.0), omega=(0.0, 0.02, 0.03),
+                         rot0=rotation_matrix((1, 1, 0), 0.4))
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg, Fo = g.step(30), o.step(30)
+    assert np.abs(Fo[:, :, 3:]).max() > 0
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], "ellipsoid")
+    assert np.array_equal(g.flags(), o.flags())
+
+
+def test_d3q27_channel_translating_sphere_fp32():
+    cfg = _q27(channel(nx=40, ny=20, nz=20, storage="fp32"))
+    g, o = _pair(cfg, "fp32")
+    law = ConstantMotion("sphere", (12.3, 10.1, 9.8), (4.0,), vel=(0.01, 0.0, 0.0))
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg, Fo = g.step(30), o.step(30)
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp32"], force_err(Fg, Fo, _scale27(o))
+    rg, ug = g.macros()
+    ro, uo = o.macros()
+    assert rel_err(rg, ro) < TOL["fp32"]
+    assert rel_err(ug, uo) < TOL["fp32"] * 10
+
+
+def test_d3q27_explicit_flags_fp64():
+    """Geometry from lbm_set_flags (no analytic extent): every cell checks its flags."""
+    cfg = _q27(periodic_box(n=16, collision="BGK"))
+    o = OracleSolver(cfg, "fp64")
+    o.set_bodies([sphere((8.2, 7.9, 8.4), 4.1)])
+    o.update_geometry()
+    flags = o.flags()
+    g = Solver(cfg)
+    g.set_flags(flags)
+    o2 = OracleSolver(cfg, "fp64")
+    o2.set_flags(flags)
+    for s in (g, o2):
+        s.init_equilibrium()
+    Fg, Fo = g.step(20), o2.step(20)
+    assert force_err(Fg, Fo, _scale27(o2)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o2))
+    _compare_state(g, o2, TOL["fp64"], "flags")
+
+
+def test_d3q27_forces_deterministic_and_mass_conserved():
+    cfg = _q27(periodic_box(n=24, collision="TRT", g=(1e-4, 0.0, 0.0)))
+    bodies = sphere_pack(4, 24, 2.5, 3.5, seed=2)
+    runs = []
+    for _ in range(2):
+        g = Solver(cfg)
+        g.set_bodies(bodies)
+        g.init_equilibrium()
+        runs.append(g.step(20))
+        g.close()
+    assert np.array_equal(runs[0], runs[1])
+    g = Solver(cfg)
+    g.init_equilibrium()
+    m0, p0 = g.conservation()
+    g.step(50)
+    m1, p1 = g.conservation()
+    assert abs(m1 - m0) / m0 < 1e-13
+    assert abs((p1[0] - p0[0]) - 50 * m0 * 1e-4) < 1e-10 * m0
