@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Throughput of the D3Q19 per-step path on one MI355X.
+"""Throughput of the per-step path (D3Q19, or D3Q27) on one MI355X.
 
 Workload (the default): 256x128x128 channel, D3Q19 TRT (Lambda = 3/16), FP32
 populations; velocity inlet u = 0.05 at x = 0, zero-gradient outlet, no-slip
@@ -11,8 +11,9 @@ starts with the populations resident in HBM.
 
 Prints one JSON line.  MLUPS counts every cell, solid ones included
 (src/Utilities.jl:913-916).  ``roofline`` is the per-step kernel k_step (one
-launch per time step: bulk rows, faces and body regions): its algorithmic
-bytes per launch (2 x 19 x s bytes per cell, s the storage size, all cells)
+launch per time step: bulk rows, faces and body regions; k27_step for the
+D3Q27 workloads): its algorithmic bytes per launch (2 x q x s bytes per cell,
+q = 19 or 27 populations, s the storage size, all cells)
 over its mean launch time from HIP events recorded on the solver's stream
 around every 16th launch in the timed region; ``traffic`` is the HBM bytes per
 launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, when
@@ -51,11 +52,13 @@ _DT = {"fp64": "f64", "fp32": "f32", "fp16s": "f16 storage, f32 compute", "bf16s
 # inlet and y walls at U = 0.075, anti-bounce-back outlet, z periodic, a
 # cylinder R = 16.16 heaving with the reference's law) with its interpolated
 # bounce-back (IBB), the simple one (SBB), or the NACA 0012 airfoil of chord 120
-# (IBB).
+# (IBB); channel256_q27*: the translating-sphere channel on the D3Q27 lattice.
 WORKLOADS = {
     "channel256": dict(case="channel", nx=256, ny=128, nz=128, storage="fp32"),
     "channel256_fp16s": dict(case="channel", nx=256, ny=128, nz=128, storage="fp16s"),
     "channel256_bf16s": dict(case="channel", nx=256, ny=128, nz=128, storage="bf16s"),
+    "channel256_q27": dict(case="channel", nx=256, ny=128, nz=128, storage="fp32", q=27),
+    "channel256_q27_fp64": dict(case="channel", nx=256, ny=128, nz=128, storage="fp64", q=27),
     "box512_spheres64": dict(case="box", nx=512, ny=256, nz=256, storage="fp32"),
     "box512_spheres64_fp16s": dict(case="box", nx=512, ny=256, nz=256, storage="fp16s"),
     "box512_spheres64_fp64": dict(case="box", nx=512, ny=256, nz=256, storage="fp64"),
@@ -66,6 +69,7 @@ WORKLOADS = {
 }
 for _w in WORKLOADS.values():
     _w["dtype"] = _DT[_w["storage"]]
+    _w.setdefault("q", 19)
 
 
 def _trt(tau_p: float) -> tuple[float, float]:
@@ -93,9 +97,9 @@ def make_case(w):
     if w["case"] == "channel":
         wp, wm = _trt(0.56)
         cfg = Config(w["nx"], w["ny"], w["nz"], collision="TRT", storage=w["storage"], omega_plus=wp, omega_minus=wm,
-                     faces=walls, u0=(0.05, 0.0, 0.0), refill=True)
+                     faces=walls, u0=(0.05, 0.0, 0.0), refill=True, q=w["q"])
         law = ConstantMotion("sphere", (48.0, 64.0, 64.0), (12.0,), vel=(0.01, 0.0, 0.0))
-        desc = (f"{cfg.nx}x{cfg.ny}x{cfg.nz} D3Q19 TRT channel, {w['storage']} populations, translating sphere "
+        desc = (f"{cfg.nx}x{cfg.ny}x{cfg.nz} D3Q{w['q']} TRT channel, {w['storage']} populations, translating sphere "
                 "R=12 with moving bounce-back, refill, per-step MEA force")
         return cfg, [law], None, desc
     rng = np.random.default_rng(0)
@@ -142,8 +146,8 @@ def setup(solver, laws, bodies) -> None:
     solver.init_equilibrium()
 
 
-def bytes_per_cell(storage: str) -> int:
-    return 2 * 19 * {"fp64": 8, "fp32": 4, "fp16s": 2, "bf16s": 2}[storage]
+def bytes_per_cell(storage: str, q: int = 19) -> int:
+    return 2 * q * {"fp64": 8, "fp32": 4, "fp16s": 2, "bf16s": 2}[storage]
 
 
 def cpu_baseline(w, budget_s: float) -> dict:
@@ -215,7 +219,7 @@ def main() -> int:
 
     N = cfg.ncells
     mlups = N * args.steps / el / 1e6
-    bpc = bytes_per_cell(w["storage"])
+    bpc = bytes_per_cell(w["storage"], w["q"])
     # k_step is the whole time step (bulk rows, faces, body regions) in one launch
     step_avg_s = prof["bulk_ms"] / max(prof["bulk_launches"], 1) / 1e3
     achieved = bpc * N / step_avg_s / 1e9 if step_avg_s > 0 else None
@@ -243,6 +247,7 @@ def main() -> int:
             "tau_plus": round(1.0 / cfg.omega_plus, 6),
             "lambda": 0.1875 if cfg.collision == "TRT" else None,
             "solid_bc": cfg.solid_bc,
+            "lattice": f"D3Q{w['q']}",
             "step_hbm_frac_of_peak": round(bpc * N * args.steps / el / 1e9 / HBM_PEAK_GBS, 4),
             "step_kernel_ms": round(step_avg_s * 1e3, 5),
             "step_kernel_samples": prof["bulk_launches"],

@@ -48,7 +48,8 @@ struct lbm_ctx {
     uint8_t* flags = nullptr;
     int q = 19;                // lattice: D3Q19 or D3Q27
 
-    // D3Q27: flags before the last pose change, and the last step's bodies
+    // D3Q27: flags before the last pose change (the two buffers swap roles at
+    // every pose change), and the last step's bodies
     // (inside the device step records) for the getters' replay
     uint8_t* flags_old = nullptr;
     const DevBody* q27_bnew = nullptr;
@@ -778,7 +779,8 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
         const DevBody* dn = (const DevBody*)(c->d_rec + (size_t)s * c->rec_bytes);
         const bool mv = moved[(size_t)s] != 0;
         if (mv) {
-            HIPCHK(c, hipMemcpyAsync(c->flags_old, c->flags, (size_t)c->g.N, hipMemcpyDeviceToDevice, c->stream));
+            // the flags of the old pose stay in the other buffer for the refill
+            std::swap(c->flags, c->flags_old);
             GEOCHK(c, launch_flags_full(c->flags, dn, nb, c->g, c->stream));
         }
         q27::StepArgs a = q27_args(c, MODE_STEP, c->fbuf[c->cur], c->fbuf[1 - c->cur]);

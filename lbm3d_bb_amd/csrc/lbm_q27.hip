@@ -177,34 +177,82 @@ __device__ inline RuleFaces rule_faces_of(const Grid& g) {
     return r;
 }
 
-// A cell on a non-periodic face with a velocity or anti-bounce-back rule.
-__device__ inline bool on_rule_face(const int q0[3], const Grid& g, RuleFaces rf) {
-    const int L[3] = {g.nx, g.ny, g.nz};
-    const uint32_t any = rf.vel | rf.abb;
-    bool r = false;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        r = r || (q0[a] == 0 && ((any >> (2 * a)) & 1u));
-        r = r || (q0[a] == L[a] - 1 && ((any >> (2 * a + 1)) & 1u));
-    }
-    return r;
+// Whether the rows (y, z) of this block's cells come within reach of a body,
+// from block-uniform values: the y and z intervals of the rows, against each
+// body's extent (min image on periodic axes; an interval of half the axis or
+// more counts as near).
+__device__ inline bool interval_near(int lo, int hi, float c, float ext, bool per, int L) {
+    const float half = 0.5f * (float)(hi - lo + 1);
+    if (per && 2 * (hi - lo + 1) >= L) return true;
+    const float m = 0.5f * (float)(lo + hi + 1);
+    return fabsf(min_image(m - c, per, L)) <= ext + half;
 }
 
-// Source of the pull along d: x - c_d wrapped on periodic axes and clamped onto
-// the cell on the others (zero gradient; the face rules replace the rest).
-__device__ inline int64_t pull_source(const int q0[3], int d, const Grid& g) {
-    const int L[3] = {g.nx, g.ny, g.nz};
-    const int cd[3] = {kX[d], kY[d], kZ[d]};
-    int s[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        s[a] = q0[a] - cd[a];
-        if (s[a] < 0 || s[a] >= L[a]) {
-            if ((g.per >> a) & 1) s[a] = s[a] < 0 ? s[a] + L[a] : s[a] - L[a];
-            else s[a] = q0[a];
+__device__ inline bool block_near(const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold, int nb,
+                                  const Grid& g) {
+    if (nb <= 0) return false;
+    // 32-bit arithmetic: cell counts stay below 2^31
+    const uint32_t c0 = blockIdx.x * (uint32_t)kBlock;
+    const uint32_t c1 = (c0 + kBlock < (uint32_t)g.N ? c0 + kBlock : (uint32_t)g.N) - 1u;
+    const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
+    const uint32_t r0 = c0 / nx, r1 = c1 / nx;  // rows y + ny z
+    const int z0 = (int)(r0 / ny), z1 = (int)(r1 / ny);
+    const int y0 = z0 == z1 ? (int)(r0 - (uint32_t)z0 * ny) : 0;
+    const int y1 = z0 == z1 ? (int)(r1 - (uint32_t)z1 * ny) : g.ny - 1;
+    bool hit = false;
+    for (int b = 0; b < nb && !hit; ++b) {
+        for (int k = 0; k < 2 && !hit; ++k) {
+            if (k == 1 && bold == nullptr) break;
+            const DevBody& B = k == 0 ? bnew[b] : bold[b];
+            hit = B.reach < 0.0f || (interval_near(y0, y1, B.c[1], B.ext[1], g.per & 2, g.ny) &&
+                                     interval_near(z0, z1, B.c[2], B.ext[2], g.per & 4, g.nz));
         }
     }
-    return (int64_t)s[0] + (int64_t)g.nx * s[1] + g.nxy * s[2];
+    return hit;
+}
+
+// Pull sources x - c of a cell for the offsets c = -1, 0, 1 on each axis
+// (index c + 1), as cell-index terms: wrapped on periodic axes, clamped onto
+// the cell on the others (zero gradient; the face rules replace the rest).
+// Bit 3a + (c + 1) of `in`: that source exists (no clamp).  The source of
+// direction d is x[1 + cx] + y[1 + cy] + z[1 + cz], two integer adds.
+struct Sources {
+    int x[3], y[3], z[3];
+    uint32_t in;
+};
+
+__device__ inline Sources sources_of(int qx, int qy, int qz, const Grid& g) {
+    Sources s;
+    const int q[3] = {qx, qy, qz};
+    const int L[3] = {g.nx, g.ny, g.nz};
+    const int sc[3] = {1, g.nx, (int)g.nxy};
+    int t[3][3];
+    s.in = 0u;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool per = (g.per >> a) & 1;
+        const bool lo_in = q[a] > 0, hi_in = q[a] < L[a] - 1;
+        const int lo = lo_in ? q[a] - 1 : (per ? L[a] - 1 : q[a]);  // source for c = +1
+        const int hi = hi_in ? q[a] + 1 : (per ? 0 : q[a]);         // source for c = -1
+        t[a][0] = hi * sc[a];
+        t[a][1] = q[a] * sc[a];
+        t[a][2] = lo * sc[a];
+        s.in |= ((hi_in || per) ? 1u : 0u) << (3 * a) | 1u << (3 * a + 1) | ((lo_in || per) ? 1u : 0u) << (3 * a + 2);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s.x[k] = t[0][k];
+        s.y[k] = t[1][k];
+        s.z[k] = t[2][k];
+    }
+    return s;
+}
+
+__device__ inline int src_of(const Sources& s, int d) { return s.x[1 + kX[d]] + s.y[1 + kY[d]] + s.z[1 + kZ[d]]; }
+
+__device__ inline bool src_in(const Sources& s, int d) {
+    const uint32_t need = 1u << (1 + kX[d]) | 1u << (3 + 1 + kY[d]) | 1u << (6 + 1 + kZ[d]);
+    return (s.in & need) == need;
 }
 
 // Face whose rule gives direction d when x - c_d lies beyond non-periodic
@@ -223,6 +271,70 @@ __device__ inline int face_rule_of(const int q0[3], int d, const Grid& g, RuleFa
     *velocity = v != 0u;
     const uint32_t m = v != 0u ? v : b;
     return m != 0u ? __builtin_ctz(m) : -1;
+}
+
+// Face rules for direction d whose source lies beyond the chosen face:
+// VELOCITY src/OtherBC.jl:143-148, PRESSURE_ABB src/OtherBC.jl:156-165.
+template <typename C>
+__device__ inline C velocity_rule(int d, C own_opp, const C uf[3], C rho_prev) {
+    const C cu = C(kX[d]) * (-uf[0]) + C(kY[d]) * (-uf[1]) + C(kZ[d]) * (-uf[2]);
+    return own_opp - C(2) * w27<C>(d) * rho_prev * cu * C(3);
+}
+
+template <typename C>
+__device__ inline C abb_rule(int d, C own_opp, const C up[3], C uu2, C rho_prev) {
+    const C cu = C(kX[d]) * up[0] + C(kY[d]) * up[1] + C(kZ[d]) * up[2];
+    return -own_opp + C(2) * w27<C>(d) * rho_prev * (C(1) + C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3));
+}
+
+// Directions whose source x - c_d lies beyond face f = 2a + side: c_a = +1 on
+// the low face (side 0), c_a = -1 on the high one.
+__host__ __device__ constexpr uint32_t face_dirs(int f) {
+    uint32_t m = 0u;
+    for (int d = 0; d < QQ; ++d) {
+        const int c = (f >> 1) == 0 ? kX[d] : ((f >> 1) == 1 ? kY[d] : kZ[d]);
+        if (c == ((f & 1) ? -1 : 1)) m |= 1u << d;
+    }
+    return m;
+}
+
+// Rule directions of a cell's faces: the union of face_dirs over the rule
+// faces the cell lies on (a direction beyond several faces takes the first
+// rule face in the precedence order, which face_rule_of picks).  *single: the
+// one rule face of a cell on exactly one, else -1.
+__device__ inline uint32_t face_rule_mask(const int q0[3], const Grid& g, RuleFaces rf, int* single) {
+    const int L[3] = {g.nx, g.ny, g.nz};
+    const uint32_t any = rf.vel | rf.abb;
+    uint32_t m = 0u;
+    int n = 0, one = -1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if ((g.per >> a) & 1) continue;
+        if (q0[a] == 0 && ((any >> (2 * a)) & 1u)) {
+            m |= face_dirs(2 * a);
+            ++n;
+            one = 2 * a;
+        }
+        if (q0[a] == L[a] - 1 && ((any >> (2 * a + 1)) & 1u)) {
+            m |= face_dirs(2 * a + 1);
+            ++n;
+            one = 2 * a + 1;
+        }
+    }
+    *single = n == 1 ? one : -1;
+    return m;
+}
+
+// The rule of face FC on its directions (the cell lies on that face only).
+// The rule of face FC on its nine directions, for a cell on that face only
+// (f[d] holds the cell's own f*[opp(d)] the rule starts from).
+template <typename C, int FC>
+__device__ inline void one_face_rule(C f[QQ], bool velocity, const C uf[3], const C up[3], C uu2, C rho_prev) {
+#pragma unroll
+    for (int d = 0; d < QQ; ++d) {
+        if (!((face_dirs(FC) >> d) & 1u)) continue;
+        f[d] = velocity ? velocity_rule<C>(d, f[d], uf, rho_prev) : abb_rule<C>(d, f[d], up, uu2, rho_prev);
+    }
 }
 
 // In-domain cell x - c_d (periodic wrap), or -1 beyond a non-periodic face.
@@ -277,40 +389,58 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
             int x, y, z;
             xyz_of(cell, g, x, y, z);
             const int q0[3] = {x, y, z};
+            // rule directions of the faces (fmask); single: the one rule face of the cell
+            const RuleFaces rf = rule_faces_of(g);
+            int single = -1;
+            const uint32_t fmask = face_rule_mask(q0, g, rf, &single);
+            const Sources src = sources_of(x, y, z, g);
             bool near = false;
-            for (int b = 0; b < nb && !near; ++b)
-                near = near_body(bnew[b], x, y, z, g) || (bold != nullptr && near_body(bold[b], x, y, z, g));
+            if (block_near(bnew, bold, nb, g)) {
+                for (int b = 0; b < nb && !near; ++b)
+                    near = near_body(bnew[b], x, y, z, g) || (bold != nullptr && near_body(bold[b], x, y, z, g));
+            }
             const int self = near ? (fl_new[cell] & kFlagBody) : 0;
             const bool fresh = near && refill && self == 0 && (fl_old[cell] & kFlagBody) != 0;
-            // rule directions: face rules (fmask), links to a solid source (lmask)
-            uint32_t fmask = 0u, lmask = 0u;
-            const RuleFaces rf = rule_faces_of(g);
-            if (on_rule_face(q0, g, rf)) {
-#pragma unroll
-                for (int d = 0; d < QQ; ++d) {
-                    bool vel;
-                    if (face_rule_of(q0, d, g, rf, &vel) >= 0) fmask |= 1u << d;
-                }
-            }
+            // links to a solid source (lmask)
+            uint32_t lmask = 0u;
+            int lbody = -1;       // body of the links (the last one met)
+            bool lmixed = false;  // links to more than one body
             if (near && self == 0 && !fresh) {
+                // 26 independent flag loads at in-domain (clamped) addresses
+#pragma unroll
                 for (int d = 0; d < QQ - 1; ++d) {
-                    const int64_t sc = in_domain_source(q0, d, g);
-                    if (sc >= 0 && (fl_new[sc] & kFlagBody) != 0) lmask |= 1u << d;
+                    const uint8_t fv = fl_new[src_of(src, d)];
+                    const int b = (fv & kFlagBody) - 1;
+                    if (src_in(src, d) && b >= 0) {
+                        lmask |= 1u << d;
+                        lmixed = lmixed || (lbody >= 0 && lbody != b);
+                        lbody = b;
+                    }
                 }
             }
             if (fresh) {
                 // equilibrium of the mean previous-step rho, u of the neighbours
                 // x + c_d that are fluid and not fresh (refill, extension)
+                // (a rare path: one neighbour at a time keeps the registers of
+                // the step's other paths low)
                 C sr = C(0), su = C(0), sv = C(0), sw = C(0);
                 int cnt = 0;
                 for (int d = 0; d < QQ - 1; ++d) {
                     const int64_t nc = in_domain_source(q0, opp(d), g);
                     if (nc < 0 || (fl_new[nc] & kFlagBody) != 0 || (fl_old[nc] & kFlagBody) != 0) continue;
-                    C nbr[QQ];
+                    C r = C(0), jx = C(0), jy = C(0), jz = C(0);
 #pragma unroll
-                    for (int e = 0; e < QQ; ++e) nbr[e] = S::load(fin + (int64_t)e * g.P + nc, e, g.sh);
-                    C r, jx, jy, jz, rr, uu[3], u2;
-                    moments27<C>(nbr, r, jx, jy, jz);
+                    for (int e = 0; e < QQ; ++e) {
+                        const C v = S::load(fin + (int64_t)e * g.P + nc, e, g.sh);
+                        r += v;
+                        if (kX[e] > 0) jx += v;
+                        if (kX[e] < 0) jx -= v;
+                        if (kY[e] > 0) jy += v;
+                        if (kY[e] < 0) jy -= v;
+                        if (kZ[e] > 0) jz += v;
+                        if (kZ[e] < 0) jz -= v;
+                    }
+                    C rr, uu[3], u2;
                     prev_moments<C, FORCED>(r, jx, jy, jz, p, rr, uu, u2);
                     sr += rr;
                     su += uu[0];
@@ -330,27 +460,68 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                 for (int d = 0; d < QQ; ++d) f[d] = feq27<C>(d, r, ux, uy, uz);
             } else {
                 const uint32_t rmask = fmask | lmask;
+                // A cell on one rule face and with no link (an x-face cell shares
+                // its wave with the row's bulk cells, a wall row is a wave of them)
+                // takes the short path: that face's rule on its nine directions.
+                const bool one_face = single >= 0 && lmask == 0u;
                 C rho_prev = C(1), up[3] = {C(0), C(0), C(0)}, uu2 = C(0);
                 if (rmask) {
-                    C own[QQ];
+                    // previous-step moments from the cell's own f*
+                    C r = C(0), jx = C(0), jy = C(0), jz = C(0);
 #pragma unroll
-                    for (int d = 0; d < QQ; ++d) own[d] = S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
-                    C r, jx, jy, jz;
-                    moments27<C>(own, r, jx, jy, jz);
+                    for (int d = 0; d < QQ; ++d) {
+                        const C v = S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
+                        r += v;
+                        if (kX[d] > 0) jx += v;
+                        if (kX[d] < 0) jx -= v;
+                        if (kY[d] > 0) jy += v;
+                        if (kY[d] < 0) jy -= v;
+                        if (kZ[d] > 0) jz += v;
+                        if (kZ[d] < 0) jz -= v;
+                    }
                     prev_moments<C, FORCED>(r, jx, jy, jz, p, rho_prev, up, uu2);
                 }
-                // pulls; a rule direction loads the cell's own f*[opp(d)] instead
+                // pulls; a rule direction loads the cell's own f*[opp(d)] instead,
+                // the value its rule starts from
+                if (rmask == 0u) {
 #pragma unroll
-                for (int d = 0; d < QQ; ++d) {
-                    const int64_t sc = pull_source(q0, d, g);
-                    const bool rule = (rmask >> d) & 1u;
-                    const int64_t off = rule ? (int64_t)opp(d) * g.P + cell : (int64_t)d * g.P + sc;
-                    f[d] = S::load(fin + off, d, g.sh);
+                    for (int d = 0; d < QQ; ++d) f[d] = S::load(fin + (int64_t)d * g.P + src_of(src, d), d, g.sh);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < QQ; ++d) {
+                        const bool rule = (rmask >> d) & 1u;
+                        const int64_t off = rule ? (int64_t)opp(d) * g.P + cell : (int64_t)d * g.P + src_of(src, d);
+                        f[d] = S::load(fin + off, d, g.sh);
+                    }
                 }
-                if (rmask) {
+                if (one_face) {
+                    const int fc = single;
+                    const C uf[3] = {C(face_u[3 * fc]), C(face_u[3 * fc + 1]), C(face_u[3 * fc + 2])};
+                    const bool vel = (rf.vel >> fc) & 1u;
+                    switch (fc) {
+                        case 0: one_face_rule<C, 0>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 1: one_face_rule<C, 1>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 2: one_face_rule<C, 2>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 3: one_face_rule<C, 3>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 4: one_face_rule<C, 4>(f, vel, uf, up, uu2, rho_prev); break;
+                        default: one_face_rule<C, 5>(f, vel, uf, up, uu2, rho_prev); break;
+                    }
+                } else if (rmask) {
                     // wall point of the moving-wall velocity: physical x + c_dl/2 or
                     // the reference's x - c_dl/2 (src/SolidBC.jl:195-196)
                     const double hs = g.wall_point == LBM_WALL_REFERENCE ? -0.5 : 0.5;
+                    // kinematics of the link body, read once when every link of the
+                    // cell meets one body (no dependent loads inside the loop)
+                    double bch[3] = {0.0, 0.0, 0.0}, bv[3] = {0.0, 0.0, 0.0}, bom[3] = {0.0, 0.0, 0.0};
+                    if (lmask != 0u && !lmixed) {
+                        const DevBody& B = bnew[lbody];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            bch[a] = B.ch[a];
+                            bv[a] = B.v[a];
+                            bom[a] = B.om[a];
+                        }
+                    }
 #pragma unroll
                     for (int d = 0; d < QQ; ++d) {
                         if (!((rmask >> d) & 1u)) continue;
@@ -360,30 +531,32 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                             bool vel;
                             const int ch = face_rule_of(q0, d, g, rf, &vel);
                             const C uf[3] = {C(face_u[3 * ch]), C(face_u[3 * ch + 1]), C(face_u[3 * ch + 2])};
-                            if (vel) {
-                                // src/OtherBC.jl:143-148
-                                const C cu = C(kX[d]) * (-uf[0]) + C(kY[d]) * (-uf[1]) + C(kZ[d]) * (-uf[2]);
-                                f[d] = own_opp - C(2) * w * rho_prev * cu * C(3);
-                            } else {
-                                // src/OtherBC.jl:156-165
-                                const C cu = C(kX[d]) * up[0] + C(kY[d]) * up[1] + C(kZ[d]) * up[2];
-                                f[d] = -own_opp +
-                                       C(2) * w * rho_prev * (C(1) + C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3));
-                            }
+                            if (vel) f[d] = velocity_rule<C>(d, own_opp, uf, rho_prev);
+                            else f[d] = abb_rule<C>(d, own_opp, up, uu2, rho_prev);
                         } else {
                             // boundary link (x, dl), dl = opp(d), solid node x + c_dl
                             // (SBB!, src/SolidBC.jl:187-208)
-                            const int64_t sc = in_domain_source(q0, d, g);
-                            const int bi = (fl_new[sc] & kFlagBody) - 1;
-                            const DevBody& B = bnew[bi];
+                            int bi = lbody;
+                            double ch[3] = {bch[0], bch[1], bch[2]}, v[3] = {bv[0], bv[1], bv[2]};
+                            double om[3] = {bom[0], bom[1], bom[2]};
+                            if (lmixed) {
+                                bi = (fl_new[src_of(src, d)] & kFlagBody) - 1;
+                                const DevBody& B = bnew[bi];
+#pragma unroll
+                                for (int a = 0; a < 3; ++a) {
+                                    ch[a] = B.ch[a];
+                                    v[a] = B.v[a];
+                                    om[a] = B.om[a];
+                                }
+                            }
                             const int cdl[3] = {-kX[d], -kY[d], -kZ[d]};
                             double r[3];
 #pragma unroll
                             for (int a = 0; a < 3; ++a)
-                                r[a] = wrap_arm((double)q0[a] + 0.5 + hs * (double)cdl[a] - B.ch[a], a, g);
-                            const C uwx = C(B.v[0] + (B.om[1] * r[2] - B.om[2] * r[1]));
-                            const C uwy = C(B.v[1] + (B.om[2] * r[0] - B.om[0] * r[2]));
-                            const C uwz = C(B.v[2] + (B.om[0] * r[1] - B.om[1] * r[0]));
+                                r[a] = wrap_arm((double)q0[a] + 0.5 + hs * (double)cdl[a] - ch[a], a, g);
+                            const C uwx = C(v[0] + (om[1] * r[2] - om[2] * r[1]));
+                            const C uwy = C(v[1] + (om[2] * r[0] - om[0] * r[2]));
+                            const C uwz = C(v[2] + (om[0] * r[1] - om[1] * r[0]));
                             const C cuw = C(cdl[0]) * uwx + C(cdl[1]) * uwy + C(cdl[2]) * uwz;
                             const C val = own_opp - C(2) * w * rho_prev * cuw * C(3);
                             f[d] = val;
@@ -394,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                                 double ra[3];
 #pragma unroll
                                 for (int a = 0; a < 3; ++a)
-                                    ra[a] = wrap_arm((double)q0[a] + 0.5 + 0.5 * (double)cdl[a] - B.ch[a], a, g);
+                                    ra[a] = wrap_arm((double)q0[a] + 0.5 + 0.5 * (double)cdl[a] - ch[a], a, g);
                                 const double v6[6] = {P[0], P[1], P[2], ra[1] * P[2] - ra[2] * P[1],
                                                       ra[2] * P[0] - ra[0] * P[2], ra[0] * P[1] - ra[1] * P[0]};
                                 if (acc.b0 < 0 || acc.b0 == bi) {
