@@ -738,6 +738,11 @@ int q27_step_launch(lbm_ctx* c, const q27::StepArgs& a) {
 // changed) go into the step record; a pose change rebuilds the flags (the old
 // ones kept for the refill) before the step's single launch.
 int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
+    // block partials of every step of the chunk: at most 1 GiB of them
+    {
+        const int64_t per_step = nblocks_of(c->g.N) * std::max(c->nb, 1) * 6 * (int64_t)sizeof(double);
+        nsteps = (int)std::max<int64_t>(1, std::min<int64_t>(nsteps, ((int64_t)1 << 30) / per_step));
+    }
     const int par = c->parity;
     if (c->rec_ev_used[par]) HIPCHK(c, hipEventSynchronize(c->rec_ev[par]));
     const int nb = c->nb;
@@ -807,7 +812,8 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
         c->raw_state = false;
     }
     if (n6 > 0 && nprep > 0)
-        GEOCHK(c, q27_launch_force_sum(c->d_partials, nblk, stride, n6, nprep, c->d_forces, c->stream));
+        GEOCHK(c, q27_launch_force_sum(c->d_partials, nblk, stride, n6, nprep, c->d_rec, c->rec_bytes, c->g,
+                                       c->d_forces, c->stream));
     if (force_out && n6 > 0) {
         HIPCHK(c, hipMemcpyAsync(c->h_forces[par], c->d_forces, (size_t)nprep * n6 * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));

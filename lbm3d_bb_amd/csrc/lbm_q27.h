@@ -30,7 +30,7 @@ struct StepArgs {
     const DevBody* bnew;    // nb bodies of the step's pose
     const DevBody* bold;    // nb bodies of the previous pose, or null when the pose did not change
     int nb;
-    double* partials;  // MODE_STEP: [blocks][nb][6]
+    double* partials;  // MODE_STEP: [blocks][nb][6], written by the blocks within reach of a body
     int refill;
     const double* face_u;  // [6][3]
 };
@@ -38,9 +38,11 @@ struct StepArgs {
 }  // namespace q27
 
 int q27_launch_step(const q27::StepArgs& a);
-// out[s][k] = sum over blocks b of partials[s * stride + b * n6 + k], s < nsteps
-int q27_launch_force_sum(const double* partials, int64_t nblocks, int64_t stride, int n6, int nsteps, double* out,
-                         hipStream_t st);
+// out[s][k] = sum over the blocks b within reach of a body of
+// partials[s * stride + b * n6 + k], s < nsteps; recs + s * rec_bytes holds
+// step s's bodies (nb of the new pose, then nb of the old one)
+int q27_launch_force_sum(const double* partials, int64_t nblocks, int64_t stride, int n6, int nsteps,
+                         const char* recs, size_t rec_bytes, const Grid& g, double* out, hipStream_t st);
 int q27_launch_equilibrium(int storage, const Grid& g, hipStream_t st, void* f, const double* rho, const double* u,
                            double rho0, const double u0[3]);
 int q27_launch_from_double(int storage, const Grid& g, hipStream_t st, void* f, const double* src);

@@ -188,11 +188,11 @@ __device__ inline bool interval_near(int lo, int hi, float c, float ext, bool pe
     return fabsf(min_image(m - c, per, L)) <= ext + half;
 }
 
-__device__ inline bool block_near(const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold, int nb,
-                                  const Grid& g) {
+__device__ inline bool block_near(uint32_t block, const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
+                                  int nb, const Grid& g) {
     if (nb <= 0) return false;
     // 32-bit arithmetic: cell counts stay below 2^31
-    const uint32_t c0 = blockIdx.x * (uint32_t)kBlock;
+    const uint32_t c0 = block * (uint32_t)kBlock;
     const uint32_t c1 = (c0 + kBlock < (uint32_t)g.N ? c0 + kBlock : (uint32_t)g.N) - 1u;
     const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
     const uint32_t r0 = c0 / nx, r1 = c1 / nx;  // rows y + ny z
@@ -301,31 +301,31 @@ __host__ __device__ constexpr uint32_t face_dirs(int f) {
 // Rule directions of a cell's faces: the union of face_dirs over the rule
 // faces the cell lies on (a direction beyond several faces takes the first
 // rule face in the precedence order, which face_rule_of picks).  *single: the
-// one rule face of a cell on exactly one, else -1.
-__device__ inline uint32_t face_rule_mask(const int q0[3], const Grid& g, RuleFaces rf, int* single) {
+// one rule face of a cell on exactly one, else -1; *abb: the directions whose
+// rule is the anti-bounce-back one (the only rule that reads the velocity).
+__device__ inline uint32_t face_rule_mask(const int q0[3], const Grid& g, RuleFaces rf, int* single, uint32_t* abb) {
     const int L[3] = {g.nx, g.ny, g.nz};
-    const uint32_t any = rf.vel | rf.abb;
-    uint32_t m = 0u;
+    uint32_t vm = 0u, am = 0u;
     int n = 0, one = -1;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if ((g.per >> a) & 1) continue;
-        if (q0[a] == 0 && ((any >> (2 * a)) & 1u)) {
-            m |= face_dirs(2 * a);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int fc = 2 * a + side;
+            if (q0[a] != (side ? L[a] - 1 : 0)) continue;
+            if ((rf.vel >> fc) & 1u) vm |= face_dirs(fc);
+            else if ((rf.abb >> fc) & 1u) am |= face_dirs(fc);
+            else continue;
             ++n;
-            one = 2 * a;
-        }
-        if (q0[a] == L[a] - 1 && ((any >> (2 * a + 1)) & 1u)) {
-            m |= face_dirs(2 * a + 1);
-            ++n;
-            one = 2 * a + 1;
+            one = fc;
         }
     }
     *single = n == 1 ? one : -1;
-    return m;
+    *abb = am & ~vm;
+    return vm | am;
 }
 
-// The rule of face FC on its directions (the cell lies on that face only).
 // The rule of face FC on its nine directions, for a cell on that face only
 // (f[d] holds the cell's own f*[opp(d)] the rule starts from).
 template <typename C, int FC>
@@ -370,7 +370,10 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
     extern __shared__ double lds_dyn[];
     const int n6 = nb * 6;
     double* s_over = lds_dyn + kWaves * n6;  // links of a second body in one thread
-    if (MODE == MODE_STEP && nb > 0) {
+    // Only blocks within reach of a body can hold links: they alone reduce
+    // forces and write their partials (the sum visits the same blocks).
+    const bool bnear = MODE != MODE_COLLIDE_ONLY && MODE != MODE_GET_RAW && block_near(blockIdx.x, bnew, bold, nb, g);
+    if (MODE == MODE_STEP && bnear) {
         for (int i = threadIdx.x; i < (kWaves + 1) * n6; i += kBlock) lds_dyn[i] = 0.0;
         __syncthreads();
     }
@@ -392,10 +395,11 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
             // rule directions of the faces (fmask); single: the one rule face of the cell
             const RuleFaces rf = rule_faces_of(g);
             int single = -1;
-            const uint32_t fmask = face_rule_mask(q0, g, rf, &single);
+            uint32_t abb_dirs = 0u;
+            const uint32_t fmask = face_rule_mask(q0, g, rf, &single, &abb_dirs);
             const Sources src = sources_of(x, y, z, g);
             bool near = false;
-            if (block_near(bnew, bold, nb, g)) {
+            if (bnear) {
                 for (int b = 0; b < nb && !near; ++b)
                     near = near_body(bnew[b], x, y, z, g) || (bold != nullptr && near_body(bold[b], x, y, z, g));
             }
@@ -466,20 +470,27 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                 const bool one_face = single >= 0 && lmask == 0u;
                 C rho_prev = C(1), up[3] = {C(0), C(0), C(0)}, uu2 = C(0);
                 if (rmask) {
-                    // previous-step moments from the cell's own f*
+                    // previous-step moments from the cell's own f*: rho for the
+                    // velocity and link rules, u too for an anti-bounce-back one
                     C r = C(0), jx = C(0), jy = C(0), jz = C(0);
+                    if (abb_dirs != 0u) {
 #pragma unroll
-                    for (int d = 0; d < QQ; ++d) {
-                        const C v = S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
-                        r += v;
-                        if (kX[d] > 0) jx += v;
-                        if (kX[d] < 0) jx -= v;
-                        if (kY[d] > 0) jy += v;
-                        if (kY[d] < 0) jy -= v;
-                        if (kZ[d] > 0) jz += v;
-                        if (kZ[d] < 0) jz -= v;
+                        for (int d = 0; d < QQ; ++d) {
+                            const C v = S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
+                            r += v;
+                            if (kX[d] > 0) jx += v;
+                            if (kX[d] < 0) jx -= v;
+                            if (kY[d] > 0) jy += v;
+                            if (kY[d] < 0) jy -= v;
+                            if (kZ[d] > 0) jz += v;
+                            if (kZ[d] < 0) jz -= v;
+                        }
+                        prev_moments<C, FORCED>(r, jx, jy, jz, p, rho_prev, up, uu2);
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < QQ; ++d) r += S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
+                        rho_prev = r;
                     }
-                    prev_moments<C, FORCED>(r, jx, jy, jz, p, rho_prev, up, uu2);
                 }
                 // pulls; a rule direction loads the cell's own f*[opp(d)] instead,
                 // the value its rule starts from
@@ -610,7 +621,7 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
             for (int d = 0; d < QQ; ++d) S::store(fout + (int64_t)d * g.P + cell, d, f[d], g.sh);
         }
     }
-    if (MODE == MODE_STEP && nb > 0) {
+    if (MODE == MODE_STEP && bnear) {
         const int wave = threadIdx.x >> 6;
         wave_reduce_to_lds(acc.b0, acc.a0, lds_dyn + wave * n6);
         __syncthreads();
@@ -622,15 +633,21 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
     }
 }
 
-// Fixed-order sum of the block partials of step blockIdx.y: one block per
-// (body, component); thread t takes blocks t, t + kBlock, ..., then an LDS tree.
+// Fixed-order sum of the block partials of step blockIdx.y over the blocks
+// within reach of a body (the step record holds the step's bodies, new pose
+// then old pose): one block per (body, component); thread t takes blocks t,
+// t + kBlock, ..., then an LDS tree.
 __global__ __launch_bounds__(kBlock) void k27_force_sum(const double* __restrict__ partials, int64_t nblocks,
-                                                         int64_t stride, int n6, double* __restrict__ out) {
+                                                         int64_t stride, int n6, const char* __restrict__ recs,
+                                                         size_t rec_bytes, Grid g, double* __restrict__ out) {
     __shared__ double s[kBlock];
     const int k = blockIdx.x;
+    const int nb = n6 / 6;
     const double* ps = partials + (int64_t)blockIdx.y * stride;
+    const DevBody* bn = reinterpret_cast<const DevBody*>(recs + (size_t)blockIdx.y * rec_bytes);
     double a = 0.0;
-    for (int64_t b = threadIdx.x; b < nblocks; b += kBlock) a += ps[b * n6 + k];
+    for (int64_t b = threadIdx.x; b < nblocks; b += kBlock)
+        if (block_near((uint32_t)b, bn, bn + nb, nb, g)) a += ps[b * n6 + k];
     s[threadIdx.x] = a;
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {
@@ -767,10 +784,10 @@ int q27_launch_step(const q27::StepArgs& a) {
     return q27::last_error();
 }
 
-int q27_launch_force_sum(const double* partials, int64_t nblocks, int64_t stride, int n6, int nsteps, double* out,
-                         hipStream_t st) {
+int q27_launch_force_sum(const double* partials, int64_t nblocks, int64_t stride, int n6, int nsteps,
+                         const char* recs, size_t rec_bytes, const Grid& g, double* out, hipStream_t st) {
     hipLaunchKernelGGL(q27::k27_force_sum, dim3((unsigned)n6, (unsigned)nsteps), dim3(kBlock), 0, st, partials,
-                       nblocks, stride, n6, out);
+                       nblocks, stride, n6, recs, rec_bytes, g, out);
     return q27::last_error();
 }
 
