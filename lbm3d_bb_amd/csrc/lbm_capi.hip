@@ -52,6 +52,8 @@ struct lbm_ctx {
     // every pose change), and the last step's bodies
     // (inside the device step records) for the getters' replay
     uint8_t* flags_old = nullptr;
+    std::vector<DevBody> q27_pose_cur, q27_pose_old;  // the pose whose flags each buffer holds
+    bool q27_cur_ok = false, q27_old_ok = false;
     const DevBody* q27_bnew = nullptr;
     const DevBody* q27_bold = nullptr;
     int q27_refill = 0;
@@ -580,7 +582,11 @@ int full_geometry(lbm_ctx* c, const std::vector<DevBody>& bodies) {
     dev_free(c, d_b, std::max<size_t>(bodies.size(), 1) * sizeof(DevBody));
     c->mask_bodies = bodies;
     c->geom_valid = true;
-    if (c->q == 27) return LBM_OK;  // the D3Q27 step finds its links from the flags
+    if (c->q == 27) {  // the D3Q27 step finds its links from the flags
+        c->q27_pose_cur = bodies;
+        c->q27_cur_ok = true;
+        return LBM_OK;
+    }
     rc = build_list(c, nullptr, nullptr, nullptr);
     if (rc) return rc;
     c->list_valid = true;
@@ -737,6 +743,10 @@ int q27_step_launch(lbm_ctx* c, const q27::StepArgs& a) {
 // D3Q27 steps: per step the bodies of the new pose (and of the old one when it
 // changed) go into the step record; a pose change rebuilds the flags (the old
 // ones kept for the refill) before the step's single launch.
+// the flag boxes of a step record sit behind its 2 nb bodies, in the room the
+// D3Q19 step geometry takes
+static_assert(sizeof(StepGeom) >= LBM_MAX_BODIES * 6 * sizeof(int32_t), "step record too small for the flag boxes");
+
 int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
     // block partials of every step of the chunk: at most 1 GiB of them
     {
@@ -746,8 +756,10 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
     const int par = c->parity;
     if (c->rec_ev_used[par]) HIPCHK(c, hipEventSynchronize(c->rec_ev[par]));
     const int nb = c->nb;
-    std::vector<char> moved((size_t)nsteps, 0);
+    std::vector<char> moved((size_t)nsteps, 0), flag_boxes((size_t)nsteps, 0);
     std::vector<DevBody> bnew;
+    const int L[3] = {c->g.nx, c->g.ny, c->g.nz};
+    int max_box = 0;
     int nprep = 0;
     for (int s = 0; s < nsteps; ++s) {
         bool moving = false;
@@ -769,7 +781,44 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
             rb[i] = bnew[i];
             rb[nb + i] = moving ? c->mask_bodies[i] : bnew[i];
         }
-        if (moving) c->mask_bodies = bnew;
+        if (moving) {
+            c->mask_bodies = bnew;
+            // The buffers swap: the one that becomes current holds the flags of
+            // its own (older) pose, which differ from the new pose's only inside
+            // each body's box around both poses; the step record carries the boxes.
+            std::swap(c->q27_pose_cur, c->q27_pose_old);
+            std::swap(c->q27_cur_ok, c->q27_old_ok);
+            if (c->q27_cur_ok && (int)c->q27_pose_cur.size() == nb) {
+                int32_t* bx = (int32_t*)(c->h_rec[par] + (size_t)s * c->rec_bytes + 2 * (size_t)nb * sizeof(DevBody));
+                for (int i = 0; i < nb; ++i) {
+                    int64_t vol = 1;
+                    for (int a = 0; a < 3; ++a) {
+                        const float r = c->reach[i][a];
+                        const float lo_f = std::min(c->q27_pose_cur[i].c[a], bnew[i].c[a]) - r - 2.0f;
+                        const float hi_f = std::max(c->q27_pose_cur[i].c[a], bnew[i].c[a]) + r + 2.0f;
+                        int lo = 0, ext = L[a];
+                        if (std::isfinite(lo_f) && std::isfinite(hi_f) && hi_f - lo_f + 1.0f < (float)L[a]) {
+                            lo = (int)std::floor(lo_f);
+                            const int hi = (int)std::ceil(hi_f);
+                            if ((c->g.per >> a) & 1) {
+                                ext = std::min(hi - lo + 1, L[a]);
+                                lo = ((lo % L[a]) + L[a]) % L[a];
+                            } else {
+                                lo = std::max(lo, 0);
+                                ext = std::max(std::min(hi, L[a] - 1) - lo + 1, 0);
+                            }
+                        }
+                        bx[6 * i + a] = lo;
+                        bx[6 * i + 3 + a] = ext;
+                        vol *= ext;
+                    }
+                    max_box = (int)std::max<int64_t>(max_box, vol);
+                }
+                flag_boxes[(size_t)s] = 1;
+            }
+            c->q27_pose_cur = bnew;
+            c->q27_cur_ok = true;
+        }
         c->step_bodies = bnew;
         moved[(size_t)s] = moving ? 1 : 0;
         ++nprep;
@@ -786,7 +835,12 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
         if (mv) {
             // the flags of the old pose stay in the other buffer for the refill
             std::swap(c->flags, c->flags_old);
-            GEOCHK(c, launch_flags_full(c->flags, dn, nb, c->g, c->stream));
+            if (flag_boxes[(size_t)s])
+                GEOCHK(c, q27_launch_flags_boxes(c->flags, dn, nb, c->g,
+                                                 (const int32_t*)((const char*)dn + 2 * (size_t)nb * sizeof(DevBody)),
+                                                 max_box, c->stream));
+            else
+                GEOCHK(c, launch_flags_full(c->flags, dn, nb, c->g, c->stream));
         }
         q27::StepArgs a = q27_args(c, MODE_STEP, c->fbuf[c->cur], c->fbuf[1 - c->cur]);
         a.fl_old = mv ? c->flags_old : c->flags;
@@ -1107,6 +1161,7 @@ static int set_body_count(lbm_ctx* c, int n) {
         c->nb = n;
         c->geom_valid = false;
         c->list_valid = false;
+        c->q27_cur_ok = c->q27_old_ok = false;
         int rc = ensure_records(c);
         if (rc) return rc;
     }
@@ -1214,6 +1269,7 @@ int lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->geom_valid = true;
     c->last_rec_valid = false;
+    c->q27_cur_ok = c->q27_old_ok = false;  // explicit flags, no analytic pose
     if (c->q == 27) return LBM_OK;
     rc = build_list(c, nullptr, nullptr, nullptr);
     if (rc) return rc;

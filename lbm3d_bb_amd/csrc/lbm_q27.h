@@ -46,6 +46,10 @@ int q27_launch_force_sum(const double* partials, int64_t nblocks, int64_t stride
 int q27_launch_equilibrium(int storage, const Grid& g, hipStream_t st, void* f, const double* rho, const double* u,
                            double rho0, const double u0[3]);
 int q27_launch_from_double(int storage, const Grid& g, hipStream_t st, void* f, const double* src);
+// flags[cell] = solid flag of the bodies for every cell of the boxes
+// boxes[b] = {lo x, lo y, lo z, ext x, ext y, ext z} (lo wraps on periodic axes), b < nb
+int q27_launch_flags_boxes(uint8_t* flags, const DevBody* bodies, int nb, const Grid& g, const int32_t* boxes,
+                           int max_volume, hipStream_t st);
 int q27_launch_link_count(const uint8_t* flags, const Grid& g, int32_t* counts, hipStream_t st);
 int q27_launch_link_fill(const uint8_t* flags, const Grid& g, const int32_t* block_off, int32_t* cells, int8_t* dirs,
                          hipStream_t st);

@@ -12,6 +12,8 @@
 // (the address is selected, the load is unconditional).  FP64 and FP32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lbm_device.h"
 #include "lbm_q27.h"
 
@@ -680,6 +682,23 @@ __global__ __launch_bounds__(kBlock) void k27_from_double(typename S::T* __restr
     for (int d = 0; d < QQ; ++d) S::store(f + (int64_t)d * P + cell, d, (typename S::C)src[(int64_t)d * N + cell], sh);
 }
 
+// Solid flags (flag_of, the mask of k_flags_full) of the cells of one box per
+// body (blockIdx.y): where a moved body's flags can change.
+__global__ __launch_bounds__(kBlock) void k27_flags_boxes(uint8_t* __restrict__ flags, const DevBody* __restrict__ bodies,
+                                                          int nb, Grid g, const int32_t* __restrict__ boxes) {
+    const int32_t* bx = boxes + 6 * blockIdx.y;
+    const uint32_t ex = (uint32_t)bx[3], ey = (uint32_t)bx[4], ez = (uint32_t)bx[5];
+    const uint32_t vol = ex * ey * ez;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < vol; i += gridDim.x * kBlock) {
+        const uint32_t t = i / ex, k = t / ey;
+        int x = bx[0] + (int)(i - t * ex), y = bx[1] + (int)(t - k * ey), z = bx[2] + (int)k;
+        x = x >= g.nx ? x - g.nx : x;  // lo lies in [0, L) and ext <= L
+        y = y >= g.ny ? y - g.ny : y;
+        z = z >= g.nz ? z - g.nz : z;
+        flags[(int64_t)x + (int64_t)g.nx * y + g.nxy * z] = (uint8_t)flag_of(bodies, nb, x, y, z, g);
+    }
+}
+
 // Links (fluid cell x, direction d with a solid node x + c_d), ascending cell
 // then direction (kernel_fill_bb_csr!, src/Geometry.jl:249-257).
 __device__ inline int links_of(const uint8_t* __restrict__ flags, int64_t cell, const Grid& g, uint32_t* mask) {
@@ -808,6 +827,15 @@ int q27_launch_from_double(int storage, const Grid& g, hipStream_t st, void* f, 
     if (storage == LBM_FP64)
         hipLaunchKernelGGL(q27::k27_from_double<StoreF64>, grid, block, 0, st, (double*)f, src, g.N, g.P, g.sh);
     else hipLaunchKernelGGL(q27::k27_from_double<StoreF32>, grid, block, 0, st, (float*)f, src, g.N, g.P, g.sh);
+    return q27::last_error();
+}
+
+int q27_launch_flags_boxes(uint8_t* flags, const DevBody* bodies, int nb, const Grid& g, const int32_t* boxes,
+                           int max_volume, hipStream_t st) {
+    if (nb <= 0 || max_volume <= 0) return 0;
+    const int64_t blocks = std::min<int64_t>(nblocks_of(max_volume), 2048);
+    hipLaunchKernelGGL(q27::k27_flags_boxes, dim3((unsigned)blocks, (unsigned)nb), dim3(kBlock), 0, st, flags, bodies,
+                       nb, g, boxes);
     return q27::last_error();
 }
 

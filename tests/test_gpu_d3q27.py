@@ -206,3 +206,20 @@ def test_d3q27_forces_deterministic_and_mass_conserved():
     m1, p1 = g.conservation()
     assert abs(m1 - m0) / m0 < 1e-13
     assert abs((p1[0] - p0[0]) - 50 * m0 * 1e-4) < 1e-10 * m0
+
+
+def test_d3q27_moving_bodies_across_periodic_boundary_fp64():
+    """Two moving bodies, one crossing the periodic x boundary: the moved flags are
+    rebuilt in boxes around each body's old and new pose (wrapping)."""
+    cfg = _q27(periodic_box(n=24, collision="TRT", g=(1e-5, 0.0, 0.0)))
+    laws = [ConstantMotion("sphere", (22.6, 11.3, 12.4), (3.2,), vel=(0.06, 0.0, 0.01)),
+            ConstantMotion("ellipsoid", (9.0, 12.0, 11.0), (5.0, 3.0, 2.5), vel=(0.0, -0.02, 0.0),
+                           omega=(0.01, 0.0, 0.02), rot0=rotation_matrix((0, 1, 1), 0.3))]
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_motion(laws)
+        s.init_equilibrium()
+    Fg, Fo = g.step(70), o.step(70)
+    assert np.array_equal(g.flags(), o.flags())
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], "moving-wrap")
