@@ -153,6 +153,7 @@ def config_from_params(p: LBMParams, storage: str = "fp32", wall_point: str = "p
       anti-bounce-back outlet at x = Nx, z periodic;
     * collision Coll with omega+- from Re, U_inf and Lambda (src/Parameters.jl:60-72);
     * solid boundary BC (SBB | IBB);
+    * the lattice q (gen_lattice_sets, src/Utilities.jl:945-993: 19 or 27);
     * the body: the z-extruded cylinder of radius R, or the NACA airfoil of
       chord Chord pitched by theta0 about its quarter-chord pivot
       (init_solid_mask_state, src/Geometry.jl:26-39), centred at (x_c, y_c);
@@ -177,7 +178,8 @@ def config_from_params(p: LBMParams, storage: str = "fp32", wall_point: str = "p
     cfg = Config(p.Nx, p.Ny, p.Nz, collision=coll, storage=storage, omega_plus=p.omega_p,
                  omega_minus=p.omega_m if coll == "TRT" else p.omega_p,
                  faces=[velocity(U), pressure(), velocity(U), velocity(U), periodic(), periodic()],
-                 rho0=float(p.rho_inf), u0=U, wall_point=wall_point, refill=refill, solid_bc=p.bc.upper())
+                 rho0=float(p.rho_inf), u0=U, wall_point=wall_point, refill=refill, solid_bc=p.bc.upper(),
+                 q=int(p.q))
     A = 0.2 * 2.0 * float(p.R)
     w = 2.0 * math.pi * 0.154 * float(p.U_inf) / (2.0 * float(p.R))
     centre = (float(p.x_c) / h, float(p.y_c) / h, 0.5 * p.Nz)

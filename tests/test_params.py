@@ -56,3 +56,18 @@ def test_airfoil_defaults_naca():
     p = read_params_input(fh.name)
     os.unlink(fh.name)
     assert p.naca_code == "0012" and p.coll == "BGK" and p.shape == "AIR"
+
+
+def test_lattice_q_reaches_the_config():
+    """gen_lattice_sets(q) (src/Utilities.jl:945-993): the input file's q selects the lattice."""
+    import dataclasses
+
+    from lbm3d_bb_amd.params import config_from_params
+
+    p = read_params_input(os.path.join(HERE, "golden", "params_sphere_case.txt"))
+    cfg, _ = config_from_params(p)
+    assert cfg.q == 19 and cfg.to_c().q == 19
+    cfg27, _ = config_from_params(dataclasses.replace(p, q=27))
+    assert cfg27.q == 27 and cfg27.to_c().q == 27
+    with pytest.raises(ValueError):
+        config_from_params(dataclasses.replace(p, q=15))[0].to_c()
