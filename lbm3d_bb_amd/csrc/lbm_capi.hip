@@ -720,6 +720,7 @@ q27::StepArgs q27_args(const lbm_ctx* c, int mode, const void* fin, void* fout) 
     a.storage = c->cfg.storage;
     a.trt = c->cfg.collision == LBM_TRT;
     a.forced = c->forced ? 1 : 0;
+    a.ibb = c->cfg.solid_bc == LBM_SOLID_IBB ? 1 : 0;
     a.g = c->g;
     a.wp = c->pd.wp;
     a.wm = c->pd.wm;
@@ -1013,8 +1014,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     if (cfg->solid_bc != LBM_SOLID_SBB && cfg->solid_bc != LBM_SOLID_IBB) return LBM_ERR_ARG;
     const int q = cfg->q == 0 ? 19 : cfg->q;
     if (q != 19 && q != 27) return LBM_ERR_ARG;
-    // D3Q27: FP64 / FP32 storage and simple bounce-back
-    if (q == 27 && (cfg->storage > LBM_FP32 || cfg->solid_bc != LBM_SOLID_SBB)) return LBM_ERR_ARG;
+    // D3Q27: FP64 / FP32 storage
+    if (q == 27 && cfg->storage > LBM_FP32) return LBM_ERR_ARG;
     const int L[3] = {cfg->nx, cfg->ny, cfg->nz};
     for (int a = 0; a < 3; ++a) {
         const int k0 = cfg->faces[2 * a].kind, k1 = cfg->faces[2 * a + 1].kind;

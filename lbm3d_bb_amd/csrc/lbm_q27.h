@@ -16,6 +16,7 @@ struct StepArgs {
     int storage;  // LBM_FP64 or LBM_FP32
     int trt;
     int forced;
+    int ibb;  // links: 1 interpolated bounce-back, 0 simple
     Grid g;
     double wp, wm;
     double grav[3];
@@ -38,6 +39,7 @@ struct StepArgs {
 }  // namespace q27
 
 int q27_launch_step(const q27::StepArgs& a);
+int q27_launch_step_ibb(const q27::StepArgs& a);  // lbm_q27_ibb.hip
 // out[s][k] = sum over the blocks b within reach of a body of
 // partials[s * stride + b * n6 + k], s < nsteps; recs + s * rec_bytes holds
 // step s's bodies (nb of the new pose, then nb of the old one)

@@ -1,0 +1,739 @@
+// lbm_q27_kernel.h -- the D3Q27 step kernel and its launch templates, shared by
+// lbm_q27.hip (simple bounce-back) and lbm_q27_ibb.hip (interpolated).
+//
+// The D3Q27 lattice (src/Utilities.jl:962-976): the same time
+// step as the D3Q19 path (pull stream of the previous post-collision state,
+// faces with precedence, moving bounce-back with the wall-velocity
+// correction, fresh-node refill, momentum exchange, collision of the next
+// step) with the reference's D3Q27 velocities, weights and opposite table.
+//
+// One thread per cell, one launch per step.  A cell away from the non-periodic
+// velocity / anti-bounce-back faces and from every body (the usual case) only
+// pulls its 27 populations and collides.  A cell with a face rule or a solid
+// neighbour also reads its own 27 populations for the previous-step moments,
+// and its rule directions load the cell's own f*[opp(d)] in place of the pull
+// (the address is selected, the load is unconditional).  FP64 and FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "lbm_device.h"
+#include "lbm_q27.h"
+
+namespace lbm {
+namespace q27 {
+
+constexpr int QQ = kQ;
+// ex, ey, ez of src/Utilities.jl:962-964: six faces, twelve edges, eight
+// corners, rest last; moving directions pair as (2k, 2k + 1)
+constexpr int kX[QQ] = {1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 0};
+constexpr int kY[QQ] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1, 0};
+constexpr int kZ[QQ] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 0};
+
+// bbOpp (src/Utilities.jl:976)
+__host__ __device__ constexpr int opp(int d) { return d == QQ - 1 ? d : (d ^ 1); }
+
+// w (src/Utilities.jl:967-973): 8/27 rest, 2/27 face, 1/54 edge, 1/216 corner
+template <typename C>
+__device__ inline C w27(int d) {
+    const int n = (kX[d] != 0) + (kY[d] != 0) + (kZ[d] != 0);
+    return n == 0 ? C(8) / C(27) : (n == 1 ? C(2) / C(27) : (n == 2 ? C(1) / C(54) : C(1) / C(216)));
+}
+
+template <typename C>
+__device__ inline void moments27(const C f[QQ], C& rho, C& jx, C& jy, C& jz) {
+    rho = C(0);
+    jx = jy = jz = C(0);
+#pragma unroll
+    for (int d = 0; d < QQ; ++d) {
+        rho += f[d];
+        if (kX[d] > 0) jx += f[d];
+        if (kX[d] < 0) jx -= f[d];
+        if (kY[d] > 0) jy += f[d];
+        if (kY[d] < 0) jy -= f[d];
+        if (kZ[d] > 0) jz += f[d];
+        if (kZ[d] < 0) jz -= f[d];
+    }
+}
+
+// Previous-step rho, u of a cell from its post-collision populations: the
+// collision keeps rho and adds F to the momentum, so u = j*/rho - g/2.
+template <typename C, bool FORCED>
+__device__ inline void prev_moments(C rho, C jx, C jy, C jz, const Phys<C>& p, C& rho_out, C u[3], C& uu2) {
+    rho_out = rho;
+    u[0] = jx / rho;
+    u[1] = jy / rho;
+    u[2] = jz / rho;
+    if (FORCED) {
+        u[0] -= C(0.5) * p.g[0];
+        u[1] -= C(0.5) * p.g[1];
+        u[2] -= C(0.5) * p.g[2];
+    }
+    uu2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+}
+
+// BGK (src/Collision.jl:232-240) / TRT (src/Collision.jl:136-170) with the Guo
+// source, as for D3Q19: u = (sum f c + F/2) / rho, S split into even and odd
+// parts scaled by (1 - omega_pm / 2).
+template <typename C, int COLL, bool FORCED>
+__device__ inline void collide27(C f[QQ], const Phys<C>& p) {
+    C rho, jx, jy, jz;
+    moments27<C>(f, rho, jx, jy, jz);
+    const C inv = C(1) / rho;
+    C ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    C Fx = C(0), Fy = C(0), Fz = C(0), uF = C(0);
+    if (FORCED) {
+        ux += C(0.5) * p.g[0];
+        uy += C(0.5) * p.g[1];
+        uz += C(0.5) * p.g[2];
+        Fx = rho * p.g[0];
+        Fy = rho * p.g[1];
+        Fz = rho * p.g[2];
+        uF = ux * Fx + uy * Fy + uz * Fz;
+    }
+    const C u2 = ux * ux + uy * uy + uz * uz;
+    const C wp = p.wp;
+    if (COLL == LBM_TRT) {
+        const C wm = p.wm;
+        const C kp = C(1) - C(0.5) * wp, km = C(1) - C(0.5) * wm;
+#pragma unroll
+        for (int d = 0; d < QQ - 1; d += 2) {
+            const C w = w27<C>(d);
+            const C cu = C(kX[d]) * ux + C(kY[d]) * uy + C(kZ[d]) * uz;
+            const C feq_p = w * rho * (C(1) + C(4.5) * cu * cu - C(1.5) * u2);
+            const C feq_m = w * rho * (C(3) * cu);
+            const C fp = C(0.5) * (f[d] + f[d + 1]);
+            const C fm = C(0.5) * (f[d] - f[d + 1]);
+            C fpn = fp - wp * (fp - feq_p);
+            C fmn = fm - wm * (fm - feq_m);
+            if (FORCED) {
+                const C cF = C(kX[d]) * Fx + C(kY[d]) * Fy + C(kZ[d]) * Fz;
+                fpn += kp * (w * (C(9) * cu * cF - C(3) * uF));
+                fmn += km * (w * (C(3) * cF));
+            }
+            f[d] = fpn + fmn;
+            f[d + 1] = fpn - fmn;
+        }
+        const C w0 = w27<C>(QQ - 1);
+        const C feq0 = w0 * rho * (C(1) - C(1.5) * u2);
+        C r = f[QQ - 1] - wp * (f[QQ - 1] - feq0);
+        if (FORCED) r += kp * (w0 * (-C(3) * uF));
+        f[QQ - 1] = r;
+    } else {
+        const C kp = C(1) - C(0.5) * wp;
+#pragma unroll
+        for (int d = 0; d < QQ; ++d) {
+            const C w = w27<C>(d);
+            const C cu = C(kX[d]) * ux + C(kY[d]) * uy + C(kZ[d]) * uz;
+            const C feq = w * rho * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * u2);
+            C r = f[d] - wp * (f[d] - feq);
+            if (FORCED) {
+                const C cF = C(kX[d]) * Fx + C(kY[d]) * Fy + C(kZ[d]) * Fz;
+                r += kp * (w * (C(3) * cF + C(9) * cu * cF - C(3) * uF));
+            }
+            f[d] = r;
+        }
+    }
+}
+
+// Equilibrium in the reference's form w rho (1 + 3cu + 4.5cu^2 - 1.5u^2)
+// (src/Initialization.jl:37-42)
+template <typename C>
+__device__ inline C feq27(int d, C r, C ux, C uy, C uz) {
+    const C cu = C(kX[d]) * ux + C(kY[d]) * uy + C(kZ[d]) * uz;
+    const C u2 = ux * ux + uy * uy + uz * uz;
+    return w27<C>(d) * r * (C(1) + cu * C(3) + C(0.5) * (cu * cu) * C(9) - C(0.5) * u2 * C(3));
+}
+
+// Cell coordinates with 32-bit divisions (cell counts stay below 2^31).
+__device__ inline void xyz_of(int64_t cell, const Grid& g, int& x, int& y, int& z) {
+    const uint32_t c = (uint32_t)cell, nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
+    const uint32_t t = c / nx;
+    x = (int)(c - t * nx);
+    const uint32_t zz = t / ny;
+    y = (int)(t - zz * ny);
+    z = (int)zz;
+}
+
+// A cell whose neighbourhood may hold a solid node of body B: every neighbour
+// of a cell farther than B.ext from the centre (on some axis) lies outside.
+__device__ inline bool near_body(const DevBody& B, int x, int y, int z, const Grid& g) {
+    if (B.reach < 0.0f) return true;  // explicit flags: no analytic extent
+    const float dx = min_image((float)x + 0.5f - B.c[0], g.per & 1, g.nx);
+    const float dy = min_image((float)y + 0.5f - B.c[1], g.per & 2, g.ny);
+    const float dz = min_image((float)z + 0.5f - B.c[2], g.per & 4, g.nz);
+    return fabsf(dx) <= B.ext[0] && fabsf(dy) <= B.ext[1] && fabsf(dz) <= B.ext[2];
+}
+
+// Faces with a velocity (bit f of vel) or anti-bounce-back (abb) rule, as bit
+// masks: a select chain over the face table would be turned back into an
+// indexed read of the kernel argument, which lives in private memory.
+struct RuleFaces {
+    uint32_t vel, abb;
+};
+
+__device__ inline RuleFaces rule_faces_of(const Grid& g) {
+    RuleFaces r{0u, 0u};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        r.vel |= (g.face_kind[f] == LBM_FACE_VELOCITY ? 1u : 0u) << f;
+        r.abb |= (g.face_kind[f] == LBM_FACE_PRESSURE_ABB ? 1u : 0u) << f;
+    }
+    return r;
+}
+
+// Whether the rows (y, z) of this block's cells come within reach of a body,
+// from block-uniform values: the y and z intervals of the rows, against each
+// body's extent (min image on periodic axes; an interval of half the axis or
+// more counts as near).
+__device__ inline bool interval_near(int lo, int hi, float c, float ext, bool per, int L) {
+    const float half = 0.5f * (float)(hi - lo + 1);
+    if (per && 2 * (hi - lo + 1) >= L) return true;
+    const float m = 0.5f * (float)(lo + hi + 1);
+    return fabsf(min_image(m - c, per, L)) <= ext + half;
+}
+
+__device__ inline bool block_near(uint32_t block, const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
+                                  int nb, const Grid& g) {
+    if (nb <= 0) return false;
+    // 32-bit arithmetic: cell counts stay below 2^31
+    const uint32_t c0 = block * (uint32_t)kBlock;
+    const uint32_t c1 = (c0 + kBlock < (uint32_t)g.N ? c0 + kBlock : (uint32_t)g.N) - 1u;
+    const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
+    const uint32_t r0 = c0 / nx, r1 = c1 / nx;  // rows y + ny z
+    const int z0 = (int)(r0 / ny), z1 = (int)(r1 / ny);
+    const int y0 = z0 == z1 ? (int)(r0 - (uint32_t)z0 * ny) : 0;
+    const int y1 = z0 == z1 ? (int)(r1 - (uint32_t)z1 * ny) : g.ny - 1;
+    bool hit = false;
+    for (int b = 0; b < nb && !hit; ++b) {
+        for (int k = 0; k < 2 && !hit; ++k) {
+            if (k == 1 && bold == nullptr) break;
+            const DevBody& B = k == 0 ? bnew[b] : bold[b];
+            hit = B.reach < 0.0f || (interval_near(y0, y1, B.c[1], B.ext[1], g.per & 2, g.ny) &&
+                                     interval_near(z0, z1, B.c[2], B.ext[2], g.per & 4, g.nz));
+        }
+    }
+    return hit;
+}
+
+// Pull sources x - c of a cell for the offsets c = -1, 0, 1 on each axis
+// (index c + 1), as cell-index terms: wrapped on periodic axes, clamped onto
+// the cell on the others (zero gradient; the face rules replace the rest).
+// Bit 3a + (c + 1) of `in`: that source exists (no clamp).  The source of
+// direction d is x[1 + cx] + y[1 + cy] + z[1 + cz], two integer adds.
+struct Sources {
+    int x[3], y[3], z[3];
+    uint32_t in;
+};
+
+__device__ inline Sources sources_of(int qx, int qy, int qz, const Grid& g) {
+    Sources s;
+    const int q[3] = {qx, qy, qz};
+    const int L[3] = {g.nx, g.ny, g.nz};
+    const int sc[3] = {1, g.nx, (int)g.nxy};
+    int t[3][3];
+    s.in = 0u;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool per = (g.per >> a) & 1;
+        const bool lo_in = q[a] > 0, hi_in = q[a] < L[a] - 1;
+        const int lo = lo_in ? q[a] - 1 : (per ? L[a] - 1 : q[a]);  // source for c = +1
+        const int hi = hi_in ? q[a] + 1 : (per ? 0 : q[a]);         // source for c = -1
+        t[a][0] = hi * sc[a];
+        t[a][1] = q[a] * sc[a];
+        t[a][2] = lo * sc[a];
+        s.in |= ((hi_in || per) ? 1u : 0u) << (3 * a) | 1u << (3 * a + 1) | ((lo_in || per) ? 1u : 0u) << (3 * a + 2);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s.x[k] = t[0][k];
+        s.y[k] = t[1][k];
+        s.z[k] = t[2][k];
+    }
+    return s;
+}
+
+__device__ inline int src_of(const Sources& s, int d) { return s.x[1 + kX[d]] + s.y[1 + kY[d]] + s.z[1 + kZ[d]]; }
+
+__device__ inline bool src_in(const Sources& s, int d) {
+    const uint32_t need = 1u << (1 + kX[d]) | 1u << (3 + 1 + kY[d]) | 1u << (6 + 1 + kZ[d]);
+    return (s.in & need) == need;
+}
+
+// Face whose rule gives direction d when x - c_d lies beyond non-periodic
+// faces: VELOCITY > PRESSURE_ABB > ZERO_GRADIENT, lower axis first
+// (velocity_BC!, src/OtherBC.jl:137-166).  -1: zero gradient or inside.
+__device__ inline int face_rule_of(const int q0[3], int d, const Grid& g, RuleFaces rf, bool* velocity) {
+    const int L[3] = {g.nx, g.ny, g.nz};
+    const int cd[3] = {kX[d], kY[d], kZ[d]};
+    uint32_t out = 0u;  // bit f: the source lies beyond face f
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int s = q0[a] - cd[a];
+        if (!((g.per >> a) & 1) && (s < 0 || s >= L[a])) out |= 1u << (2 * a + (s >= L[a] ? 1 : 0));
+    }
+    const uint32_t v = out & rf.vel, b = out & rf.abb;
+    *velocity = v != 0u;
+    const uint32_t m = v != 0u ? v : b;
+    return m != 0u ? __builtin_ctz(m) : -1;
+}
+
+// Face rules for direction d whose source lies beyond the chosen face:
+// VELOCITY src/OtherBC.jl:143-148, PRESSURE_ABB src/OtherBC.jl:156-165.
+template <typename C>
+__device__ inline C velocity_rule(int d, C own_opp, const C uf[3], C rho_prev) {
+    const C cu = C(kX[d]) * (-uf[0]) + C(kY[d]) * (-uf[1]) + C(kZ[d]) * (-uf[2]);
+    return own_opp - C(2) * w27<C>(d) * rho_prev * cu * C(3);
+}
+
+template <typename C>
+__device__ inline C abb_rule(int d, C own_opp, const C up[3], C uu2, C rho_prev) {
+    const C cu = C(kX[d]) * up[0] + C(kY[d]) * up[1] + C(kZ[d]) * up[2];
+    return -own_opp + C(2) * w27<C>(d) * rho_prev * (C(1) + C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3));
+}
+
+// Directions whose source x - c_d lies beyond face f = 2a + side: c_a = +1 on
+// the low face (side 0), c_a = -1 on the high one.
+__host__ __device__ constexpr uint32_t face_dirs(int f) {
+    uint32_t m = 0u;
+    for (int d = 0; d < QQ; ++d) {
+        const int c = (f >> 1) == 0 ? kX[d] : ((f >> 1) == 1 ? kY[d] : kZ[d]);
+        if (c == ((f & 1) ? -1 : 1)) m |= 1u << d;
+    }
+    return m;
+}
+
+// Rule directions of a cell's faces: the union of face_dirs over the rule
+// faces the cell lies on (a direction beyond several faces takes the first
+// rule face in the precedence order, which face_rule_of picks).  *single: the
+// one rule face of a cell on exactly one, else -1; *abb: the directions whose
+// rule is the anti-bounce-back one (the only rule that reads the velocity).
+__device__ inline uint32_t face_rule_mask(const int q0[3], const Grid& g, RuleFaces rf, int* single, uint32_t* abb) {
+    const int L[3] = {g.nx, g.ny, g.nz};
+    uint32_t vm = 0u, am = 0u;
+    int n = 0, one = -1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if ((g.per >> a) & 1) continue;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int fc = 2 * a + side;
+            if (q0[a] != (side ? L[a] - 1 : 0)) continue;
+            if ((rf.vel >> fc) & 1u) vm |= face_dirs(fc);
+            else if ((rf.abb >> fc) & 1u) am |= face_dirs(fc);
+            else continue;
+            ++n;
+            one = fc;
+        }
+    }
+    *single = n == 1 ? one : -1;
+    *abb = am & ~vm;
+    return vm | am;
+}
+
+// The rule of face FC on its nine directions, for a cell on that face only
+// (f[d] holds the cell's own f*[opp(d)] the rule starts from).
+template <typename C, int FC>
+__device__ inline void one_face_rule(C f[QQ], bool velocity, const C uf[3], const C up[3], C uu2, C rho_prev) {
+#pragma unroll
+    for (int d = 0; d < QQ; ++d) {
+        if (!((face_dirs(FC) >> d) & 1u)) continue;
+        f[d] = velocity ? velocity_rule<C>(d, f[d], uf, rho_prev) : abb_rule<C>(d, f[d], up, uu2, rho_prev);
+    }
+}
+
+// In-domain cell x - c_d (periodic wrap), or -1 beyond a non-periodic face.
+__device__ inline int64_t in_domain_source(const int q0[3], int d, const Grid& g) {
+    const int L[3] = {g.nx, g.ny, g.nz};
+    int s[3] = {q0[0] - kX[d], q0[1] - kY[d], q0[2] - kZ[d]};
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ok = ok && wrap_coord(s[a], L[a], (g.per >> a) & 1);
+    return ok ? (int64_t)s[0] + (int64_t)g.nx * s[1] + g.nxy * s[2] : (int64_t)-1;
+}
+
+// Arm component on axis a, minimum image on periodic axes.
+__device__ inline double wrap_arm(double r, int a, const Grid& g) {
+    if ((g.per >> a) & 1) {
+        const int L = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+        const double h = 0.5 * L;
+        if (r > h) r -= L;
+        else if (r < -h) r += L;
+    }
+    return r;
+}
+
+// IBB: the interpolated bounce-back (IBB!, src/SolidBC.jl:253-350) on the links
+// instead of the simple one: wall fraction q from the body's level set at the
+// cell and at the solid node (FP32, clamped to [kEpsQ, 1 - kEpsQ]); rule 1
+// (q < 1/2 with a fluid second node x - c_dl), rule 2 (q >= 1/2, the wall
+// term over 2q), else the simple rule; wall velocity and torque arm at x + q c_dl.
+template <class S, int COLL, bool FORCED, int MODE, bool IBB>
+__global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                   double* __restrict__ gout, double* __restrict__ rho_out,
+                                                   double* __restrict__ u_out, Grid g, Phys<typename S::C> p,
+                                                   const uint8_t* __restrict__ fl_new, const uint8_t* __restrict__ fl_old,
+                                                   const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
+                                                   int nb, double* __restrict__ partials, int refill,
+                                                   const double* __restrict__ face_u) {
+    using C = typename S::C;
+    extern __shared__ double lds_dyn[];
+    const int n6 = nb * 6;
+    double* s_over = lds_dyn + kWaves * n6;  // links of a second body in one thread
+    // IBB: the wall fraction of each link direction, one column per thread
+    float* lds_q = reinterpret_cast<float*>(lds_dyn + (kWaves + 1) * n6) + threadIdx.x;
+    // Only blocks within reach of a body can hold links: they alone reduce
+    // forces and write their partials (the sum visits the same blocks).
+    const bool bnear = MODE != MODE_COLLIDE_ONLY && MODE != MODE_GET_RAW && block_near(blockIdx.x, bnew, bold, nb, g);
+    if (MODE == MODE_STEP && bnear) {
+        for (int i = threadIdx.x; i < (kWaves + 1) * n6; i += kBlock) lds_dyn[i] = 0.0;
+        __syncthreads();
+    }
+    ForceAcc acc;
+    acc.b0 = -1;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc.a0[c] = 0.0;
+    const int64_t N = g.N;
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell < N) {
+        C f[QQ];
+        if (MODE == MODE_COLLIDE_ONLY || MODE == MODE_GET_RAW) {
+#pragma unroll
+            for (int d = 0; d < QQ; ++d) f[d] = S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
+        } else {
+            int x, y, z;
+            xyz_of(cell, g, x, y, z);
+            const int q0[3] = {x, y, z};
+            // rule directions of the faces (fmask); single: the one rule face of the cell
+            const RuleFaces rf = rule_faces_of(g);
+            int single = -1;
+            uint32_t abb_dirs = 0u;
+            const uint32_t fmask = face_rule_mask(q0, g, rf, &single, &abb_dirs);
+            const Sources src = sources_of(x, y, z, g);
+            bool near = false;
+            if (bnear) {
+                for (int b = 0; b < nb && !near; ++b)
+                    near = near_body(bnew[b], x, y, z, g) || (bold != nullptr && near_body(bold[b], x, y, z, g));
+            }
+            const int self = near ? (fl_new[cell] & kFlagBody) : 0;
+            const bool fresh = near && refill && self == 0 && (fl_old[cell] & kFlagBody) != 0;
+            // links to a solid source (lmask)
+            uint32_t lmask = 0u;
+            int lbody = -1;       // body of the links (the last one met)
+            bool lmixed = false;  // links to more than one body
+            if (near && self == 0 && !fresh) {
+                // 26 independent flag loads at in-domain (clamped) addresses
+#pragma unroll
+                for (int d = 0; d < QQ - 1; ++d) {
+                    const uint8_t fv = fl_new[src_of(src, d)];
+                    const int b = (fv & kFlagBody) - 1;
+                    if (src_in(src, d) && b >= 0) {
+                        lmask |= 1u << d;
+                        lmixed = lmixed || (lbody >= 0 && lbody != b);
+                        lbody = b;
+                    }
+                }
+            }
+            // IBB: rule of every link (2 bits per direction) and its wall fraction,
+            // in a loop that is not unrolled (the level sets are long)
+            uint64_t rules = 0u;
+            if (IBB && lmask != 0u) {
+                const float px = (float)x + 0.5f, py = (float)y + 0.5f, pz = (float)z + 0.5f;
+#pragma unroll 1
+                for (int d = 0; d < QQ - 1; ++d) {
+                    if (!((lmask >> d) & 1u)) continue;
+                    // link (x, dl), dl = opp(d): the solid node x + c_dl is the pull source
+                    const int64_t sc = in_domain_source(q0, d, g);
+                    const int bi = lmixed ? (fl_new[sc] & kFlagBody) - 1 : lbody;
+                    const DevBody& B = bnew[bi];
+                    const float cx = (float)(-kX[d]), cy = (float)(-kY[d]), cz = (float)(-kZ[d]);
+                    const float phb = body_phi(B, px, py, pz, g);
+                    const float phs = body_phi(B, px + cx, py + cy, pz + cz, g);
+                    const float den = phb - phs;
+                    if (!(phb > 0.0f && phs < 0.0f && fabsf(den) >= 1e-8f)) continue;
+                    const float q = fminf(fmaxf(phb / den, kEpsQ), 1.0f - kEpsQ);
+                    uint64_t rule = 2u;
+                    if (q < 0.5f) {
+                        const int64_t s2 = in_domain_source(q0, opp(d), g);  // x - c_dl = x + c_d
+                        rule = s2 >= 0 && (fl_new[s2] & kFlagBody) == 0 ? 1u : 0u;
+                    }
+                    if (rule != 0u) {
+                        lds_q[d * kBlock] = q;
+                        rules |= rule << (2 * d);
+                    }
+                }
+            }
+            if (fresh) {
+                // equilibrium of the mean previous-step rho, u of the neighbours
+                // x + c_d that are fluid and not fresh (refill, extension)
+                // (a rare path: one neighbour at a time keeps the registers of
+                // the step's other paths low)
+                C sr = C(0), su = C(0), sv = C(0), sw = C(0);
+                int cnt = 0;
+                for (int d = 0; d < QQ - 1; ++d) {
+                    const int64_t nc = in_domain_source(q0, opp(d), g);
+                    if (nc < 0 || (fl_new[nc] & kFlagBody) != 0 || (fl_old[nc] & kFlagBody) != 0) continue;
+                    C r = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+                    for (int e = 0; e < QQ; ++e) {
+                        const C v = S::load(fin + (int64_t)e * g.P + nc, e, g.sh);
+                        r += v;
+                        if (kX[e] > 0) jx += v;
+                        if (kX[e] < 0) jx -= v;
+                        if (kY[e] > 0) jy += v;
+                        if (kY[e] < 0) jy -= v;
+                        if (kZ[e] > 0) jz += v;
+                        if (kZ[e] < 0) jz -= v;
+                    }
+                    C rr, uu[3], u2;
+                    prev_moments<C, FORCED>(r, jx, jy, jz, p, rr, uu, u2);
+                    sr += rr;
+                    su += uu[0];
+                    sv += uu[1];
+                    sw += uu[2];
+                    ++cnt;
+                }
+                C r = C(g.rho0), ux = C(0), uy = C(0), uz = C(0);
+                if (cnt > 0) {
+                    const C n = C(cnt);
+                    r = sr / n;
+                    ux = su / n;
+                    uy = sv / n;
+                    uz = sw / n;
+                }
+#pragma unroll
+                for (int d = 0; d < QQ; ++d) f[d] = feq27<C>(d, r, ux, uy, uz);
+            } else {
+                const uint32_t rmask = fmask | lmask;
+                // A cell on one rule face and with no link (an x-face cell shares
+                // its wave with the row's bulk cells, a wall row is a wave of them)
+                // takes the short path: that face's rule on its nine directions.
+                const bool one_face = single >= 0 && lmask == 0u;
+                C rho_prev = C(1), up[3] = {C(0), C(0), C(0)}, uu2 = C(0);
+                if (rmask) {
+                    // previous-step moments from the cell's own f*: rho for the
+                    // velocity and link rules, u too for an anti-bounce-back one
+                    C r = C(0), jx = C(0), jy = C(0), jz = C(0);
+                    if (abb_dirs != 0u) {
+#pragma unroll
+                        for (int d = 0; d < QQ; ++d) {
+                            const C v = S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
+                            r += v;
+                            if (kX[d] > 0) jx += v;
+                            if (kX[d] < 0) jx -= v;
+                            if (kY[d] > 0) jy += v;
+                            if (kY[d] < 0) jy -= v;
+                            if (kZ[d] > 0) jz += v;
+                            if (kZ[d] < 0) jz -= v;
+                        }
+                        prev_moments<C, FORCED>(r, jx, jy, jz, p, rho_prev, up, uu2);
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < QQ; ++d) r += S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
+                        rho_prev = r;
+                    }
+                }
+                // pulls; a rule direction loads the cell's own f*[opp(d)] instead,
+                // the value its rule starts from
+                if (rmask == 0u) {
+#pragma unroll
+                    for (int d = 0; d < QQ; ++d) f[d] = S::load(fin + (int64_t)d * g.P + src_of(src, d), d, g.sh);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < QQ; ++d) {
+                        const bool rule = (rmask >> d) & 1u;
+                        const int64_t off = rule ? (int64_t)opp(d) * g.P + cell : (int64_t)d * g.P + src_of(src, d);
+                        f[d] = S::load(fin + off, d, g.sh);
+                    }
+                }
+                if (one_face) {
+                    const int fc = single;
+                    const C uf[3] = {C(face_u[3 * fc]), C(face_u[3 * fc + 1]), C(face_u[3 * fc + 2])};
+                    const bool vel = (rf.vel >> fc) & 1u;
+                    switch (fc) {
+                        case 0: one_face_rule<C, 0>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 1: one_face_rule<C, 1>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 2: one_face_rule<C, 2>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 3: one_face_rule<C, 3>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 4: one_face_rule<C, 4>(f, vel, uf, up, uu2, rho_prev); break;
+                        default: one_face_rule<C, 5>(f, vel, uf, up, uu2, rho_prev); break;
+                    }
+                } else if (rmask) {
+                    // wall point of the moving-wall velocity: physical x + c_dl/2 or
+                    // the reference's x - c_dl/2 (src/SolidBC.jl:195-196)
+                    const double hs = g.wall_point == LBM_WALL_REFERENCE ? -0.5 : 0.5;
+                    // kinematics of the link body, read once when every link of the
+                    // cell meets one body (no dependent loads inside the loop)
+                    double bch[3] = {0.0, 0.0, 0.0}, bv[3] = {0.0, 0.0, 0.0}, bom[3] = {0.0, 0.0, 0.0};
+                    if (lmask != 0u && !lmixed) {
+                        const DevBody& B = bnew[lbody];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            bch[a] = B.ch[a];
+                            bv[a] = B.v[a];
+                            bom[a] = B.om[a];
+                        }
+                    }
+#pragma unroll
+                    for (int d = 0; d < QQ; ++d) {
+                        if (!((rmask >> d) & 1u)) continue;
+                        const C own_opp = f[d];
+                        const C w = w27<C>(d);
+                        if ((fmask >> d) & 1u) {
+                            bool vel;
+                            const int ch = face_rule_of(q0, d, g, rf, &vel);
+                            const C uf[3] = {C(face_u[3 * ch]), C(face_u[3 * ch + 1]), C(face_u[3 * ch + 2])};
+                            if (vel) f[d] = velocity_rule<C>(d, own_opp, uf, rho_prev);
+                            else f[d] = abb_rule<C>(d, own_opp, up, uu2, rho_prev);
+                        } else {
+                            // boundary link (x, dl), dl = opp(d), solid node x + c_dl
+                            // (SBB!, src/SolidBC.jl:187-208)
+                            int bi = lbody;
+                            double ch[3] = {bch[0], bch[1], bch[2]}, v[3] = {bv[0], bv[1], bv[2]};
+                            double om[3] = {bom[0], bom[1], bom[2]};
+                            if (lmixed) {
+                                bi = (fl_new[src_of(src, d)] & kFlagBody) - 1;
+                                const DevBody& B = bnew[bi];
+#pragma unroll
+                                for (int a = 0; a < 3; ++a) {
+                                    ch[a] = B.ch[a];
+                                    v[a] = B.v[a];
+                                    om[a] = B.om[a];
+                                }
+                            }
+                            const int cdl[3] = {-kX[d], -kY[d], -kZ[d]};
+                            const int rule = IBB ? (int)((rules >> (2 * d)) & 3u) : 0;
+                            const double qw = IBB && rule != 0 ? (double)lds_q[d * kBlock] : 0.5;
+                            // wall point x + h c_dl: h = q (interpolated) or 1/2 (simple),
+                            // mirrored for the reference's sampling point
+                            const double hw = rule == 0 ? hs : (g.wall_point == LBM_WALL_REFERENCE ? -qw : qw);
+                            double r[3];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a)
+                                r[a] = wrap_arm((double)q0[a] + 0.5 + hw * (double)cdl[a] - ch[a], a, g);
+                            const C uwx = C(v[0] + (om[1] * r[2] - om[2] * r[1]));
+                            const C uwy = C(v[1] + (om[2] * r[0] - om[0] * r[2]));
+                            const C uwz = C(v[2] + (om[0] * r[1] - om[1] * r[0]));
+                            const C cuw = C(cdl[0]) * uwx + C(cdl[1]) * uwy + C(cdl[2]) * uwz;
+                            const C corr = C(2) * w * rho_prev * cuw * C(3);
+                            C val = own_opp - corr;
+                            if (IBB && rule != 0) {
+                                // rule 1: f*_dl at the second fluid node x - c_dl; rule 2: the
+                                // cell's own f*_d (SBB/IBB forms of src/SolidBC.jl:321-347)
+                                const bool near1 = rule == 1;
+                                const int64_t off = near1 ? (int64_t)opp(d) * g.P + src_of(src, opp(d))
+                                                          : (int64_t)d * g.P + cell;
+                                const C fx = S::load(fin + off, d, g.sh);
+                                const C q2 = C(2) * C(qw);
+                                val = near1 ? q2 * own_opp + (C(1) - q2) * fx - corr
+                                            : (own_opp + (q2 - C(1)) * fx - corr) / q2;
+                            }
+                            f[d] = val;
+                            if (MODE == MODE_STEP) {
+                                // momentum exchange (src/MEA.jl:102-107) in FP64, torque arm to the
+                                // wall point x + c_dl/2 (simple) or x + q c_dl (interpolated)
+                                const double t = (double)(own_opp + val);
+                                const double P[3] = {t * cdl[0], t * cdl[1], t * cdl[2]};
+                                const double ha = rule == 0 ? 0.5 : qw;
+                                double ra[3];
+#pragma unroll
+                                for (int a = 0; a < 3; ++a)
+                                    ra[a] = wrap_arm((double)q0[a] + 0.5 + ha * (double)cdl[a] - ch[a], a, g);
+                                const double v6[6] = {P[0], P[1], P[2], ra[1] * P[2] - ra[2] * P[1],
+                                                      ra[2] * P[0] - ra[0] * P[2], ra[0] * P[1] - ra[1] * P[0]};
+                                if (acc.b0 < 0 || acc.b0 == bi) {
+                                    acc.b0 = bi;
+#pragma unroll
+                                    for (int c = 0; c < 6; ++c) acc.a0[c] += v6[c];
+                                } else {
+#pragma unroll
+                                    for (int c = 0; c < 6; ++c) atomicAdd(&s_over[bi * 6 + c], v6[c]);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (MODE == MODE_GET || MODE == MODE_GET_RAW) {
+            // f_old and its moments u = sum f c / rho + g/2 (src/ComputeMacro.jl:26-45)
+            C rho, jx, jy, jz;
+            moments27<C>(f, rho, jx, jy, jz);
+            C ux = jx / rho, uy = jy / rho, uz = jz / rho;
+            if (FORCED) {
+                ux += C(0.5) * p.g[0];
+                uy += C(0.5) * p.g[1];
+                uz += C(0.5) * p.g[2];
+            }
+            if (gout) {
+#pragma unroll
+                for (int d = 0; d < QQ; ++d) gout[(int64_t)d * N + cell] = (double)f[d];
+            }
+            if (rho_out) rho_out[cell] = (double)rho;
+            if (u_out) {
+                u_out[cell] = (double)ux;
+                u_out[N + cell] = (double)uy;
+                u_out[2 * N + cell] = (double)uz;
+            }
+        } else {
+            collide27<C, COLL, FORCED>(f, p);
+#pragma unroll
+            for (int d = 0; d < QQ; ++d) S::store(fout + (int64_t)d * g.P + cell, d, f[d], g.sh);
+        }
+    }
+    if (MODE == MODE_STEP && bnear) {
+        const int wave = threadIdx.x >> 6;
+        wave_reduce_to_lds(acc.b0, acc.a0, lds_dyn + wave * n6);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n6; i += kBlock) {
+            double s = s_over[i];
+            for (int w = 0; w < kWaves; ++w) s += lds_dyn[w * n6 + i];
+            partials[(int64_t)blockIdx.x * n6 + i] = s;
+        }
+    }
+}
+
+template <class S, int MODE, bool IBB>
+static void launch_mode(const StepArgs& a) {
+    using T = typename S::T;
+    using C = typename S::C;
+    Phys<C> p;
+    p.wp = (C)a.wp;
+    p.wm = (C)a.wm;
+    for (int k = 0; k < 3; ++k) p.g[k] = (C)a.grav[k];
+    const dim3 grid((unsigned)nblocks_of(a.g.N)), block(kBlock);
+    size_t lds = MODE == MODE_STEP ? (size_t)(kWaves + 1) * a.nb * 6 * sizeof(double) : 0;
+    if (IBB) lds = (size_t)(kWaves + 1) * a.nb * 6 * sizeof(double) + (size_t)(QQ - 1) * kBlock * sizeof(float);
+#define K27(COLL, FORCED)                                                                                           \
+    hipLaunchKernelGGL((k27_step<S, COLL, FORCED, MODE, IBB>), grid, block, lds, a.stream, (const T*)a.fin, (T*)a.fout, \
+                       a.gout, a.rho_out, a.u_out, a.g, p, a.fl_new, a.fl_old, a.bnew, a.bold, a.nb, a.partials,   \
+                       a.refill, a.face_u)
+    if (a.trt) {
+        if (a.forced) K27(LBM_TRT, true);
+        else K27(LBM_TRT, false);
+    } else {
+        if (a.forced) K27(LBM_BGK, true);
+        else K27(LBM_BGK, false);
+    }
+#undef K27
+}
+
+template <class S, bool IBB>
+static void launch_storage(const StepArgs& a) {
+    switch (a.mode) {
+        case MODE_STEP: launch_mode<S, MODE_STEP, IBB>(a); break;
+        case MODE_GET: launch_mode<S, MODE_GET, IBB>(a); break;
+        case MODE_COLLIDE_ONLY: launch_mode<S, MODE_COLLIDE_ONLY, IBB>(a); break;
+        default: launch_mode<S, MODE_GET_RAW, IBB>(a); break;
+    }
+}
+
+static int last_error() {
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : (int)e;
+}
+
+}  // namespace q27
+}  // namespace lbm

@@ -73,8 +73,8 @@ def test_missing_library_fails_loudly(tmp_path):
 
 
 def test_d3q27_rejects_unsupported_combinations():
-    """D3Q27 runs FP64/FP32 storage with simple bounce-back; lbm_create refuses
-    the rest (and lattices other than 19 / 27) before touching a device."""
+    """D3Q27 runs FP64/FP32 storage; lbm_create refuses the 16-bit formats for it
+    (and lattices other than 19 / 27) before touching a device."""
     import pytest
 
     from lbm3d_bb_amd.solver import Config, Solver, periodic
@@ -86,7 +86,7 @@ def test_d3q27_rejects_unsupported_combinations():
             setattr(c, k, v)
         return c
 
-    for bad in (cfg(q=27, storage="fp16s"), cfg(q=27, storage="bf16s"), cfg(q=27, solid_bc="IBB")):
+    for bad in (cfg(q=27, storage="fp16s"), cfg(q=27, storage="bf16s")):
         with pytest.raises(_ffi.LbmError):
             Solver(bad)
     with pytest.raises(ValueError):  # the host layer names the lattice as the reference does

@@ -223,3 +223,101 @@ def test_d3q27_moving_bodies_across_periodic_boundary_fp64():
     assert np.array_equal(g.flags(), o.flags())
     assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
     _compare_state(g, o, TOL["fp64"], "moving-wrap")
+
+
+# --- interpolated bounce-back on the D3Q27 links ---------------------------------------
+
+def _feq27(U):
+    cu = C27.astype(np.float64) @ np.asarray(U, dtype=np.float64)
+    return W27 * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * np.dot(U, U))
+
+
+@pytest.mark.parametrize("shape", ["sphere", "ellipsoid", "cylinder", "naca4"])
+def test_d3q27_ibb_uniform_flow_moving_with_the_body(shape):
+    """f_eq(U) stays a steady solution when the wall moves at U (every q, both branches)."""
+    from lbm3d_bb_amd.bodies import naca_params
+    from lbm3d_bb_amd.solver import Config, periodic
+
+    U = (0.04, -0.025, 0.015)
+    cfg = Config(32, 30, 28, collision="TRT", storage="fp64", omega_plus=1 / 0.7, omega_minus=1 / 1.1,
+                 faces=[periodic()] * 6, u0=U, refill=True, solid_bc="IBB", q=27)
+    semi = {"sphere": (6.3,), "ellipsoid": (7.7, 4.2, 3.6), "cylinder": (6.3,), "naca4": (18.0,)}[shape]
+    rot0 = rotation_matrix((1, 2, 3), 0.7) if shape == "ellipsoid" else (
+        rotation_matrix((0, 0, 1), 0.21) if shape == "naca4" else np.eye(3))
+    law = ConstantMotion(shape, (15.3, 14.8, 13.9), semi, vel=U, rot0=rot0,
+                         naca=naca_params(18.0, "2412") if shape == "naca4" else (0.0,) * 5)
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_motion([law])
+        s.init_equilibrium()
+    Fg, Fo = g.step(6), o.step(6)
+    assert np.abs(g.populations() - _feq27(U)[:, None]).max() < 1e-13
+    assert np.array_equal(g.flags(), o.flags())
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+
+
+@pytest.mark.parametrize("collision", ["BGK", "TRT"])
+def test_d3q27_ibb_static_cylinder_reference_channel_fp64(collision):
+    cfg = _q27(reference_channel(nx=64, ny=40, nz=4, storage="fp64"))
+    cfg.collision = collision
+    cfg.solid_bc = "IBB"
+    g, o = _pair(cfg, "fp64")
+    from lbm3d_bb_amd.bodies import cylinder
+
+    for s in (g, o):
+        s.set_bodies([cylinder((20.3, 19.6, 2.0), 6.37)])
+        s.init_equilibrium()
+    Fg, Fo = g.step(60), o.step(60)
+    assert np.abs(Fo).max() > 0
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], collision)
+
+
+def test_d3q27_ibb_heaving_cylinder_from_params_fp64():
+    """The reference's own input file with q = 27 (faces, BGK, IBB, heave law) on a
+    small lattice, the heave quickened so that the mask moves within the run."""
+    import dataclasses
+    import os
+
+    from lbm3d_bb_amd.params import config_from_params, read_params_input
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    p = read_params_input(os.path.join(here, "golden", "params_reference_default.txt"))
+    p = dataclasses.replace(p, Nx=120, Ny=80, Nz=3, R=6.3, x_c=40.5, y_c=40.0, Re=20.0, q=27)
+    cfg, laws = config_from_params(p, storage="fp64", refill=True)
+    assert cfg.q == 27
+    laws[0].freq = 0.05
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_motion(laws)
+        s.init_equilibrium()
+    Fg, Fo = g.step(40), o.step(40)
+    assert np.array_equal(g.flags(), o.flags())
+    cg, dg = g.links()
+    co, do = o.links()
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], "heave")
+
+
+@pytest.mark.parametrize("wall_point", ["physical", "reference"])
+def test_d3q27_ibb_moving_airfoil_and_spinning_ellipsoid_fp32(wall_point):
+    from lbm3d_bb_amd.bodies import naca_params, rotation_z
+
+    cfg = _q27(channel(nx=64, ny=40, nz=36, storage="fp32"))
+    cfg.solid_bc = "IBB"
+    cfg.wall_point = wall_point
+    laws = [ConstantMotion("naca4", (24.3, 20.2, 18.0), (22.0,), vel=(0.0, 0.01, 0.0), rot0=rotation_z(0.12),
+                           naca=naca_params(22.0, "2412")),
+            ConstantMotion("ellipsoid", (46.0, 20.0, 18.0), (6.0, 3.5, 3.0), omega=(0.01, 0.02, 0.015),
+                           rot0=rotation_matrix((1, 0, 1), 0.5))]
+    g, o = _pair(cfg, "fp32")
+    for s in (g, o):
+        s.set_motion(laws)
+        s.init_equilibrium()
+    Fg, Fo = g.step(30), o.step(30)
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp32"], force_err(Fg, Fo, _scale27(o))
+    rg, ug = g.macros()
+    ro, uo = o.macros()
+    assert rel_err(rg, ro) < TOL["fp32"]
+    assert rel_err(ug, uo) < TOL["fp32"] * 10
