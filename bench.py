@@ -52,7 +52,8 @@ _DT = {"fp64": "f64", "fp32": "f32", "fp16s": "f16 storage, f32 compute", "bf16s
 # inlet and y walls at U = 0.075, anti-bounce-back outlet, z periodic, a
 # cylinder R = 16.16 heaving with the reference's law) with its interpolated
 # bounce-back (IBB), the simple one (SBB), or the NACA 0012 airfoil of chord 120
-# (IBB); channel256_q27*: the translating-sphere channel on the D3Q27 lattice.
+# (IBB); channel256_q27*: the translating-sphere channel on the D3Q27 lattice;
+# refcase_cyl_ibb_q27: the reference's case with q = 27 in its input file.
 WORKLOADS = {
     "channel256": dict(case="channel", nx=256, ny=128, nz=128, storage="fp32"),
     "channel256_fp16s": dict(case="channel", nx=256, ny=128, nz=128, storage="fp16s"),
@@ -66,6 +67,7 @@ WORKLOADS = {
     "refcase_cyl_ibb": dict(case="refparams", shape="CYL", bc="IBB", nx=1201, ny=1201, nz=3, storage="fp32"),
     "refcase_cyl_sbb": dict(case="refparams", shape="CYL", bc="SBB", nx=1201, ny=1201, nz=3, storage="fp32"),
     "refcase_naca_ibb": dict(case="refparams", shape="AIR", bc="IBB", nx=1201, ny=1201, nz=3, storage="fp32"),
+    "refcase_cyl_ibb_q27": dict(case="refparams", shape="CYL", bc="IBB", nx=1201, ny=1201, nz=3, storage="fp32", q=27),
 }
 for _w in WORKLOADS.values():
     _w["dtype"] = _DT[_w["storage"]]
@@ -85,13 +87,13 @@ def make_case(w):
     if w["case"] == "refparams":
         from lbm3d_bb_amd.params import LBMParams, config_from_params
 
-        p = LBMParams(Nx=w["nx"], Ny=w["ny"], Nz=w["nz"], h=1.0, R=16.16, x_c=600.5, y_c=600.5, theta0=0.0, q=19,
+        p = LBMParams(Nx=w["nx"], Ny=w["ny"], Nz=w["nz"], h=1.0, R=16.16, x_c=600.5, y_c=600.5, theta0=0.0, q=w["q"],
                       Re=185.0, U_inf=0.075, V_inf=0.0, W_inf=0.0, rho_inf=1.0, dt=1.0, Lambda=0.1875,
                       n_max=40000, n_save=10000, bc=w["bc"], coll="BGK", shape=w["shape"], naca_code="0012",
                       chord=120.0)
         cfg, laws = config_from_params(p, storage=w["storage"])
         body = "cylinder R=16.16" if w["shape"] == "CYL" else "NACA 0012 airfoil, chord 120"
-        desc = (f"{cfg.nx}x{cfg.ny}x{cfg.nz} D3Q19 BGK, the reference's input/params.txt case: {body} heaving "
+        desc = (f"{cfg.nx}x{cfg.ny}x{cfg.nz} D3Q{w['q']} BGK, the reference's input/params.txt case: {body} heaving "
                 f"with the reference's law, {w['bc']} links, {w['storage']} populations, per-step MEA force")
         return cfg, laws, None, desc
     if w["case"] == "channel":
