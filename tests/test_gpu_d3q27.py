@@ -321,3 +321,44 @@ def test_d3q27_ibb_moving_airfoil_and_spinning_ellipsoid_fp32(wall_point):
     ro, uo = o.macros()
     assert rel_err(rg, ro) < TOL["fp32"]
     assert rel_err(ug, uo) < TOL["fp32"] * 10
+
+
+# --- full size -----------------------------------------------------------------------
+
+def test_d3q27_periodic_box_64_bgk_sphere_full_size_fp64():
+    """64^3 periodic box, BGK tau = 0.8, FP64, body force 1e-6 along x, fixed sphere
+    R = 8 at the centre, 300 steps: the drag every 10 steps and the final state
+    against the FP64 oracle."""
+    cfg = _q27(periodic_box(n=64, collision="BGK", storage="fp64", tau=0.8, g=(1e-6, 0.0, 0.0)))
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies([sphere((32.0, 32.0, 32.0), 8.0)])
+        s.init_equilibrium()
+    Fg, Fo = g.step(300), o.step(300)
+    assert np.abs(Fo[9::10, 0, 0]).min() > 0
+    assert force_err(Fg[9::10], Fo[9::10], _scale27(o)) < TOL["fp64"]
+    _compare_state(g, o, TOL["fp64"], "box64")
+
+
+def test_d3q27_channel256_full_size_fp32_against_fp64():
+    """The 256x128x128 D3Q27 channel with the translating sphere R = 12 at its full
+    size, where the oracle is too slow: the FP32 run keeps the FP64 run's flags and
+    links bit for bit, its drag to FP32 rounding, and repeats bitwise."""
+    import bench
+
+    runs = {}
+    for storage in ("fp64", "fp32", "fp32"):
+        w = dict(bench.WORKLOADS["channel256_q27"], storage=storage)
+        cfg, laws, _, _ = bench.make_case(w)
+        s = Solver(cfg)
+        s.set_motion(laws)
+        s.init_equilibrium()
+        F = s.step(150)
+        runs.setdefault(storage, []).append((F, s.flags(), s.links()))
+        s.close()
+    (F64, fl64, (c64, d64)), = runs["fp64"]
+    (F32, fl32, (c32, d32)), (F32b, _, _) = runs["fp32"]
+    assert np.array_equal(fl64, fl32) and np.array_equal(c64, c32) and np.array_equal(d64, d32)
+    assert np.array_equal(F32, F32b)
+    drag64, drag32 = F64[-50:, 0, 0], F32[-50:, 0, 0]
+    assert np.abs(drag32 - drag64).max() < 1e-4 * np.abs(drag64).max(), np.abs(drag32 - drag64).max()
