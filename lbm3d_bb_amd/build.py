@@ -19,8 +19,8 @@ REPO = os.path.dirname(PKG_DIR)
 
 SOURCES = ["lbm_capi.hip", "lbm_geom.hip", "lbm_store_f64.hip", "lbm_store_f32.hip", "lbm_store_f16s.hip",
            "lbm_store_bf16s.hip", "lbm_store_f64_ibb.hip", "lbm_store_f32_ibb.hip", "lbm_store_f16s_ibb.hip",
-           "lbm_store_bf16s_ibb.hip", "lbm_q27.hip", "lbm_q27_ibb.hip",
-           "unet_capi.hip"]
+           "lbm_store_bf16s_ibb.hip", "lbm_q27.hip", "lbm_q27_ibb.hip", "lbm_q27_16.hip",
+           "lbm_q27_16_ibb.hip", "unet_capi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -1014,8 +1014,6 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     if (cfg->solid_bc != LBM_SOLID_SBB && cfg->solid_bc != LBM_SOLID_IBB) return LBM_ERR_ARG;
     const int q = cfg->q == 0 ? 19 : cfg->q;
     if (q != 19 && q != 27) return LBM_ERR_ARG;
-    // D3Q27: FP64 / FP32 storage
-    if (q == 27 && cfg->storage > LBM_FP32) return LBM_ERR_ARG;
     const int L[3] = {cfg->nx, cfg->ny, cfg->nz};
     for (int a = 0; a < 3; ++a) {
         const int k0 = cfg->faces[2 * a].kind, k1 = cfg->faces[2 * a + 1].kind;
@@ -1025,8 +1023,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     }
     const int64_t N = (int64_t)cfg->nx * cfg->ny * cfg->nz;
     if (N >= ((int64_t)1 << 31)) return LBM_ERR_ARG;  // cell indices are int32 in the lists
-    // 16-bit storage: a bulk thread owns an aligned pair of x cells
-    if ((cfg->storage == LBM_FP16S || cfg->storage == LBM_BF16S) && (cfg->nx % 2) != 0) return LBM_ERR_ARG;
+    // 16-bit storage on D3Q19: a bulk thread owns an aligned pair of x cells
+    if (q == 19 && (cfg->storage == LBM_FP16S || cfg->storage == LBM_BF16S) && (cfg->nx % 2) != 0) return LBM_ERR_ARG;
     if (!(cfg->omega_plus > 0.0 && cfg->omega_plus < 2.0)) return LBM_ERR_ARG;
     if (cfg->collision == LBM_TRT && !(cfg->omega_minus > 0.0 && cfg->omega_minus < 2.0)) return LBM_ERR_ARG;
 

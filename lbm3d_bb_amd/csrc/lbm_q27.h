@@ -13,7 +13,7 @@ constexpr int kQ = 27;
 // of the start state (MODE_COLLIDE_ONLY).
 struct StepArgs {
     int mode;
-    int storage;  // LBM_FP64 or LBM_FP32
+    int storage;  // lbm_storage
     int trt;
     int forced;
     int ibb;  // links: 1 interpolated bounce-back, 0 simple
@@ -40,6 +40,12 @@ struct StepArgs {
 
 int q27_launch_step(const q27::StepArgs& a);
 int q27_launch_step_ibb(const q27::StepArgs& a);  // lbm_q27_ibb.hip
+// 16-bit storage (shifted FP16 / BF16 with the D3Q27 weights): lbm_q27_16.hip, lbm_q27_16_ibb.hip
+int q27_launch_step16(const q27::StepArgs& a);
+int q27_launch_step16_ibb(const q27::StepArgs& a);
+int q27_launch_equilibrium16(int storage, const Grid& g, hipStream_t st, void* f, const double* rho, const double* u,
+                             double rho0, const double u0[3]);
+int q27_launch_from_double16(int storage, const Grid& g, hipStream_t st, void* f, const double* src);
 // out[s][k] = sum over the blocks b within reach of a body of
 // partials[s * stride + b * n6 + k], s < nsteps; recs + s * rec_bytes holds
 // step s's bodies (nb of the new pose, then nb of the old one)

@@ -30,29 +30,6 @@ __global__ __launch_bounds__(kBlock) void k27_force_sum(const double* __restrict
     if (threadIdx.x == 0) out[(int64_t)blockIdx.y * n6 + k] = s[0];
 }
 
-// initialize_f (src/Initialization.jl:37-42)
-template <class S>
-__global__ __launch_bounds__(kBlock) void k27_equilibrium(typename S::T* __restrict__ f, const double* __restrict__ rho,
-                                                          const double* __restrict__ u, double rho0, double u0x,
-                                                          double u0y, double u0z, int64_t N, int64_t P, Shift sh) {
-    using C = typename S::C;
-    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (cell >= N) return;
-    const C r = rho ? C(rho[cell]) : C(rho0);
-    const C ux = u ? C(u[cell]) : C(u0x), uy = u ? C(u[N + cell]) : C(u0y), uz = u ? C(u[2 * N + cell]) : C(u0z);
-#pragma unroll
-    for (int d = 0; d < QQ; ++d) S::store(f + (int64_t)d * P + cell, d, feq27<C>(d, r, ux, uy, uz), sh);
-}
-
-template <class S>
-__global__ __launch_bounds__(kBlock) void k27_from_double(typename S::T* __restrict__ f, const double* __restrict__ src,
-                                                          int64_t N, int64_t P, Shift sh) {
-    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (cell >= N) return;
-#pragma unroll
-    for (int d = 0; d < QQ; ++d) S::store(f + (int64_t)d * P + cell, d, (typename S::C)src[(int64_t)d * N + cell], sh);
-}
-
 // Solid flags (flag_of, the mask of k_flags_full) of the cells of one box per
 // body (blockIdx.y): where a moved body's flags can change.
 __global__ __launch_bounds__(kBlock) void k27_flags_boxes(uint8_t* __restrict__ flags, const DevBody* __restrict__ bodies,
@@ -130,6 +107,7 @@ __global__ __launch_bounds__(kBlock) void k27_link_fill(const uint8_t* __restric
 }  // namespace q27
 
 int q27_launch_step(const q27::StepArgs& a) {
+    if (a.storage == LBM_FP16S || a.storage == LBM_BF16S) return a.ibb ? q27_launch_step16_ibb(a) : q27_launch_step16(a);
     if (a.ibb) return q27_launch_step_ibb(a);
     if (a.storage == LBM_FP64) q27::launch_storage<StoreF64, false>(a);
     else q27::launch_storage<StoreF32, false>(a);
@@ -145,6 +123,7 @@ int q27_launch_force_sum(const double* partials, int64_t nblocks, int64_t stride
 
 int q27_launch_equilibrium(int storage, const Grid& g, hipStream_t st, void* f, const double* rho, const double* u,
                            double rho0, const double u0[3]) {
+    if (storage == LBM_FP16S || storage == LBM_BF16S) return q27_launch_equilibrium16(storage, g, st, f, rho, u, rho0, u0);
     const dim3 grid((unsigned)nblocks_of(g.N)), block(kBlock);
     if (storage == LBM_FP64)
         hipLaunchKernelGGL(q27::k27_equilibrium<StoreF64>, grid, block, 0, st, (double*)f, rho, u, rho0, u0[0], u0[1],
@@ -156,6 +135,7 @@ int q27_launch_equilibrium(int storage, const Grid& g, hipStream_t st, void* f, 
 }
 
 int q27_launch_from_double(int storage, const Grid& g, hipStream_t st, void* f, const double* src) {
+    if (storage == LBM_FP16S || storage == LBM_BF16S) return q27_launch_from_double16(storage, g, st, f, src);
     const dim3 grid((unsigned)nblocks_of(g.N)), block(kBlock);
     if (storage == LBM_FP64)
         hipLaunchKernelGGL(q27::k27_from_double<StoreF64>, grid, block, 0, st, (double*)f, src, g.N, g.P, g.sh);

@@ -12,7 +12,8 @@
 // pulls its 27 populations and collides.  A cell with a face rule or a solid
 // neighbour also reads its own 27 populations for the previous-step moments,
 // and its rule directions load the cell's own f*[opp(d)] in place of the pull
-// (the address is selected, the load is unconditional).  FP64 and FP32.
+// (the address is selected, the load is unconditional).  FP64, FP32 and
+// shifted 16-bit storage.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,6 +42,32 @@ __device__ inline C w27(int d) {
     const int n = (kX[d] != 0) + (kY[d] != 0) + (kZ[d] != 0);
     return n == 0 ? C(8) / C(27) : (n == 1 ? C(2) / C(27) : (n == 2 ? C(1) / C(54) : C(1) / C(216)));
 }
+
+// 16-bit population storage on D3Q27: the deviation from the rest state
+// f_d - w_d (rho0 = 1) with this lattice's weights, times 2^12 in FP16 (RNE) or
+// as is in BF16 (RNE); FP32 arithmetic.
+struct StoreF16S27 {
+    using T = __half;
+    using C = float;
+    __device__ static inline C load(const T* p, int d, const Shift& sh) {
+        return __fmaf_rn(__half2float(*p), sh.ik, w27<float>(d));
+    }
+    __device__ static inline void store(T* p, int d, C v, const Shift& sh) {
+        const __half h = __float2half_rn((v - w27<float>(d)) * sh.k);
+        __builtin_nontemporal_store(__half_as_ushort(h), reinterpret_cast<unsigned short*>(p));
+    }
+};
+
+struct StoreBF16S27 {
+    using T = uint16_t;
+    using C = float;
+    __device__ static inline C load(const T* p, int d, const Shift&) {
+        return StoreBF16S::bf16_to_f32(*p) + w27<float>(d);
+    }
+    __device__ static inline void store(T* p, int d, C v, const Shift&) {
+        __builtin_nontemporal_store(StoreBF16S::f32_to_bf16_rne(v - w27<float>(d)), p);
+    }
+};
 
 template <typename C>
 __device__ inline void moments27(const C f[QQ], C& rho, C& jx, C& jy, C& jz) {
@@ -693,6 +720,29 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
             partials[(int64_t)blockIdx.x * n6 + i] = s;
         }
     }
+}
+
+// initialize_f (src/Initialization.jl:37-42)
+template <class S>
+__global__ __launch_bounds__(kBlock) void k27_equilibrium(typename S::T* __restrict__ f, const double* __restrict__ rho,
+                                                          const double* __restrict__ u, double rho0, double u0x,
+                                                          double u0y, double u0z, int64_t N, int64_t P, Shift sh) {
+    using C = typename S::C;
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= N) return;
+    const C r = rho ? C(rho[cell]) : C(rho0);
+    const C ux = u ? C(u[cell]) : C(u0x), uy = u ? C(u[N + cell]) : C(u0y), uz = u ? C(u[2 * N + cell]) : C(u0z);
+#pragma unroll
+    for (int d = 0; d < QQ; ++d) S::store(f + (int64_t)d * P + cell, d, feq27<C>(d, r, ux, uy, uz), sh);
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k27_from_double(typename S::T* __restrict__ f, const double* __restrict__ src,
+                                                          int64_t N, int64_t P, Shift sh) {
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= N) return;
+#pragma unroll
+    for (int d = 0; d < QQ; ++d) S::store(f + (int64_t)d * P + cell, d, (typename S::C)src[(int64_t)d * N + cell], sh);
 }
 
 template <class S, int MODE, bool IBB>

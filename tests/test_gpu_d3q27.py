@@ -362,3 +362,72 @@ def test_d3q27_channel256_full_size_fp32_against_fp64():
     assert np.array_equal(F32, F32b)
     drag64, drag32 = F64[-50:, 0, 0], F32[-50:, 0, 0]
     assert np.abs(drag32 - drag64).max() < 1e-4 * np.abs(drag64).max(), np.abs(drag32 - drag64).max()
+
+
+# --- 16-bit population storage -----------------------------------------------------------
+
+def _mean_drag(F, last=100):
+    return np.asarray(F)[-last:, :, 0].mean(axis=0)
+
+
+@pytest.mark.parametrize("storage,tol", [("fp16s", 5e-3), ("bf16s", 5e-2)])
+def test_d3q27_shifted_storage_drag_vs_fp64(storage, tol):
+    """Periodic box, body force, fixed sphere: the 16-bit storages' drag against the
+    FP64 oracle (the same bars as on D3Q19)."""
+    cfg64 = _q27(periodic_box(n=24, collision="TRT", storage="fp64", g=(1e-5, 0, 0)))
+    cfgs = _q27(periodic_box(n=24, collision="TRT", storage=storage, g=(1e-5, 0, 0)))
+    o = OracleSolver(cfg64, "fp64")
+    g = Solver(cfgs)
+    for s in (g, o):
+        s.set_bodies([sphere((12.0, 12.0, 12.0), 4.0)])
+        s.init_equilibrium()
+    Fg, Fo = g.step(200), o.step(200)
+    err = abs(Fg[-1, 0, 0] - Fo[-1, 0, 0]) / abs(Fo[-1, 0, 0])
+    print(storage, "drag error vs FP64", err)
+    assert err < tol, err
+
+
+def test_d3q27_shifted_storage_channel_drag_vs_fp64():
+    """Channel (inlet, zero-gradient outlet, no-slip walls) with a translating
+    sphere: mean drag of the last 100 of 1000 steps, FP16S and BF16S against the
+    FP64 oracle; the simple and the interpolated bounce-back."""
+    law = ConstantMotion("sphere", (20.0, 16.0, 16.0), (5.0,), vel=(0.01, 0.0, 0.0))
+    for bc in ("SBB", "IBB"):
+        cfg = _q27(channel(nx=64, ny=32, nz=32, storage="fp64", tau=0.6))
+        cfg.solid_bc = bc
+        o = OracleSolver(cfg, "fp64")
+        o.set_motion([law])
+        o.init_equilibrium()
+        ref = _mean_drag(o.step(1000))
+        o.close()
+        for storage, tol in (("fp16s", 5e-3), ("bf16s", 5e-2)):
+            cfg = _q27(channel(nx=64, ny=32, nz=32, storage=storage, tau=0.6))
+            cfg.solid_bc = bc
+            g = Solver(cfg)
+            g.set_motion([law])
+            g.init_equilibrium()
+            d = _mean_drag(g.step(1000))
+            g.close()
+            err = float(abs(d[0] - ref[0]) / abs(ref[0]))
+            print(bc, storage, "channel drag error vs FP64", err)
+            assert err < tol, (bc, storage, err)
+
+
+def test_d3q27_shifted_storage_start_state():
+    """The shifted formats hold f - w with the D3Q27 weights: a uniform start
+    reads back as f_eq to the format's resolution, the rest state as the FP32
+    weights exactly (a zero deviation)."""
+    # half an FP16 ulp of a deviation near 7e-3 scaled by 2^12 is 1.9e-6; half a BF16 ulp 1.5e-5
+    for storage, tol in (("fp16s", 4e-6), ("bf16s", 5e-5)):
+        cfg = _q27(periodic_box(n=8, collision="TRT", storage=storage, g=(0.0, 0.0, 0.0)))
+        cfg.u0 = (0.0, 0.0, 0.0)
+        g = Solver(cfg)
+        g.init_equilibrium()
+        w32 = W27.astype(np.float32).astype(np.float64)
+        assert np.array_equal(g.populations(), np.repeat(w32[:, None], cfg.ncells, axis=1))
+        g.close()
+        cfg.u0 = (0.03, -0.02, 0.01)
+        g = Solver(cfg)
+        g.init_equilibrium()
+        assert np.abs(g.populations() - _feq27(cfg.u0)[:, None]).max() < tol
+        g.close()
