@@ -60,6 +60,8 @@ WORKLOADS = {
     "channel256_bf16s": dict(case="channel", nx=256, ny=128, nz=128, storage="bf16s"),
     "channel256_q27": dict(case="channel", nx=256, ny=128, nz=128, storage="fp32", q=27),
     "channel256_q27_fp64": dict(case="channel", nx=256, ny=128, nz=128, storage="fp64", q=27),
+    "channel256_q27_fp16s": dict(case="channel", nx=256, ny=128, nz=128, storage="fp16s", q=27),
+    "channel256_q27_bf16s": dict(case="channel", nx=256, ny=128, nz=128, storage="bf16s", q=27),
     "box512_spheres64": dict(case="box", nx=512, ny=256, nz=256, storage="fp32"),
     "box512_spheres64_fp16s": dict(case="box", nx=512, ny=256, nz=256, storage="fp16s"),
     "box512_spheres64_fp64": dict(case="box", nx=512, ny=256, nz=256, storage="fp64"),
