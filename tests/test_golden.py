@@ -33,9 +33,10 @@ def _load(name):
 
 
 def _scale(gold):
-    from lbm3d_bb_amd.lattice import W
+    from lbm3d_bb_amd.lattice import W, W27
 
-    return float(2.0 * W[gold["link_dir"].astype(np.int64)].sum())
+    w = W27 if gold["f"].shape[0] == 27 else W
+    return float(2.0 * w[gold["link_dir"].astype(np.int64)].sum())
 
 
 def _check(s, gold, steps, tol):
