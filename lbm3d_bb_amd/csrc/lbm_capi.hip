@@ -53,6 +53,7 @@ struct lbm_ctx {
     // (inside the device step records) for the getters' replay
     uint8_t* flags_old = nullptr;
     std::vector<DevBody> q27_pose_cur, q27_pose_old;  // the pose whose flags each buffer holds
+    std::vector<std::array<float, 3>> q27_reach_cur, q27_reach_old;  // and the bodies' reach then
     bool q27_cur_ok = false, q27_old_ok = false;
     const DevBody* q27_bnew = nullptr;
     const DevBody* q27_bold = nullptr;
@@ -584,6 +585,7 @@ int full_geometry(lbm_ctx* c, const std::vector<DevBody>& bodies) {
     c->geom_valid = true;
     if (c->q == 27) {  // the D3Q27 step finds its links from the flags
         c->q27_pose_cur = bodies;
+        c->q27_reach_cur = c->reach;
         c->q27_cur_ok = true;
         return LBM_OK;
     }
@@ -788,13 +790,16 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
             // its own (older) pose, which differ from the new pose's only inside
             // each body's box around both poses; the step record carries the boxes.
             std::swap(c->q27_pose_cur, c->q27_pose_old);
+            std::swap(c->q27_reach_cur, c->q27_reach_old);
             std::swap(c->q27_cur_ok, c->q27_old_ok);
-            if (c->q27_cur_ok && (int)c->q27_pose_cur.size() == nb) {
+            if (c->q27_cur_ok && (int)c->q27_pose_cur.size() == nb && (int)c->q27_reach_cur.size() == nb &&
+                (int)c->reach.size() == nb) {
                 int32_t* bx = (int32_t*)(c->h_rec[par] + (size_t)s * c->rec_bytes + 2 * (size_t)nb * sizeof(DevBody));
                 for (int i = 0; i < nb; ++i) {
                     int64_t vol = 1;
                     for (int a = 0; a < 3; ++a) {
-                        const float r = c->reach[i][a];
+                        // the larger reach of the two poses (lbm_set_bodies may change a shape)
+                        const float r = std::max(c->reach[i][a], c->q27_reach_cur[i][a]);
                         const float lo_f = std::min(c->q27_pose_cur[i].c[a], bnew[i].c[a]) - r - 2.0f;
                         const float hi_f = std::max(c->q27_pose_cur[i].c[a], bnew[i].c[a]) + r + 2.0f;
                         int lo = 0, ext = L[a];
@@ -818,6 +823,7 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
                 flag_boxes[(size_t)s] = 1;
             }
             c->q27_pose_cur = bnew;
+            c->q27_reach_cur = c->reach;
             c->q27_cur_ok = true;
         }
         c->step_bodies = bnew;

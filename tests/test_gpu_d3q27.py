@@ -431,3 +431,23 @@ def test_d3q27_shifted_storage_start_state():
         g.init_equilibrium()
         assert np.abs(g.populations() - _feq27(cfg.u0)[:, None]).max() < tol
         g.close()
+
+
+def test_d3q27_shape_change_between_steps_fp64():
+    """lbm_set_bodies between steps with a smaller shape at a shifted pose: the
+    flags of the larger old shape are cleared (the box rebuild takes the larger
+    reach of the two poses)."""
+    cfg = _q27(periodic_box(n=24, collision="TRT", g=(1e-5, 0.0, 0.0)))
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies([sphere((12.0, 12.0, 12.0), 9.0)])
+        s.init_equilibrium()
+        s.step(3)
+        s.set_bodies([sphere((12.4, 12.0, 12.0), 2.5)])
+        s.step(3)
+        # the flags buffer swapped in holds the R = 9 pose, far beyond the new reach
+        s.set_bodies([sphere((12.8, 11.7, 12.0), 2.0)])
+    Fg, Fo = g.step(4), o.step(4)
+    assert np.array_equal(g.flags(), o.flags())
+    assert force_err(Fg, Fo, _scale27(o)) < TOL["fp64"], force_err(Fg, Fo, _scale27(o))
+    _compare_state(g, o, TOL["fp64"], "shape change")
