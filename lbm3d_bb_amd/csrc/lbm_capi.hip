@@ -78,6 +78,7 @@ struct lbm_ctx {
     bool pending_set = false;
     bool flags_only = false;      // geometry from lbm_set_flags (no analytic bodies)
     std::vector<DevBody> mask_bodies;  // pose that flags reflect
+    std::vector<std::array<float, 3>> mask_reach;  // the bodies' reach in that pose
     std::vector<DevBody> step_bodies;  // pose + kinematics of the current step
     std::vector<std::array<float, 3>> reach;  // bounding half-extents per body and axis
     bool have_flags_box = false;       // lbm_set_flags: bounding box of the solid cells
@@ -582,6 +583,7 @@ int full_geometry(lbm_ctx* c, const std::vector<DevBody>& bodies) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dev_free(c, d_b, std::max<size_t>(bodies.size(), 1) * sizeof(DevBody));
     c->mask_bodies = bodies;
+    c->mask_reach = c->reach;
     c->geom_valid = true;
     if (c->q == 27) {  // the D3Q27 step finds its links from the flags
         c->q27_pose_cur = bodies;
@@ -655,11 +657,13 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
     if (moving) {
         sg.mode = 1;
         bold = c->mask_bodies;
-        // body regions: union of the old and new bounding boxes, dilated
+        // body regions: union of the old and new bounding boxes, dilated by the
+        // larger reach of the two poses (lbm_set_bodies may change a shape)
+        const bool old_reach = (int)c->mask_reach.size() == c->nb;
         for (int i = 0; i < c->nb; ++i) {
             float lo_f[3], hi_f[3];
             for (int a = 0; a < 3; ++a) {
-                const float rr = c->reach[i][a];
+                const float rr = old_reach ? std::max(c->reach[i][a], c->mask_reach[i][a]) : c->reach[i][a];
                 lo_f[a] = std::min(bnew[i].c[a], bold[i].c[a]) - rr;
                 hi_f[a] = std::max(bnew[i].c[a], bold[i].c[a]) + rr;
             }
@@ -668,6 +672,7 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
         finish_boxes(c, sg);
         sg.list_blocks = 0;
         c->mask_bodies = bnew;
+        c->mask_reach = c->reach;
         c->list_valid = false;
     } else {
         sg.mode = 0;
@@ -786,6 +791,7 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
         }
         if (moving) {
             c->mask_bodies = bnew;
+            c->mask_reach = c->reach;
             // The buffers swap: the one that becomes current holds the flags of
             // its own (older) pose, which differ from the new pose's only inside
             // each body's box around both poses; the step record carries the boxes.
@@ -1270,6 +1276,7 @@ int lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
     unknown.reach = -1.0f;  // no analytic shape: every face cell checks its neighbours' flags
     c->step_bodies.assign(maxf, unknown);
     c->mask_bodies = c->step_bodies;
+    c->mask_reach = c->reach;
     HIPCHK(c, hipMemcpyAsync(c->flags, flags, (size_t)c->g.N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->geom_valid = true;

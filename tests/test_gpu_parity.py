@@ -291,3 +291,20 @@ def test_periodic_box_64_bgk_sphere_full_size_fp64():
     err = force_err(Fg[9::10], Fo[9::10], force_scale(o))
     assert err < TOL["fp64"], err
     _compare_state(g, o, TOL["fp64"], "box64")
+
+
+def test_shape_change_between_steps_fp64():
+    """lbm_set_bodies between steps with a smaller shape at a shifted pose: the
+    cells of the larger old shape turn fluid (the body region spans the larger
+    reach of the two poses), also once the body stops moving."""
+    cfg = periodic_box(n=24, collision="TRT", g=(1e-5, 0.0, 0.0))
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies([sphere((12.0, 12.0, 12.0), 9.0)])
+        s.init_equilibrium()
+        s.step(3)
+        s.set_bodies([sphere((12.8, 11.7, 12.0), 2.0)])
+    Fg, Fo = g.step(4), o.step(4)
+    assert np.array_equal(g.flags(), o.flags())
+    assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
+    _compare_state(g, o, TOL["fp64"], "shape change")
