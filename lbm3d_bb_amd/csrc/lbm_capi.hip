@@ -643,6 +643,12 @@ int prepare_step(lbm_ctx* c, int64_t n, char* rec, int* grid, int* face_grid, bo
     else bnew = c->step_bodies;
     std::vector<DevBody> bold;
     bool moving = false;
+    if (c->nb == 0 && !c->flags_only && !c->geom_valid) {
+        // no bodies (any more): clear the flags once
+        if (!allow_sync) return kStopChunk;
+        int rc = full_geometry(c, bnew);
+        if (rc) return rc;
+    }
     if (c->nb > 0 && !c->flags_only) {
         if (!c->geom_valid) {
             if (!allow_sync) return kStopChunk;
@@ -775,7 +781,7 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
             bnew = c->step_bodies;
         } else {
             bodies_for_step(c, c->step + 1 + s, bnew);
-            if (nb > 0 && !c->geom_valid) {
+            if (!c->geom_valid) {  // (with no bodies: the flags are cleared once)
                 if (s > 0) break;  // the full build runs before any launch of the chunk
                 int rc = full_geometry(c, bnew);
                 if (rc) return rc;

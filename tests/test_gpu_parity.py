@@ -308,3 +308,22 @@ def test_shape_change_between_steps_fp64():
     assert np.array_equal(g.flags(), o.flags())
     assert force_err(Fg, Fo, force_scale(o)) < TOL["fp64"], force_err(Fg, Fo, force_scale(o))
     _compare_state(g, o, TOL["fp64"], "shape change")
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_bodies_removed_between_steps_fp64(q):
+    """lbm_set_bodies with no bodies after steps with one: the flags clear and the
+    step continues as the oracle's (no links, no force rows)."""
+    cfg = periodic_box(n=16, collision="TRT", g=(1e-5, 0.0, 0.0))
+    cfg.q = q
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.set_bodies([sphere((8.0, 8.0, 8.0), 3.0)])
+        s.init_equilibrium()
+        s.step(3)
+        s.set_bodies([])
+        s.step(3)
+    assert not g.flags().any() and not o.flags().any()
+    rg, ug = g.macros()
+    ro, uo = o.macros()
+    assert rel_err(rg, ro) < TOL["fp64"] and np.abs(ug - uo).max() < TOL["fp64"]
