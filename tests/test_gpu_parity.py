@@ -327,3 +327,29 @@ def test_bodies_removed_between_steps_fp64(q):
     rg, ug = g.macros()
     ro, uo = o.macros()
     assert rel_err(rg, ro) < TOL["fp64"] and np.abs(ug - uo).max() < TOL["fp64"]
+
+
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("dims,faces", [
+    ((2, 3, 5), "pvz"),   # the smallest periodic axis (2 cells) and the smallest walled one (3)
+    ((3, 4, 3), "vav"),   # every axis non-periodic at its minimum size
+    ((5, 2, 2), "zpp"),
+])
+def test_tiny_domains_fp64(q, dims, faces):
+    """Lattices at the minimum sizes lbm_create accepts, with mixed faces per axis
+    (p periodic, v velocity walls, a velocity inlet + anti-bounce-back outlet,
+    z velocity inlet + zero-gradient outlet): HIP path against the oracle."""
+    from lbm3d_bb_amd.solver import Config, periodic, pressure, velocity, zero_gradient
+
+    fs = []
+    for k in faces:
+        fs += {"p": [periodic(), periodic()], "v": [velocity((0.02, -0.01, 0.0)), velocity((0.0, 0.0, 0.0))],
+               "a": [velocity((0.03, 0.0, 0.01)), pressure()], "z": [velocity((0.02, 0.0, 0.0)), zero_gradient()]}[k]
+    cfg = Config(*dims, collision="TRT", storage="fp64", omega_plus=1 / 0.7, omega_minus=1 / 1.1, faces=fs,
+                 u0=(0.01, 0.0, 0.0), q=q)
+    g, o = _pair(cfg, "fp64")
+    for s in (g, o):
+        s.init_equilibrium()
+    g.step(7)
+    o.step(7)
+    _compare_state(g, o, TOL["fp64"], f"tiny {dims} {faces}")
