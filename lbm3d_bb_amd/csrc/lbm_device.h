@@ -1195,7 +1195,6 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
                              ForceAcc& acc, double* lds_overflow, const double* lds_face_u, int pre_self,
                              const LinkCodes pre, float* lds_q = nullptr) {
     using C = typename S::C;
-    const int64_t N = g.N;
     const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
     const int L[3] = {g.nx, g.ny, g.nz};
     const int q0[3] = {x, y, z};
