@@ -188,6 +188,9 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-launch HIP events (no roofline)")
+    ap.add_argument("--start", default="uniform", choices=["uniform", "net"],
+                    help="net: start from the 128^3 U-Net applied tile-wise (untimed, before the warm-up)")
+    ap.add_argument("--params", default=None, help="U-Net parameters (.npy) for --start net; random init if absent")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
                     help="after the run, ten device-to-device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -207,6 +210,9 @@ def main() -> int:
     cfg, laws, bodies, desc = make_case(w)
     s = Solver(cfg)
     setup(s, laws, bodies)
+    start = {"start": "uniform", "reason": "uniform start asked"}
+    if args.start == "net":
+        start = net_start(s, cfg, laws, bodies, args.params)
     if args.warmup:
         s.step(args.warmup)
     s.sync()
@@ -244,7 +250,7 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": w["dtype"],
-        "data": "synthetic (uniform start, prescribed body motion; no dataset)",
+        "data": f"synthetic ({start['start']} start, prescribed body motion; no dataset)",
         "config": {
             "workload": f"{args.workload}: {desc}",
             "cells": N,
@@ -269,12 +275,39 @@ def main() -> int:
             "traffic": traffic,
         },
     }
+    if args.start == "net":
+        out["start"] = start
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def net_start(s, cfg, laws, bodies, params) -> dict:
+    """The 4-level 16/32/64/128-channel U-Net on 128^3 tiles (halo 16) from
+    the solver's solid flags; Re = |u0| D / nu with D the smallest diameter of
+    the first body.  Returns the start taken and the inference time."""
+    from lbm3d_bb_amd.warmstart import UNet, warm_start_tiled
+
+    u_ref = float(np.linalg.norm(cfg.u0))
+    if u_ref == 0.0:
+        raise SystemExit("--start net needs a workload with an inflow velocity")
+    semi = (laws[0] if laws is not None else bodies[0]).semi
+    nu = (1.0 / cfg.omega_plus - 0.5) / 3.0
+    re = u_ref * 2.0 * min(semi) / nu
+    tile = tuple(min(128, n) for n in (cfg.nz, cfg.ny, cfg.nx))
+    net = UNet(tile, batch=2, levels=4, base=16, seed=0)
+    try:
+        if params:
+            net.load(params)
+        info = warm_start_tiled(s, net, re, u_ref, halo=16)
+    finally:
+        net.close()
+    info.update(re=round(re, 3), tile=list(tile), halo=16, params=params or "random init")
+    info["seconds"] = round(info["seconds"], 3)
+    return info
 
 
 def copy_probe(mib: int) -> None:

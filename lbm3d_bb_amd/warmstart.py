@@ -3,7 +3,9 @@
 ``UNet`` wraps the C ABI (training and inference run on this library's BF16
 MFMA kernels).  ``warm_start(solver, net, mask, re, ...)`` predicts (rho, u) and
 hands it to ``lbm_init_equilibrium``; the solver's own drag residual
-(``DragMonitor``) still decides convergence.  The reference has no learned
+(``DragMonitor``) still decides convergence.  ``warm_start_tiled`` does the
+same on lattices larger than the network by running it on overlapping tiles
+and keeping each tile's core.  The reference has no learned
 component: it always starts from a uniform equilibrium
 (src/Initialization.jl:57-83).
 """
@@ -248,6 +250,85 @@ def predict(net: UNet, case) -> tuple[np.ndarray, np.ndarray]:
     solid = mask.reshape(-1)
     u[:, solid] = 0.0
     return rho, u
+
+
+def tile_spans(n: int, t: int, halo: int) -> list[tuple[int, int, int]]:
+    """Tiles of t cells covering an axis of n cells, as (start, lo, hi): the
+    tile reads [start, start + t) and keeps its core [lo, hi).  The cores
+    partition [0, n) in order; a core stays `halo` cells away from its tile's
+    ends except where that end is the end of the lattice, and the tiles at the
+    two ends are moved inward so that every tile lies inside the lattice."""
+    if t > n:
+        raise ValueError(f"tile of {t} cells is longer than the lattice axis of {n}")
+    if halo < 0 or 2 * halo >= t:
+        raise ValueError(f"halo {halo} leaves no core in a tile of {t} cells")
+    out = []
+    lo = 0
+    while lo < n:
+        start = min(max(lo - halo, 0), n - t)
+        hi = n if start + t >= n else start + t - halo
+        out.append((start, lo, hi))
+        lo = hi
+    return out
+
+
+def predict_tiled(net, mask: np.ndarray, re: float, u_ref: float, halo: int = 16) -> tuple[np.ndarray, np.ndarray]:
+    """(rho, u) of a lattice larger than the network: the network (fixed
+    input shape ``net.shape`` = (tz, ty, tx), at most ``net.batch`` tiles per
+    call) runs on overlapping tiles placed by tile_spans along each axis, and
+    each tile contributes only its core, so that every kept cell sees at least
+    `halo` cells of context on each side inside the lattice.  mask is the
+    (nz, ny, nx) solid indicator; u is (3, N) with solid cells at rest."""
+    mask = np.asarray(mask) != 0
+    nz, ny, nx = mask.shape
+    tz, ty, tx = net.shape
+    spans = [tile_spans(n, t, halo) for n, t in zip((nz, ny, nx), (tz, ty, tx))]
+    tiles = [(a, b, c) for a in spans[0] for b in spans[1] for c in spans[2]]
+    rho = np.empty((nz, ny, nx), dtype=np.float64)
+    u = np.empty((3, nz, ny, nx), dtype=np.float64)
+    for i in range(0, len(tiles), net.batch):
+        group = tiles[i:i + net.batch]
+        x = np.stack([encode_input(mask[sz:sz + tz, sy:sy + ty, sx:sx + tx], re)
+                      for (sz, _, _), (sy, _, _), (sx, _, _) in group])
+        y = net.forward(x)
+        for b, ((sz, lz, hz), (sy, ly, hy), (sx, lx, hx)) in enumerate(group):
+            core = y[b][:, lz - sz:hz - sz, ly - sy:hy - sy, lx - sx:hx - sx].astype(np.float64)
+            rho[lz:hz, ly:hy, lx:hx] = 1.0 + u_ref * core[0]
+            u[:, lz:hz, ly:hy, lx:hx] = u_ref * core[1:4]
+    rho = rho.reshape(-1)
+    u = u.reshape(3, -1)
+    u[:, mask.reshape(-1)] = 0.0
+    return rho, u
+
+
+def solid_mask(solver) -> np.ndarray:
+    """(nz, ny, nx) solid indicator of a solver's current flags (0 = fluid)."""
+    c = solver.config
+    return (solver.flags() != 0).reshape(c.nz, c.ny, c.nx)
+
+
+def warm_start_tiled(solver, net, re: float, u_ref: float, halo: int = 16) -> dict:
+    """Initialises ``solver`` (any lattice, D3Q19 or D3Q27) at the equilibrium
+    of the tile-wise prediction of the network from the solver's own solid
+    flags, or at the uniform state when the prediction is not plausible.  The
+    bodies must be set first.  Returns which start was taken, why, and the
+    inference time."""
+    import time
+
+    if net is None:
+        solver.init_equilibrium()
+        return {"start": "uniform", "reason": "no network", "tiles": 0, "seconds": 0.0}
+    t0 = time.perf_counter()
+    mask = solid_mask(solver)
+    rho, u = predict_tiled(net, mask, re, u_ref, halo)
+    seconds = time.perf_counter() - t0
+    ntiles = int(np.prod([len(tile_spans(n, t, halo)) for n, t in zip(mask.shape, net.shape)]))
+    ok, why = plausible(rho, u, u_ref)
+    if ok:
+        solver.init_equilibrium(rho, u)
+        return {"start": "net", "reason": why, "tiles": ntiles, "seconds": seconds}
+    solver.init_equilibrium()
+    return {"start": "uniform", "reason": why, "tiles": ntiles, "seconds": seconds}
 
 
 def warm_start(solver, net: UNet | None, case) -> dict:
