@@ -142,6 +142,13 @@ int conv(lbm_unet* n, int geom, const Buf& x, int xco, int cin, const bf16* w, i
     a.in = in;
     a.out = out;
     a.m = (int64_t)rows.n * rows.d * rows.h * rows.w;
+    // k_conv indexes voxels in 32 bits and takes a same-size 3x3x3 convolution's rows on the input grid
+    const int64_t nin = (int64_t)in.n * in.d * in.h * in.w;
+    if (nin >= (int64_t)1 << 31 || a.m >= (int64_t)1 << 31) return fail(n, -2, "convolution tensor of 2^31 voxels or more");
+    if (geom == G_CONV3 && (rows.n != in.n || rows.d != in.d || rows.h != in.h || rows.w != in.w))
+        return fail(n, -2, "3x3x3 convolution rows and input differ in size");
+    if (geom != G_UP2 && (rows.n != out.n || rows.d != out.d || rows.h != out.h || rows.w != out.w))
+        return fail(n, -2, "convolution rows and output differ in size");
     const int nt = std::min(coutp / 16, 4);
     const dim3 grid(nblk(a.m), (unsigned)(coutp / (16 * nt)), (unsigned)npar), block(kThreads);
 #define LAUNCH_CONV(G, NTV) hipLaunchKernelGGL((k_conv<G, NTV>), grid, block, 0, n->stream, a)
@@ -380,6 +387,8 @@ int lbm_unet_create(const lbm_unet_config* cfg, lbm_unet** out) {
         return -1;
     const int f = 1 << (cfg->levels - 1);
     if (cfg->nx % f || cfg->ny % f || cfg->nz % f || cfg->nx < f || cfg->ny < f || cfg->nz < f) return -1;
+    // the kernels index voxels (not elements) in 32 bits
+    if ((int64_t)cfg->batch * cfg->nx * cfg->ny * cfg->nz >= (int64_t)1 << 31) return -1;
     lbm_unet* n = new lbm_unet();
     n->cfg = *cfg;
     n->L = cfg->levels;

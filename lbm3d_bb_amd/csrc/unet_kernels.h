@@ -30,13 +30,18 @@ __device__ inline int64_t vox(const Dims& s, int n, int z, int y, int x) {
     return (((int64_t)n * s.d + z) * s.h + y) * s.w + x;
 }
 
+// Voxel v of a tensor as (n, z, y, x).  Every tensor of a network holds
+// fewer than 2^31 voxels (checked when the network is created), so the
+// divisions are 32-bit ones: a 64-bit division has no hardware instruction.
 __device__ inline void unvox(const Dims& s, int64_t v, int& n, int& z, int& y, int& x) {
-    x = (int)(v % s.w);
-    int64_t t = v / s.w;
-    y = (int)(t % s.h);
-    t /= s.h;
-    z = (int)(t % s.d);
-    n = (int)(t / s.d);
+    const uint32_t u = (uint32_t)v;
+    const uint32_t t1 = u / (uint32_t)s.w;
+    x = (int)(u - t1 * (uint32_t)s.w);
+    const uint32_t t2 = t1 / (uint32_t)s.h;
+    y = (int)(t1 - t2 * (uint32_t)s.h);
+    const uint32_t t3 = t2 / (uint32_t)s.d;
+    z = (int)(t2 - t3 * (uint32_t)s.d);
+    n = (int)t3;
 }
 
 struct ConvArgs {
@@ -81,8 +86,22 @@ __device__ inline bf16x8 zero8() {
     return z;
 }
 
+// Input-voxel offset of tap t from the row's base voxel (see k_conv).
+template <int GEOM>
+__device__ inline int tap_offset(const Dims& in, int t) {
+    if (GEOM == G_CONV3) return (t / 9 - 1) * in.h * in.w + ((t / 3) % 3 - 1) * in.w + (t % 3 - 1);
+    if (GEOM == G_DOWN2) return (t >> 2) * in.h * in.w + ((t >> 1) & 1) * in.w + (t & 1);
+    return 0;
+}
+
 // One wave computes 64 rows x 16*NT output channels; a block is 4 waves
-// (256 rows).  grid = (ceil(m/256), coutp/(16*NT), npar).
+// (256 rows).  grid = (ceil(m/256), coutp/(16*NT), npar).  Each row keeps its
+// base input voxel and a bit mask of the taps that fall inside the input
+// (the 27 taps of a same-size 3x3x3 convolution, whose rows and input share
+// their dims; every tap otherwise), so the reduction loop does no division
+// and no bounds test: tap t of a row reads voxel base + tap_offset(t) when bit
+// t is set and zero padding when it is not.  Voxel indices are 32-bit (the
+// host checks that every tensor has fewer than 2^31 voxels).
 template <int GEOM, int NT>
 __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -92,14 +111,29 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
     const int64_t row0 = (int64_t)blockIdx.x * kThreads + wave * 64;
     const bf16* wt = a.wt + (int64_t)par * a.coutp * a.kpad;
 
-    // rows of this lane in each of the 4 row tiles
-    int rn[4], rz[4], ry[4], rx[4];
-    bool rok[4];
+    int vb[4];
+    uint32_t tmask[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         const int64_t r = row0 + mt * 16 + r16;
-        rok[mt] = r < a.m;
-        unvox(a.rows, rok[mt] ? r : 0, rn[mt], rz[mt], ry[mt], rx[mt]);
+        const bool ok = r < a.m;
+        int n, z, y, x;
+        unvox(a.rows, ok ? r : 0, n, z, y, x);
+        if (GEOM == G_CONV3) {
+            vb[mt] = (int)vox(a.in, n, z, y, x);
+            const uint32_t xb = (x > 0 ? 1u : 0u) | 2u | (x + 1 < a.in.w ? 4u : 0u);
+            const uint32_t yb = (y > 0 ? 1u : 0u) | 2u | (y + 1 < a.in.h ? 4u : 0u);
+            const uint32_t zb = (z > 0 ? 1u : 0u) | 2u | (z + 1 < a.in.d ? 4u : 0u);
+            const uint32_t xy = ((yb & 1u) ? xb : 0u) | ((yb & 2u) ? xb << 3 : 0u) | ((yb & 4u) ? xb << 6 : 0u);
+            const uint32_t m = ((zb & 1u) ? xy : 0u) | ((zb & 2u) ? xy << 9 : 0u) | ((zb & 4u) ? xy << 18 : 0u);
+            tmask[mt] = ok ? m : 0u;
+        } else if (GEOM == G_DOWN2) {
+            vb[mt] = (int)vox(a.in, n, 2 * z, 2 * y, 2 * x);
+            tmask[mt] = ok ? 0xffu : 0u;
+        } else {
+            vb[mt] = (int)vox(a.in, n, z, y, x);
+            tmask[mt] = ok ? 1u : 0u;
+        }
     }
     f32x4 acc[4][NT];
 #pragma unroll
@@ -107,18 +141,19 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const int groups_per_tap = a.cin >> 3;
-    const int ngroups = a.ntaps * groups_per_tap;
+    // this lane's 8-channel group G = k0/8 + g as (tap t, group cg within the tap)
+    const int gpt = a.cin >> 3;
+    int t = g / gpt, cg = g - (g / gpt) * gpt;
+    const bf16* xb = a.x + a.xco;
     for (int k0 = 0; k0 < a.kpad; k0 += 32) {
-        const int G = (k0 >> 3) + g;  // this lane's 8-channel group
         bf16x8 A[4];
-        if (G < ngroups) {
-            const int t = G / groups_per_tap, c0 = (G - t * groups_per_tap) * 8;
+        if (t < a.ntaps) {
+            const int off = tap_offset<GEOM>(a.in, t);
+            const int c0 = cg * 8;
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
-                int64_t v;
-                if (rok[mt] && in_voxel<GEOM>(a, rn[mt], rz[mt], ry[mt], rx[mt], t, v))
-                    A[mt] = load8(a.x + v * a.xcs + a.xco + c0);
+                if ((tmask[mt] >> t) & 1u)
+                    A[mt] = load8(xb + (int64_t)(vb[mt] + off) * a.xcs + c0);
                 else
                     A[mt] = zero8();
             }
@@ -134,6 +169,11 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
                 acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B[nt], acc[mt][nt], 0, 0, 0);
+        cg += 4;
+        while (cg >= gpt) {
+            cg -= gpt;
+            ++t;
+        }
     }
     // epilogue: lane holds rows 4g..4g+3 of column r16 of each tile
 #pragma unroll
@@ -142,11 +182,13 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
         for (int i = 0; i < 4; ++i) {
             const int64_t r = row0 + mt * 16 + 4 * g + i;
             if (r >= a.m) continue;
-            int n, z, y, x;
-            unvox(a.rows, r, n, z, y, x);
-            int64_t ov;
-            if (GEOM == G_UP2) ov = vox(a.out, n, 2 * z + (par >> 2), 2 * y + ((par >> 1) & 1), 2 * x + (par & 1));
-            else ov = vox(a.out, n, z, y, x);
+            // rows enumerate the output grid except for UP2, whose rows are coarse voxels
+            int64_t ov = r;
+            if (GEOM == G_UP2) {
+                int n, z, y, x;
+                unvox(a.rows, r, n, z, y, x);
+                ov = vox(a.out, n, 2 * z + (par >> 2), 2 * y + ((par >> 1) & 1), 2 * x + (par & 1));
+            }
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const int co = co_base + nt * 16 + r16;
