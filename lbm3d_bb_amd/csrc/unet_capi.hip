@@ -53,6 +53,7 @@ struct lbm_unet {
     float* d_in = nullptr;  // FP32 staging of host x
     float* d_tgt = nullptr;
     float* d_part = nullptr;
+    float* d_bpart = nullptr;  // bias-gradient partials [kBiasParts][largest cout]
     int64_t part_n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -187,22 +188,37 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     a.in = in;
     a.out = out;
     a.m = (int64_t)rows.n * rows.d * rows.h * rows.w;
-    a.chunk = std::max<int64_t>(512, rup(a.m / 512 + 1, 32));
-    a.nci_tiles = (L.cinp + 63) / 64;
-    const int nco_tiles = (L.coutp + 63) / 64;
+    a.chunk = std::max<int64_t>(512, rup(a.m / 512 + 1, 128));
+    // channel tile 16*N: the smallest that holds both channel counts, at most 64
+    const int cmax = std::max(L.cinp, L.coutp);
+    const int N = cmax <= 16 ? 1 : (cmax <= 32 ? 2 : 4);
+    const int TC = 16 * N;
+    a.nci_tiles = (L.cinp + TC - 1) / TC;
+    const int nco_tiles = (L.coutp + TC - 1) / TC;
     const unsigned gy = L.geom == G_UP2 ? 8 : (unsigned)L.ntaps;
     const dim3 grid((unsigned)((a.m + a.chunk - 1) / a.chunk), gy, (unsigned)(nco_tiles * a.nci_tiles));
+#define LAUNCH_WGRAD(G)                                                                          \
+    if (N == 1) hipLaunchKernelGGL((k_wgrad<G, 1>), grid, dim3(kThreads), 0, n->stream, a);    \
+    else if (N == 2) hipLaunchKernelGGL((k_wgrad<G, 2>), grid, dim3(kThreads), 0, n->stream, a); \
+    else hipLaunchKernelGGL((k_wgrad<G, 4>), grid, dim3(kThreads), 0, n->stream, a);
     switch (L.geom) {
-        case G_CONV3: hipLaunchKernelGGL(k_wgrad<G_CONV3>, grid, dim3(kThreads), 0, n->stream, a); break;
-        case G_UP2: hipLaunchKernelGGL(k_wgrad<G_UP2>, grid, dim3(kThreads), 0, n->stream, a); break;
-        default: hipLaunchKernelGGL(k_wgrad<G_PW>, grid, dim3(kThreads), 0, n->stream, a); break;
+        case G_CONV3: LAUNCH_WGRAD(G_CONV3); break;
+        case G_UP2: LAUNCH_WGRAD(G_UP2); break;
+        default: LAUNCH_WGRAD(G_PW); break;
     }
+#undef LAUNCH_WGRAD
     UCHK(n, hipGetLastError());
     const int64_t ng = (int64_t)L.npar * L.cout * L.ntaps * L.cin;
     hipLaunchKernelGGL(k_unpack_grad, dim3(nblk(ng)), dim3(kThreads), 0, n->stream, n->dwp[li], L, n->d_grad);
     const int64_t nv = (int64_t)out.n * out.d * out.h * out.w;
-    hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)L.cout), dim3(kThreads), 0, n->stream, dy.p, dy.cs, dco, nv, L.b_off,
-                       n->d_grad);
+    int cp = 1;
+    while (cp < L.cout && cp < kThreads) cp <<= 1;
+    const int64_t nb = std::min<int64_t>(kBiasParts, (nv + kThreads - 1) / kThreads);
+    const int64_t chunk = (nv + nb - 1) / nb;
+    hipLaunchKernelGGL(k_bias_part, dim3((unsigned)nb, (unsigned)((L.cout + cp - 1) / cp)), dim3(kThreads), 0,
+                       n->stream, dy.p, dy.cs, dco, L.cout, cp, nv, chunk, n->d_bpart);
+    hipLaunchKernelGGL(k_bias_sum, dim3((unsigned)L.cout), dim3(kThreads), 0, n->stream, n->d_bpart, (int)nb, L.cout,
+                       L.b_off, n->d_grad);
     UCHK(n, hipGetLastError());
     return 0;
 }
@@ -471,6 +487,9 @@ int lbm_unet_create(const lbm_unet_config* cfg, lbm_unet** out) {
     rc = rc ? rc : alloc(n, &n->d_tgt, N0b * cfg->cout);
     n->part_n = (N0b * 16 + kThreads - 1) / kThreads;
     rc = rc ? rc : alloc(n, &n->d_part, n->part_n);
+    int maxc = 0;
+    for (const LayerDesc& L : n->layers) maxc = std::max(maxc, L.cout);
+    rc = rc ? rc : alloc(n, &n->d_bpart, (int64_t)kBiasParts * maxc);
     if (rc) {
         lbm_unet_destroy(n);
         return rc;

@@ -205,9 +205,13 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
 }
 
 // Weight gradient: dw[par][co][t*cin + ci] += sum_rows dy[out(row)][co] * x[in(row,t)][ci].
-// Block: one tap (blockIdx.y), a 16*NCO x 16*NCI channel tile (blockIdx.z), a
-// chunk of rows (blockIdx.x).  32 rows at a time are staged in LDS transposed
-// (channel-major) so each lane reads 8 consecutive rows with one 16-byte load.
+// Block: one tap (blockIdx.y), a TC x TC channel tile with TC = 16*N
+// (blockIdx.z), a chunk of rows (blockIdx.x).  The 4 waves are N ci sub-tiles
+// times 4/N row slabs of 32 rows; each step stages R = 128/N rows of TC dy and
+// TC x channels in LDS transposed (channel-major, one 16-byte load per thread
+// and tensor), so a lane reads 8 consecutive rows of a channel with one
+// 16-byte LDS load.  N is the smallest of 1, 2, 4 with 16*N >= the channel
+// counts, so the 16-channel layers do not run a mostly empty 64x64 tile.
 struct WgradArgs {
     const bf16* x;
     int xcs, xco, cin;     // cin = padded input channels (multiple of 8)
@@ -218,16 +222,17 @@ struct WgradArgs {
     Dims rows, in, out;
     int64_t m;
     int64_t chunk;         // rows per block
-    int nci_tiles;         // ceil(cin / 64)
+    int nci_tiles;         // ceil(cin / TC)
 };
 
-template <int GEOM>
+template <int GEOM, int N>
 __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
-    constexpr int TC = 64;  // channel tile (co and ci)
-    __shared__ __attribute__((aligned(16))) bf16 s_dy[TC][40];
-    __shared__ __attribute__((aligned(16))) bf16 s_x[TC][40];
+    constexpr int TC = 16 * N, R = 128 / N, RP = R + 8;
+    __shared__ __attribute__((aligned(16))) bf16 s_dy[TC][RP];
+    __shared__ __attribute__((aligned(16))) bf16 s_x[TC][RP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r16 = lane & 15;
+    const int cis = wave % N, slab = wave / N;
     const int t = GEOM == G_UP2 ? 0 : blockIdx.y;
     const int par = GEOM == G_UP2 ? blockIdx.y : 0;
     const int co_tile = blockIdx.z / a.nci_tiles, ci_tile = blockIdx.z % a.nci_tiles;
@@ -235,17 +240,16 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
     const int64_t rbeg = (int64_t)blockIdx.x * a.chunk;
     const int64_t rend = rbeg + a.chunk < a.m ? rbeg + a.chunk : a.m;
 
-    // 16 sub-tiles (4 co x 4 ci) of 16x16; wave w owns ci sub-tile w for all 4 co
-    f32x4 acc[4];
+    f32x4 acc[N];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < N; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     ConvArgs ca;
     ca.in = a.in;
-    for (int64_t r0 = rbeg; r0 < rend; r0 += 32) {
-        // stage: thread handles row (tid & 31) and 8 channels (tid >> 5) * 8
+    // staging role: row rr of the step, channels cc..cc+7 of the tile
+    const int rr = threadIdx.x % R, cc = (threadIdx.x / R) * 8;
+    for (int64_t r0 = rbeg; r0 < rend; r0 += R) {
         {
-            const int rr = threadIdx.x & 31, cc = (threadIdx.x >> 5) * 8;
             const int64_t r = r0 + rr;
             bf16x8 vdy = zero8(), vx = zero8();
             if (r < rend) {
@@ -265,20 +269,20 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
             }
         }
         __syncthreads();
-        const bf16x8 B = load8(&s_x[wave * 16 + r16][g * 8]);
+        const bf16x8 B = load8(&s_x[cis * 16 + r16][slab * 32 + g * 8]);
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-            const bf16x8 A = load8(&s_dy[ct * 16 + r16][g * 8]);
+        for (int ct = 0; ct < N; ++ct) {
+            const bf16x8 A = load8(&s_dy[ct * 16 + r16][slab * 32 + g * 8]);
             acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[ct], 0, 0, 0);
         }
         __syncthreads();
     }
-    // acc[ct]: rows co = co0 + ct*16 + 4g + i, column ci = ci0 + wave*16 + r16
-    const int ci = ci0 + wave * 16 + r16;
+    // acc[ct]: rows co = co0 + ct*16 + 4g + i, column ci = ci0 + cis*16 + r16
+    const int ci = ci0 + cis * 16 + r16;
     if (ci >= a.cin) return;
     float* dw = a.dw + (int64_t)par * a.coutp * a.kpad;
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
+    for (int ct = 0; ct < N; ++ct)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int co = co0 + ct * 16 + 4 * g + i;
@@ -422,7 +426,7 @@ __global__ __launch_bounds__(kThreads) void k_pack_dgrad(const float* __restrict
 }
 
 // padded FP32 weight gradient -> master layout, plus the bias gradient computed
-// separately (k_bias_grad).
+// separately (k_bias_part, k_bias_sum).
 __global__ __launch_bounds__(kThreads) void k_unpack_grad(const float* __restrict__ dwp, LayerDesc L,
                                                            float* __restrict__ grad) {
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -436,13 +440,40 @@ __global__ __launch_bounds__(kThreads) void k_unpack_grad(const float* __restric
     grad[L.w_off + i] = dwp[((int64_t)par * L.coutp + co) * L.kpad + t * L.cinp + ci];
 }
 
-// bias gradient: sum over voxels of dy[v][co]; one block per channel, fixed order.
-__global__ __launch_bounds__(kThreads) void k_bias_grad(const bf16* __restrict__ dy, int dcs, int dco, int64_t nvox,
-                                                         int64_t b_off, float* __restrict__ grad) {
+// Bias gradient, sum over voxels of dy[v][co], in two fixed-order passes.
+// Pass 1: block (b, cb) sums its range of `chunk` voxels for channels
+// cb*cp .. cb*cp + cp - 1 (cp a power of two <= kThreads): thread = (channel
+// c = tid % cp, phase tid / cp), so a wave reads whole channel runs of
+// consecutive voxels; the phases are then summed by a tree in LDS.
+constexpr int kBiasParts = 512;
+
+__global__ __launch_bounds__(kThreads) void k_bias_part(const bf16* __restrict__ dy, int dcs, int dco, int cout,
+                                                         int cp, int64_t nvox, int64_t chunk,
+                                                         float* __restrict__ part) {
+    __shared__ float s[kThreads];
+    const int nph = kThreads / cp, ph = threadIdx.x / cp;
+    const int c = blockIdx.y * cp + (int)(threadIdx.x % cp);
+    const int64_t v0 = (int64_t)blockIdx.x * chunk;
+    const int64_t v1 = v0 + chunk < nvox ? v0 + chunk : nvox;
+    float a = 0.f;
+    if (c < cout)
+        for (int64_t v = v0 + ph; v < v1; v += nph) a += (float)dy[v * dcs + dco + c];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = nph / 2; off > 0; off >>= 1) {
+        if (ph < off) s[threadIdx.x] += s[threadIdx.x + off * cp];
+        __syncthreads();
+    }
+    if (ph == 0 && c < cout) part[(int64_t)blockIdx.x * cout + c] = s[threadIdx.x];
+}
+
+// Pass 2: grad[b_off + co] = sum of the nb partials of channel co (one block per channel).
+__global__ __launch_bounds__(kThreads) void k_bias_sum(const float* __restrict__ part, int nb, int cout,
+                                                        int64_t b_off, float* __restrict__ grad) {
     __shared__ float s[kThreads];
     const int co = blockIdx.x;
     float a = 0.f;
-    for (int64_t v = threadIdx.x; v < nvox; v += kThreads) a += (float)dy[v * dcs + dco + co];
+    for (int b = threadIdx.x; b < nb; b += kThreads) a += part[(int64_t)b * cout + co];
     s[threadIdx.x] = a;
     __syncthreads();
     for (int off = kThreads / 2; off > 0; off >>= 1) {
