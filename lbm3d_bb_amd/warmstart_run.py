@@ -1,6 +1,7 @@
 """The learned warm-start workload end to end on one GPU.
 
   python -m lbm3d_bb_amd.warmstart_run --size 128 --train 2000 --eval 200 --out DIR
+  python -m lbm3d_bb_amd.warmstart_run --size 64 --batch 4 --timing 10   (network timing only)
 
 1. generates steady FP32 fields of seeded sphere packs with the solver from
    uniform starts (training seeds 0..train-1, held-out seeds from 1_000_000);
@@ -42,7 +43,12 @@ def main(argv=None) -> int:
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--out", default="warmstart_out")
     ap.add_argument("--params", default=None, help="start from these network parameters (.npy)")
+    ap.add_argument("--timing", type=int, default=0, metavar="K",
+                    help="only time K forward calls and K train steps of the network at --size / --batch on "
+                         "synthetic inputs (no data, no solver) and print one JSON line")
     args = ap.parse_args(argv)
+    if args.timing > 0:
+        return timing(args.size, args.batch, args.timing)
     os.makedirs(args.out, exist_ok=True)
 
     def log(msg):
@@ -81,6 +87,31 @@ def main(argv=None) -> int:
     with open(os.path.join(args.out, "summary.json"), "w", encoding="utf-8") as fh:
         json.dump(summary, fh, indent=1)
     log(json.dumps({k: v for k, v in summary.items() if k != "rows"}))
+    return 0
+
+
+def timing(L: int, batch: int, k: int) -> int:
+    """Device time (HIP events) of the network's forward and train step at
+    L^3 and the given batch, on a random mask and random targets; the best of
+    k calls after one untimed call of each."""
+    rng = np.random.default_rng(0)
+    shape = (L, L, L)
+    net = UNet(shape, batch=batch, levels=4, base=16, seed=0)
+    x = np.zeros((batch, 2) + shape, dtype=np.float32)
+    x[:, 0] = rng.random((batch,) + shape) < 0.2
+    x[:, 1] = 0.5
+    t = (0.1 * rng.standard_normal((batch, 4) + shape)).astype(np.float32)
+    out = {"workload": f"U-Net 4 levels 16/32/64/128 channels, {L}^3, batch {batch}", "calls": k}
+    for name, fn in (("forward", lambda: net.forward(x)), ("train_step", lambda: net.train_step(x, t))):
+        fn()
+        best, flops = float("inf"), 0.0
+        for _ in range(k):
+            fn()
+            ms, flops = net.last_timing()
+            best = min(best, ms)
+        out[name] = {"ms": round(best, 4), "gflop": round(flops / 1e9, 2), "tflops": round(flops / best / 1e9, 2)}
+    net.close()
+    print(json.dumps(out), flush=True)
     return 0
 
 
