@@ -12,6 +12,7 @@ component: it always starts from a uniform equilibrium
 from __future__ import annotations
 
 import ctypes as ct
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -313,8 +314,6 @@ def warm_start_tiled(solver, net, re: float, u_ref: float, halo: int = 16) -> di
     flags, or at the uniform state when the prediction is not plausible.  The
     bodies must be set first.  Returns which start was taken, why, and the
     inference time."""
-    import time
-
     if net is None:
         solver.init_equilibrium()
         return {"start": "uniform", "reason": "no network", "tiles": 0, "seconds": 0.0}
