@@ -277,6 +277,23 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
         }
         __syncthreads();
     }
+    // the row slabs of one ci sub-tile add their sums in LDS (fixed order), so
+    // only the first slab's wave issues the atomics
+    if (N < 4) {
+        __shared__ float s_acc[4][4 * N][64];
+#pragma unroll
+        for (int ct = 0; ct < N; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s_acc[wave][ct * 4 + i][lane] = acc[ct][i];
+        __syncthreads();
+        if (slab != 0) return;
+#pragma unroll
+        for (int sl = 1; sl < 4 / N; ++sl)
+#pragma unroll
+            for (int ct = 0; ct < N; ++ct)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[ct][i] += s_acc[sl * N + cis][ct * 4 + i][lane];
+    }
     // acc[ct]: rows co = co0 + ct*16 + 4g + i, column ci = ci0 + cis*16 + r16
     const int ci = ci0 + cis * 16 + r16;
     if (ci >= a.cin) return;
