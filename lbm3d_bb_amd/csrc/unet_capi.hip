@@ -150,7 +150,10 @@ int conv(lbm_unet* n, int geom, const Buf& x, int xco, int cin, const bf16* w, i
         return fail(n, -2, "3x3x3 convolution rows and input differ in size");
     if (geom != G_UP2 && (rows.n != out.n || rows.d != out.d || rows.h != out.h || rows.w != out.w))
         return fail(n, -2, "convolution rows and output differ in size");
-    const int nt = std::min(coutp / 16, 4);
+    // 16*nt output channels per wave; narrower where a wide tile leaves the chip short of blocks
+    // (the coarse levels have few rows)
+    int nt = std::min(coutp / 16, 4);
+    while (nt > 1 && (int64_t)nblk(a.m) * (coutp / (16 * nt)) * npar < 512) nt >>= 1;
     const dim3 grid(nblk(a.m), (unsigned)(coutp / (16 * nt)), (unsigned)npar), block(kThreads);
 #define LAUNCH_CONV(G, NTV) hipLaunchKernelGGL((k_conv<G, NTV>), grid, block, 0, n->stream, a)
 #define BY_NT(G)                       \
