@@ -45,7 +45,8 @@ typedef enum {
     LBM_ERR_HIP = -2,      /* a HIP runtime call failed                  */
     LBM_ERR_STATE = -3,    /* call out of order (e.g. step before init)  */
     LBM_ERR_CAPACITY = -4, /* caller buffer too small                    */
-    LBM_ERR_NUMERIC = -5   /* NaN or non-positive density detected       */
+    LBM_ERR_NUMERIC = -5   /* lbm_step: a body's force/torque record is  */
+                           /* not finite (NaN or Inf reached its links)  */
 } lbm_status;
 
 /* Collision operator (src/Collision.jl:50-69 "BGK" / "TRT"). */

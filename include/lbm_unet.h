@@ -32,7 +32,7 @@ typedef struct {
     int32_t cin;         /* input channels (<= 8), e.g. 2: solid mask, Re */
     int32_t cout;        /* output channels (<= 16), e.g. 4: rho, ux, uy, uz */
     int32_t base;        /* channels at level 0 (multiple of 16), e.g. 16 */
-    int32_t levels;      /* 2..4 */
+    int32_t levels;      /* 2..5 */
     uint64_t seed;       /* weight initialisation */
     float lr, beta1, beta2, eps;
 } lbm_unet_config;

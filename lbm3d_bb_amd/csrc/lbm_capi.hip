@@ -887,7 +887,8 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
     if (n6 > 0 && nprep > 0)
         GEOCHK(c, q27_launch_force_sum(c->d_partials, nblk, stride, n6, nprep, c->d_rec, c->rec_bytes, c->g,
                                        c->d_forces, c->stream));
-    if (force_out && n6 > 0) {
+    // the records always come back: lbm_step checks them for non-finite values
+    if (n6 > 0) {
         HIPCHK(c, hipMemcpyAsync(c->h_forces[par], c->d_forces, (size_t)nprep * n6 * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
     }
@@ -946,7 +947,8 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
     if (n6 > 0 && nprep > 0)
         GEOCHK(c, launch_force_finalize(c->d_partials, stride, c->d_rec, c->rec_bytes, nprep, n6, c->d_forces,
                                         c->stream));
-    if (force_out && n6 > 0) {
+    // the records always come back: lbm_step checks them for non-finite values
+    if (n6 > 0) {
         HIPCHK(c, hipMemcpyAsync(c->h_forces[par], c->d_forces, (size_t)nprep * n6 * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
     }
@@ -1415,15 +1417,28 @@ int lbm_step(lbm_ctx* c, int32_t nsteps, double* force_out) {
     const int n6 = c->nb * 6;
     int64_t chunk_off[2] = {-1, -1};  // first step (within this call) of the chunk held by each parity
     int chunk_cnt[2] = {0, 0};
+    int64_t bad_step = -1;  // first step whose force/torque record is not finite
+    int bad_body = -1;
+    // hands a finished chunk's records to the caller and checks them
+    auto take = [&](int par) {
+        const double* r = c->h_forces[par];
+        for (int64_t i = 0; i < (int64_t)chunk_cnt[par] * n6 && bad_step < 0; ++i)
+            if (!std::isfinite(r[i])) {
+                bad_step = c->step - done_total + chunk_off[par] + i / n6 + 1;
+                bad_body = (int)((i % n6) / 6);
+            }
+        if (force_out)
+            std::memcpy(force_out + (size_t)chunk_off[par] * n6, r, (size_t)chunk_cnt[par] * n6 * sizeof(double));
+        chunk_off[par] = -1;
+    };
     int rc = [&]() -> int {
         while (done_total < nsteps) {
             const int par = c->parity;
             if (chunk_off[par] >= 0) {
                 HIPCHK(c, hipEventSynchronize(c->rec_ev[par]));
-                if (force_out && n6 > 0)
-                    std::memcpy(force_out + (size_t)chunk_off[par] * n6, c->h_forces[par],
-                                (size_t)chunk_cnt[par] * n6 * sizeof(double));
+                if (n6 > 0) take(par);
                 chunk_off[par] = -1;
+                if (bad_step >= 0) break;  // the state is no longer finite
             }
             const int k = (int)std::min<int64_t>(kChunk, nsteps - done_total);
             int64_t done = 0;
@@ -1447,11 +1462,11 @@ int lbm_step(lbm_ctx* c, int32_t nsteps, double* force_out) {
         }
         c->ev_used = 0;
     }
-    for (int par = 0; par < 2; ++par) {
-        if (chunk_off[par] >= 0 && force_out && n6 > 0)
-            std::memcpy(force_out + (size_t)chunk_off[par] * n6, c->h_forces[par],
-                        (size_t)chunk_cnt[par] * n6 * sizeof(double));
-    }
+    for (int par = 0; par < 2; ++par)
+        if (chunk_off[par] >= 0 && n6 > 0) take(par);
+    if (bad_step >= 0)
+        return fail(c, LBM_ERR_NUMERIC, "non-finite force or torque of body " + std::to_string(bad_body) +
+                                            " at step " + std::to_string(bad_step));
     return LBM_OK;
 }
 

@@ -1009,9 +1009,12 @@ __global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Force accumulation: per thread one body in registers (a fluid node next to a
-// second body accumulates that one through LDS atomics), per wave a
-// deterministic segmented shuffle reduction, per block a fixed-order sum.
+// Force accumulation: per thread one body in registers; a link of a fluid node
+// to a second body goes straight into its wave's LDS slots, one lane at a time
+// in lane order (wave_ordered_add); per wave a deterministic segmented shuffle
+// reduction into the same slots; per block a fixed-order sum over the waves.
+// Every addition happens in an order fixed by the cell layout alone, so the
+// force of a step is bitwise reproducible.
 // ---------------------------------------------------------------------------
 struct ForceAcc {
     int b0;
@@ -1022,6 +1025,22 @@ __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
+}
+
+// Adds v[6] of every active lane with `on` set to lds_wave[key * 6 ...], lane
+// by lane in ascending lane order.  The slots belong to this wave alone and a
+// wave's LDS operations complete in program order, so the sums are fixed.
+__device__ __forceinline__ void wave_ordered_add(bool on, int key, const double v[6], double* lds_wave) {
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(on);
+    while (pending) {
+        const int l = __ffsll((unsigned long long)pending) - 1;
+        if (lane == l) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) atomicAdd(&lds_wave[key * 6 + c], v[c]);
+        }
+        pending &= pending - 1;
+    }
 }
 
 __device__ inline void wave_reduce_to_lds(int key, const double vals[6], double* lds_wave /*[nb][6]*/) {
@@ -1192,7 +1211,7 @@ template <class S, int COLL, bool FORCED, int MODE, bool IBB = false>
 __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                              double* __restrict__ gout, double* __restrict__ rho_out, double* __restrict__ u_out,
                              const Grid& g, const Phys<typename S::C>& p, const SolidView& sv, int x, int y, int z,
-                             ForceAcc& acc, double* lds_overflow, const double* lds_face_u, int pre_self,
+                             ForceAcc& acc, double* lds_wave, const double* lds_face_u, int pre_self,
                              const LinkCodes pre, float* lds_q = nullptr) {
     using C = typename S::C;
     const int64_t cell = (int64_t)x + (int64_t)g.nx * y + g.nxy * z;
@@ -1436,12 +1455,12 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
                 }
                 const double v6[6] = {P[0], P[1], P[2], ra[1] * P[2] - ra[2] * P[1], ra[2] * P[0] - ra[0] * P[2],
                                       ra[0] * P[1] - ra[1] * P[0]};
-                if (acc.b0 < 0 || acc.b0 == aux) {
+                const bool first = acc.b0 < 0 || acc.b0 == aux;
+                if (first) {
                     acc.b0 = aux;
                     for (int c = 0; c < 6; ++c) acc.a0[c] += v6[c];
-                } else {
-                    for (int c = 0; c < 6; ++c) atomicAdd(&lds_overflow[aux * 6 + c], v6[c]);
                 }
+                wave_ordered_add(!first, aux, v6, lds_wave);
             }
         }
         // the interpolated rule's extra loads and FP64 sums: one direction at a time
@@ -1478,12 +1497,12 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
     const int nb = sg.nbodies;
     const int n6 = nb * 6;
     const int tid = threadIdx.x;
-    // LDS: per wave nbodies x 6 partial sums, then the cross-body overflow sums
-    double* s_over = lds + kWaves * n6;
-    for (int i = tid; i < (kWaves + 1) * n6; i += kBlock) lds[i] = 0.0;
+    // LDS: per wave nbodies x 6 partial sums
+    double* s_wave = lds + (tid >> 6) * n6;
+    for (int i = tid; i < kWaves * n6; i += kBlock) lds[i] = 0.0;
     // then the bodies (new poses, then old), when few enough
     const bool staged = nb <= kLdsBodies;
-    DevBody* s_bodies = reinterpret_cast<DevBody*>(lds + (kWaves + 1) * n6);
+    DevBody* s_bodies = reinterpret_cast<DevBody*>(lds + kWaves * n6);
     // then the faces' wall / inlet velocities (6 x 3), then, with the interpolated
     // bounce-back, the wall fractions (18 per thread)
     double* s_face_u = reinterpret_cast<double*>(reinterpret_cast<char*>(s_bodies) +
@@ -1613,16 +1632,15 @@ __device__ __forceinline__ void fixup_block(const int blk, const typename S::T* 
         if (plain)
             plain_cell<S, COLL, FORCED, MODE>(fin, fout, gout, rho_out, u_out, g, p, x, y, z);
         else
-            general_cell<S, COLL, FORCED, MODE, IBB>(fin, fout, gout, rho_out, u_out, g, p, sv, x, y, z, acc, s_over,
+            general_cell<S, COLL, FORCED, MODE, IBB>(fin, fout, gout, rho_out, u_out, g, p, sv, x, y, z, acc, s_wave,
                                                      s_face_u, pre_self, pre, s_q);
     }
     if (MODE == MODE_STEP && nb > 0) {
-        const int wave = tid >> 6;
-        wave_reduce_to_lds(acc.b0, acc.a0, lds + wave * n6);
+        wave_reduce_to_lds(acc.b0, acc.a0, s_wave);
         __syncthreads();
         for (int i = tid; i < n6; i += kBlock) {
-            double s = s_over[i];
-            for (int w = 0; w < kWaves; ++w) s += lds[w * n6 + i];
+            double s = lds[i];
+            for (int w = 1; w < kWaves; ++w) s += lds[w * n6 + i];
             partials[(int64_t)blk * n6 + i] = s;
         }
     }
@@ -1653,7 +1671,7 @@ inline BulkShape bulk_shape(const Grid& g) {
 // dynamic LDS of a step launch: the fix-up blocks' force partials, staged
 // bodies, face velocities and (interpolated bounce-back) wall fractions
 inline size_t step_lds_bytes(int nbodies, bool ibb = false) {
-    size_t b = (size_t)(kWaves + 1) * (size_t)nbodies * 6 * sizeof(double);
+    size_t b = (size_t)kWaves * (size_t)nbodies * 6 * sizeof(double);
     if (nbodies <= kLdsBodies) b += 2 * (size_t)nbodies * sizeof(DevBody);
     b += 18 * sizeof(double);
     if (ibb) b += 18 * (size_t)kBlock * sizeof(float);

@@ -408,14 +408,14 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
     using C = typename S::C;
     extern __shared__ double lds_dyn[];
     const int n6 = nb * 6;
-    double* s_over = lds_dyn + kWaves * n6;  // links of a second body in one thread
+    double* s_wave = lds_dyn + (threadIdx.x >> 6) * n6;  // this wave's force slots
     // IBB: the wall fraction of each link direction, one column per thread
-    float* lds_q = reinterpret_cast<float*>(lds_dyn + (kWaves + 1) * n6) + threadIdx.x;
+    float* lds_q = reinterpret_cast<float*>(lds_dyn + kWaves * n6) + threadIdx.x;
     // Only blocks within reach of a body can hold links: they alone reduce
     // forces and write their partials (the sum visits the same blocks).
     const bool bnear = MODE != MODE_COLLIDE_ONLY && MODE != MODE_GET_RAW && block_near(blockIdx.x, bnew, bold, nb, g);
     if (MODE == MODE_STEP && bnear) {
-        for (int i = threadIdx.x; i < (kWaves + 1) * n6; i += kBlock) lds_dyn[i] = 0.0;
+        for (int i = threadIdx.x; i < kWaves * n6; i += kBlock) lds_dyn[i] = 0.0;
         __syncthreads();
     }
     ForceAcc acc;
@@ -670,14 +670,13 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                                     ra[a] = wrap_arm((double)q0[a] + 0.5 + ha * (double)cdl[a] - ch[a], a, g);
                                 const double v6[6] = {P[0], P[1], P[2], ra[1] * P[2] - ra[2] * P[1],
                                                       ra[2] * P[0] - ra[0] * P[2], ra[0] * P[1] - ra[1] * P[0]};
-                                if (acc.b0 < 0 || acc.b0 == bi) {
+                                const bool first = acc.b0 < 0 || acc.b0 == bi;
+                                if (first) {
                                     acc.b0 = bi;
 #pragma unroll
                                     for (int c = 0; c < 6; ++c) acc.a0[c] += v6[c];
-                                } else {
-#pragma unroll
-                                    for (int c = 0; c < 6; ++c) atomicAdd(&s_over[bi * 6 + c], v6[c]);
                                 }
+                                wave_ordered_add(!first, bi, v6, s_wave);
                             }
                         }
                     }
@@ -711,12 +710,11 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
         }
     }
     if (MODE == MODE_STEP && bnear) {
-        const int wave = threadIdx.x >> 6;
-        wave_reduce_to_lds(acc.b0, acc.a0, lds_dyn + wave * n6);
+        wave_reduce_to_lds(acc.b0, acc.a0, s_wave);
         __syncthreads();
         for (int i = threadIdx.x; i < n6; i += kBlock) {
-            double s = s_over[i];
-            for (int w = 0; w < kWaves; ++w) s += lds_dyn[w * n6 + i];
+            double s = lds_dyn[i];
+            for (int w = 1; w < kWaves; ++w) s += lds_dyn[w * n6 + i];
             partials[(int64_t)blockIdx.x * n6 + i] = s;
         }
     }
@@ -754,8 +752,8 @@ static void launch_mode(const StepArgs& a) {
     p.wm = (C)a.wm;
     for (int k = 0; k < 3; ++k) p.g[k] = (C)a.grav[k];
     const dim3 grid((unsigned)nblocks_of(a.g.N)), block(kBlock);
-    size_t lds = MODE == MODE_STEP ? (size_t)(kWaves + 1) * a.nb * 6 * sizeof(double) : 0;
-    if (IBB) lds = (size_t)(kWaves + 1) * a.nb * 6 * sizeof(double) + (size_t)(QQ - 1) * kBlock * sizeof(float);
+    size_t lds = MODE == MODE_STEP ? (size_t)kWaves * a.nb * 6 * sizeof(double) : 0;
+    if (IBB) lds = (size_t)kWaves * a.nb * 6 * sizeof(double) + (size_t)(QQ - 1) * kBlock * sizeof(float);
 #define K27(COLL, FORCED)                                                                                           \
     hipLaunchKernelGGL((k27_step<S, COLL, FORCED, MODE, IBB>), grid, block, lds, a.stream, (const T*)a.fin, (T*)a.fout, \
                        a.gout, a.rho_out, a.u_out, a.g, p, a.fl_new, a.fl_old, a.bnew, a.bold, a.nb, a.partials,   \

@@ -152,7 +152,9 @@ int conv(lbm_unet* n, int geom, const Buf& x, int xco, int cin, const bf16* w, i
         return fail(n, -2, "convolution rows and output differ in size");
     // 16*nt output channels per wave; narrower where a wide tile leaves the chip short of blocks
     // (the coarse levels have few rows)
-    int nt = std::min(coutp / 16, 4);
+    // nt in {1, 2, 4} divides coutp / 16, so the grid's y extent covers every channel
+    int nt = 4;
+    while (nt > 1 && (coutp / 16) % nt != 0) nt >>= 1;
     while (nt > 1 && (int64_t)nblk(a.m) * (coutp / (16 * nt)) * npar < 512) nt >>= 1;
     const dim3 grid(nblk(a.m), (unsigned)(coutp / (16 * nt)), (unsigned)npar), block(kThreads);
 #define LAUNCH_CONV(G, NTV) hipLaunchKernelGGL((k_conv<G, NTV>), grid, block, 0, n->stream, a)

@@ -160,10 +160,10 @@ def config_from_params(p: LBMParams, storage: str = "fp32", wall_point: str = "p
     * its heave law y(t) = y_c - A sin(w t), A = 0.2 (2R), w = 2 pi 0.154 U_inf / (2R)
       (src/MainCore.jl:72-80; the airfoil heaves with the same law, R included).
 
-    The start is the uniform equilibrium of (rho_inf, U_inf, V_inf, W_inf); the
-    reference's random 20 % perturbation of the start velocity
-    (src/Initialization.jl:70) is not reproduced.  wall_point="reference" samples
-    the moving-wall velocity where the reference does (src/SolidBC.jl:195-196).
+    The start the reference takes, the equilibrium of a velocity perturbed by up
+    to 20 % per node, comes from perturbed_start(); the config alone starts
+    uniform.  wall_point="reference" samples the moving-wall velocity where the
+    reference does (src/SolidBC.jl:195-196).
 
     Returns (Config, [ConstantMotion]) for Solver(config).set_motion(laws).
     """
@@ -192,3 +192,18 @@ def config_from_params(p: LBMParams, storage: str = "fp32", wall_point: str = "p
         law = ConstantMotion("naca4", centre, (chord,), amp=amp, freq=freq, rot0=rotation_z(float(p.theta0)),
                              naca=naca_params(chord, p.naca_code or "0012", 0.25, X_INT))
     return cfg, [law]
+
+
+def perturbed_start(cfg, seed: int = 0, fac: float = 0.2) -> tuple[np.ndarray, np.ndarray]:
+    """(rho, u) of the reference's start (initialize_f_gpu, src/Initialization.jl:27-44,
+    with rand_node of :70): rho = rho0 and u = u0 (1 + fac r) componentwise, one
+    r = 2 rand - 1 in [-1, 1) per node shared by the three components, drawn in
+    FP32 from numpy's PCG64 stream of `seed` (the reference draws CUDA.rand; its
+    stream is not reproduced).  Arrays for Solver.init_equilibrium(rho, u)."""
+    N = cfg.nx * cfg.ny * cfg.nz
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = (np.float32(2.0) * rng.random(N, dtype=np.float32) - np.float32(1.0)).astype(np.float64)
+    u0 = np.asarray(cfg.u0, dtype=np.float64)
+    rho = np.full(N, float(cfg.rho0))
+    u = u0[:, None] * (1.0 + fac * r)[None, :]
+    return rho, u

@@ -10,7 +10,10 @@ n_max time steps, the aerodynamic coefficients of every step
 n_save steps into output/final_fields.h5 (write_fields_h5_time!,
 src/I_O.jl:296-405), and the MLUPS summary of measure_Mlups
 (src/Utilities.jl:903-926).  The steps between two outputs run as one lbm_step
-call (the body's law is evaluated on the device).
+call (the body's law is evaluated on the device).  The start is the reference's
+(initialize_f, src/Initialization.jl:57-83: the equilibrium of a start velocity
+perturbed by up to 20 % per node, seeded) or, with --start uniform, the
+unperturbed equilibrium.
 """
 from __future__ import annotations
 
@@ -21,13 +24,17 @@ import sys
 import time
 
 from .h5io import AeroCoefficients, FieldSnapshots
-from .params import aero_coefficients, config_from_params, read_params_input
+from .params import aero_coefficients, config_from_params, perturbed_start, read_params_input
 
 
 def run(params_path: str, out_dir: str = "output", storage: str = "fp32", n_max: int | None = None,
-        n_save: int | None = None, wall_point: str = "physical", log=print, params=None, make_solver=None) -> dict:
+        n_save: int | None = None, wall_point: str = "physical", log=print, params=None, make_solver=None,
+        start: str = "perturbed", seed: int = 0) -> dict:
     """params (an LBMParams) replaces reading params_path; make_solver(config)
-    builds the solver (default: the HIP Solver)."""
+    builds the solver (default: the HIP Solver); start "perturbed" (the
+    reference's, seeded) or "uniform"."""
+    if start not in ("perturbed", "uniform"):
+        raise ValueError(f"unknown start {start!r}")
     from .solver import Solver
 
     p = params if params is not None else read_params_input(params_path)
@@ -43,7 +50,10 @@ def run(params_path: str, out_dir: str = "output", storage: str = "fp32", n_max:
     coeffs = AeroCoefficients(os.path.join(out_dir, "aero_coeffs.h5"), p.n_max)
     s = make_solver(cfg) if make_solver is not None else Solver(cfg)
     s.set_motion(laws)
-    s.init_equilibrium()
+    if start == "perturbed":
+        s.init_equilibrium(*perturbed_start(cfg, seed))
+    else:
+        s.init_equilibrium()
     s.sync()
     t0 = time.perf_counter()
     n, t_steps = 0, 0.0
@@ -82,9 +92,12 @@ def main(argv=None) -> int:
     ap.add_argument("--n-max", type=int, default=None, help="override n_max of the input file")
     ap.add_argument("--n-save", type=int, default=None, help="override n_save of the input file")
     ap.add_argument("--wall-point", default="physical", choices=["physical", "reference"])
+    ap.add_argument("--start", default="perturbed", choices=["perturbed", "uniform"],
+                    help="perturbed: the reference's start, u = U (1 + 0.2 r) per node; uniform: u = U")
+    ap.add_argument("--seed", type=int, default=0, help="seed of the perturbed start")
     a = ap.parse_args(argv)
     run(a.params, a.out, a.storage, a.n_max, a.n_save, a.wall_point,
-        log=lambda m: print(m, flush=True))
+        log=lambda m: print(m, flush=True), start=a.start, seed=a.seed)
     return 0
 
 

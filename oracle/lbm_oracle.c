@@ -110,6 +110,14 @@ void orc_naca_test(float x, float chord, float m, float p, float t, float x_int,
     orc_naca_lines(x, chord, m, p, t, x_int, yl, yu);
 }
 
+/* Plants one known mistake (ORC_PLANT_*, oracle_geom.h) in every later step, or
+ * removes it (0); tests use it to show that a pin fails on a wrong oracle. */
+int orc_plant(int code) {
+    if (code < ORC_PLANT_NONE || code > ORC_PLANT_WALL_HALF) return LBM_ERR_ARG;
+    orc_planted = code;
+    return LBM_OK;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     extern int omp_get_max_threads(void);

@@ -67,6 +67,8 @@ def load_oracle() -> ct.CDLL:
         lib.orc_naca_test.argtypes = [ct.c_float] * 6 + [ct.POINTER(ct.c_float)] * 2
         lib.orc_naca_test.restype = None
         lib.orc_num_threads.restype = ct.c_int
+        lib.orc_plant.argtypes = [ct.c_int]
+        lib.orc_plant.restype = ct.c_int
         _LIB = lib
     return _LIB
 
@@ -93,6 +95,24 @@ class OracleSolver(Solver):
 
     def info(self) -> dict:  # not part of the oracle API
         return {"ncells": self.N}
+
+
+PLANTS = {"none": 0, "spin_sign": 1, "arm_sign": 2, "arm_mirror": 3, "wall_half": 4}
+
+
+class planted:
+    """Context manager: the oracle runs with one known mistake planted (tests only)."""
+
+    def __init__(self, name: str):
+        self.code = PLANTS[name]
+
+    def __enter__(self):
+        if load_oracle().orc_plant(self.code) != 0:
+            raise ValueError("unknown plant")
+        return self
+
+    def __exit__(self, *exc):
+        load_oracle().orc_plant(0)
 
 
 def body_at(law, n: int) -> _ffi.LbmBody:

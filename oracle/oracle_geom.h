@@ -41,6 +41,17 @@ static const int OCY19[27] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0
 static const int OCZ19[27] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 0};
 static const int OOPP19[27] = {1, 0, 3, 2, 5, 4, 7, 6, 9, 8, 11, 10, 13, 12, 15, 14, 17, 16, 18};
 
+/* Planted mistakes, for tests that show a closed-form pin fails when the
+ * oracle is wrong (set by orc_plant, 0 = none).  Never set outside tests. */
+enum {
+    ORC_PLANT_NONE = 0,
+    ORC_PLANT_SPIN_SIGN = 1,   /* wall velocity U + r x Omega instead of U + Omega x r */
+    ORC_PLANT_ARM_SIGN = 2,    /* torque arm X - x_w instead of x_w - X */
+    ORC_PLANT_ARM_MIRROR = 3,  /* torque arm to x - h c_d instead of x + h c_d */
+    ORC_PLANT_WALL_HALF = 4,   /* moving-wall term w rho 3 (c.u_w), half of 2 w rho 3 (c.u_w) */
+};
+static int orc_planted = ORC_PLANT_NONE;
+
 /* Per-body data prepared on the host once per pose (FP32 mask, FP64 kinematics). */
 typedef struct {
     int shape;

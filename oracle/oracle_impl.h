@@ -327,6 +327,8 @@ static void SFX(orc_wall_velocity)(const SFX(orc)* o, const orc_body* b, int x, 
         }
     }
     REAL om[3] = {(REAL)b->om[0], (REAL)b->om[1], (REAL)b->om[2]};
+    if (orc_planted == ORC_PLANT_SPIN_SIGN)
+        for (int a = 0; a < 3; ++a) om[a] = -om[a];
     uw[0] = (REAL)b->v[0] + (om[1] * r[2] - om[2] * r[1]);
     uw[1] = (REAL)b->v[1] + (om[2] * r[0] - om[0] * r[2]);
     uw[2] = (REAL)b->v[2] + (om[0] * r[1] - om[1] * r[0]);
@@ -383,6 +385,7 @@ static void SFX(orc_sbb)(SFX(orc)* o) {
         SFX(orc_wall_velocity)(o, b, x, y, z, d, o->cfg.wall_point, rule == 0 ? (REAL)0.5 : (REAL)qf, uw);
         REAL cuw = (REAL)o->cx[d] * uw[0] + (REAL)o->cy[d] * uw[1] + (REAL)o->cz[d] * uw[2];
         REAL corr = (REAL)2 * o->w[d] * o->rho[c] * cuw * (REAL)3;
+        if (orc_planted == ORC_PLANT_WALL_HALF) corr *= (REAL)0.5;
         REAL fd = o->f_new[(int64_t)d * N + c];
         REAL val;
         if (rule == 1) {
@@ -522,8 +525,11 @@ static void SFX(orc_mea)(SFX(orc)* o, double* out) {
         float qf;
         int64_t c2;
         double h = SFX(orc_link_rule)(o, x, y, z, d, b, &qf, &c2) == 0 ? 0.5 : (double)qf;
+        if (orc_planted == ORC_PLANT_ARM_MIRROR) h = -h;
         double r[3] = {(double)x + 0.5 + h * o->cx[d] - b->ch[0], (double)y + 0.5 + h * o->cy[d] - b->ch[1],
                        (double)z + 0.5 + h * o->cz[d] - b->ch[2]};
+        if (orc_planted == ORC_PLANT_ARM_SIGN)
+            for (int a = 0; a < 3; ++a) r[a] = -r[a];
         for (int a = 0; a < 3; ++a) {
             if (o->per[a]) {
                 double h = 0.5 * o->L[a];

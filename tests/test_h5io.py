@@ -56,7 +56,7 @@ def test_aero_coefficients_blocks(tmp_path):
 def test_run_driver_on_the_oracle(tmp_path):
     """The reference's loop (scaled input file) on the oracle: the files hold the
     oracle's own moments at every n_save step and its coefficients at every step."""
-    from lbm3d_bb_amd.params import aero_coefficients, config_from_params, read_params_input
+    from lbm3d_bb_amd.params import aero_coefficients, config_from_params, perturbed_start, read_params_input
     from lbm3d_bb_amd.run_reference import run
     from oracle.oracle import OracleSolver
 
@@ -67,7 +67,7 @@ def test_run_driver_on_the_oracle(tmp_path):
     cfg, laws = config_from_params(p)
     o = OracleSolver(cfg, "fp32")
     o.set_motion(laws)
-    o.init_equilibrium()
+    o.init_equilibrium(*perturbed_start(cfg, 0))  # the driver's default start
     F, snaps = [], []
     for k in (10, 10, 5):
         F.append(o.step(k))

@@ -17,7 +17,7 @@ from scipy.spatial.transform import Rotation
 from lbm3d_bb_amd.bodies import ConstantMotion, Body, ellipsoid, rotation_matrix, sphere
 from lbm3d_bb_amd.lattice import CX, CY, CZ, OPP, W, equilibrium
 from lbm3d_bb_amd.solver import Config, no_slip, periodic, pressure, velocity, zero_gradient
-from oracle.oracle import OracleSolver, body_at
+from oracle.oracle import OracleSolver, body_at, planted
 
 from cases import periodic_box, rel_err, trt_omegas
 
@@ -281,6 +281,105 @@ def test_mea_single_cell_moving_wall():
     F = o.step(1)
     assert (o.flags() != 0).sum() == 1
     assert np.abs(F[0, 0, :3] + 2.0 * uw).max() < 1e-15
+
+
+# A solid cell at s (a sphere of radius 0.4 on the node covers it alone) whose
+# half-step centre X differs from s, translating at V and spinning at Omega, in
+# fluid at rest (rho = 1).  At the first step every link term is
+# t_d = 2 w_d rho - 6 w_d rho c_d.u_w(x_w), u_w = V + Omega x (x_w - X), with the
+# wall point x_w = s - (1 - h) c_d on the link; c_d.(Omega x c_d) = 0, so
+# c_d.u_w = c_d.U with U = V + Omega x (s - X), and sum_d w_d c_d c_d = I/3 gives
+#     F = -2 rho U,        T = sum_d (x_w - X) x c_d t_d = (s - X) x F
+# (the arm's part along c_d drops out of c_d x c_d).  A planted sign flip of
+# Omega x r moves F, a flipped arm moves T; a mirrored arm (x - h c_d, the
+# reference's wall sampling point, src/SolidBC.jl:195-196) provably does not.
+_S = np.array([4.5, 4.5, 4.5])
+_X = np.array([4.8, 4.3, 5.2])
+_V = np.array([0.011, -0.007, 0.004])
+_OM = np.array([0.003, 0.005, -0.002])
+
+
+def _spinning_cell_step(q, solid_bc="SBB", wall_point="physical", X=_X):
+    cfg = periodic_box(n=9, g=(0, 0, 0))
+    cfg.q, cfg.solid_bc, cfg.wall_point = q, solid_bc, wall_point
+    o = OracleSolver(cfg, "fp64")
+    o.set_bodies([Body("sphere", tuple(_S), (0.4, 0.4, 0.4), center_half=tuple(X), vel=tuple(_V),
+                       omega=tuple(_OM))])
+    o.init_equilibrium()
+    F = o.step(1)[0, 0]
+    assert (o.flags() != 0).sum() == 1
+    return F[:3], F[3:]
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_mea_moving_spinning_cell_force_and_torque_closed_form(q):
+    U = _V + np.cross(_OM, _S - _X)
+    Fex = -2.0 * U
+    F, T = _spinning_cell_step(q)
+    assert np.abs(F - Fex).max() < 1e-15
+    assert np.abs(T - np.cross(_S - _X, Fex)).max() < 1e-15
+    # the reference's mirrored wall point gives the same force and torque
+    Fr, Tr = _spinning_cell_step(q, wall_point="reference")
+    assert np.abs(Fr - Fex).max() < 1e-15 and np.abs(Tr - T).max() < 1e-15
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_mea_spinning_cell_pins_fail_on_planted_mistakes(q):
+    U = _V + np.cross(_OM, _S - _X)
+    Fex = -2.0 * U
+    Tex = np.cross(_S - _X, Fex)
+    with planted("spin_sign"):
+        F, T = _spinning_cell_step(q)
+    assert np.abs(F - Fex).max() > 1e-3 * np.abs(Fex).max()
+    with planted("arm_sign"):
+        F, T = _spinning_cell_step(q)
+    assert np.abs(F - Fex).max() < 1e-15 and np.abs(T - Tex).max() > 0.5 * np.abs(Tex).max()
+    with planted("wall_half"):
+        F, T = _spinning_cell_step(q)
+    assert np.abs(F - Fex).max() > 0.4 * np.abs(Fex).max()
+    # a mirrored torque arm is invisible: the arm's change is parallel to c_d
+    with planted("arm_mirror"):
+        F, T = _spinning_cell_step(q)
+    assert np.abs(T - Tex).max() < 1e-15
+    F, T = _spinning_cell_step(q)
+    assert np.abs(F - Fex).max() < 1e-15 and np.abs(T - Tex).max() < 1e-15
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_mea_spinning_cell_interpolated_closed_form(q):
+    """Interpolated bounce-back on the same cell, centred on the node (the level
+    set |x - X| - 0.4 is measured from the half-step centre X = s): every link
+    is on the q >= 1/2 branch with q = (|c| - 0.4)/|c|, so at rest
+    t_d = 2 w_d rho - 6 w_d rho c_d.V / (2 q_d) and
+    F = -3 rho sum_d w_d c_d c_d V / q_d, i.e. -(rho/3)(1/q_a + 2/q_e) V on
+    D3Q19 (axis and edge directions) and -(rho/9)(4/q_a + 4/q_e + 1/q_c) V on
+    D3Q27 (with the corners); the spin adds nothing (Omega x (s - X) = 0) and
+    T = 0.  q = 1/2 everywhere would give the simple rule's -2 rho V."""
+    qa, qe, qc = 0.6, 1.0 - 0.4 / np.sqrt(2.0), 1.0 - 0.4 / np.sqrt(3.0)
+    k = (1.0 / qa + 2.0 / qe) / 3.0 if q == 19 else (4.0 / qa + 4.0 / qe + 1.0 / qc) / 9.0
+    Fex = -k * _V
+    F, T = _spinning_cell_step(q, solid_bc="IBB", X=_S)
+    # q is formed from the FP32 level set: relative rounding ~1e-7
+    assert np.abs(F - Fex).max() < 1e-6 * np.abs(Fex).max()
+    assert np.abs(T).max() < 1e-15
+    with planted("wall_half"):
+        F, T = _spinning_cell_step(q, solid_bc="IBB", X=_S)
+    assert np.abs(F - Fex).max() > 0.4 * np.abs(Fex).max()
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_mea_static_cell_uniform_flow_torque(q):
+    """A static solid cell at s in uniform equilibrium flow u: F = 2 rho u and
+    T = (s - X) x F about a half-step centre X != s."""
+    u0 = np.array([0.03, -0.01, 0.02])
+    cfg = periodic_box(n=9, g=(0, 0, 0))
+    cfg.q, cfg.u0 = q, tuple(u0)
+    o = OracleSolver(cfg, "fp64")
+    o.set_bodies([Body("sphere", tuple(_S), (0.4, 0.4, 0.4), center_half=tuple(_X))])
+    o.init_equilibrium()
+    F = o.step(1)[0, 0]
+    assert np.abs(F[:3] - 2.0 * u0).max() < 1e-15
+    assert np.abs(F[3:] - np.cross(_S - _X, 2.0 * u0)).max() < 1e-15
 
 
 def test_mea_momentum_budget():

@@ -71,3 +71,25 @@ def test_lattice_q_reaches_the_config():
     assert cfg27.q == 27 and cfg27.to_c().q == 27
     with pytest.raises(ValueError):
         config_from_params(dataclasses.replace(p, q=15))[0].to_c()
+
+
+def test_perturbed_start_is_the_reference_form():
+    """initialize_f's start (src/Initialization.jl:27-44,70): u = U (1 + 0.2 r)
+    with one r in [-1, 1) per node for all three components, rho = rho_inf;
+    seeded, so the same seed gives the same field."""
+    import numpy as np
+
+    from lbm3d_bb_amd.params import perturbed_start
+    from lbm3d_bb_amd.solver import Config
+
+    cfg = Config(20, 16, 12, u0=(0.075, -0.02, 0.01), rho0=1.02)
+    rho, u = perturbed_start(cfg, seed=3)
+    assert rho.shape == (cfg.ncells,) and (rho == 1.02).all()
+    assert u.shape == (3, cfg.ncells)
+    r = u / np.asarray(cfg.u0)[:, None]
+    assert np.allclose(r[0], r[1]) and np.allclose(r[0], r[2])  # one r per node
+    assert r.min() >= 0.8 - 1e-12 and r.max() < 1.2
+    assert abs(r.mean() - 1.0) < 0.01 and r.std() > 0.1  # uniform on [0.8, 1.2): std 0.2/sqrt(3)
+    rho2, u2 = perturbed_start(cfg, seed=3)
+    assert np.array_equal(u, u2)
+    assert not np.array_equal(u, perturbed_start(cfg, seed=4)[1])

@@ -35,13 +35,16 @@ def _data(shape, batch, seed=0):
     return x, t
 
 
-@pytest.mark.parametrize("levels,shape", [(2, (8, 8, 16)), (3, (16, 16, 16)), (4, (16, 16, 24))])
-def test_forward_matches_torch(levels, shape):
+@pytest.mark.parametrize("levels,shape,base", [(2, (8, 8, 16), 16), (3, (16, 16, 16), 16), (4, (16, 16, 24), 16),
+                                               (2, (8, 8, 16), 48), (2, (8, 16, 16), 80)])
+def test_forward_matches_torch(levels, shape, base):
+    """Bases 48 and 80 give channel counts (48, 96, 80, 160) that are not
+    16 times a power of two: every output channel tile must still be computed."""
     import torch
 
     from unet_reference import forward
 
-    net = _net(shape, batch=2, levels=levels)
+    net = _net(shape, batch=2, levels=levels, base=base)
     x, _ = _data(shape, 2)
     y = net.forward(x)
     p = torch.tensor(net.get_params(), dtype=torch.float64)
@@ -50,11 +53,11 @@ def test_forward_matches_torch(levels, shape):
     assert _rel(y, yr) < 2e-2, _rel(y, yr)
 
 
-def test_gradient_matches_torch_autograd():
+@pytest.mark.parametrize("levels,shape,base", [(3, (16, 16, 16), 16), (2, (8, 8, 16), 48)])
+def test_gradient_matches_torch_autograd(levels, shape, base):
     from unet_reference import loss_and_grad
 
-    shape, levels = (16, 16, 16), 3
-    net = _net(shape, batch=2, levels=levels)
+    net = _net(shape, batch=2, levels=levels, base=base)
     x, t = _data(shape, 2, seed=5)
     loss, g = net.gradient(x, t)
     lr, gr, _ = loss_and_grad(net.get_params(), net.layers(), x, t, levels)
