@@ -222,6 +222,17 @@ int lbm_get_populations(lbm_ctx* ctx, double* f);
 int lbm_set_populations(lbm_ctx* ctx, const double* f);
 /* FP64 sums over fluid cells of rho and rho*u for the last completed step. */
 int lbm_conservation(lbm_ctx* ctx, double* mass, double* mom3);
+/* lbm_conservation without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory (HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The same sums, formed from the
+ * stored post-collision state (collision conserves rho and adds exactly rho g
+ * to rho u), so with FP64/FP32 storage they equal lbm_conservation's to
+ * rounding and with 16-bit storage they differ by that storage's rounding of
+ * f*.  Its partial sums live in a library-owned device array of
+ * 1024 x 4 + 4 doubles per GPU, which bounds no lattice size; calls that
+ * overlap on one GPU share that array and race.  Call from the context's
+ * device; synchronises the context stream. */
+int lbm_conservation_static(lbm_ctx* ctx, double* mass, double* mom3);
 
 /* Introspection: N, number of boundary (link-carrying) nodes, number of links,
  * completed steps, device bytes held. */

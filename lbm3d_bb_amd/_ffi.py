@@ -92,6 +92,7 @@ SIGNATURES = {
     "get_populations": [_P, _D],
     "set_populations": [_P, _D],
     "conservation": [_P, _D, _D],
+    "conservation_static": [_P, _D, _D],
     "info": [_P, _I64, _I64, _I64, _I64, _I64],
     "profile": [_P, ct.c_int32],
     "profile_read": [_P, _D, _I64, _D, _I64],

@@ -6,14 +6,18 @@
 // src/Motion.jl:41-122) and the per-step operator sequence.
 #include <hip/hip_runtime.h>
 
+#include <hip/hip_fp16.h>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
+#include "lbm_lattice.h"
 #include "lbm_q27.h"
 #include "lbm_types.h"
 
@@ -987,6 +991,115 @@ int get_state(lbm_ctx* c, double* d_f, double* d_rho, double* d_u) {
     return launch_step(c, MODE_GET, c->last_grid, prev, nullptr, d_f, d_rho, d_u, c->last_rec, 0, nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free conservation sums (lbm_conservation_static).
+//
+// The state the context holds is f* = C(f_old) for every cell, and the
+// collision conserves rho and adds exactly rho g to rho u (the Guo source), so
+// the fluid cells' mass and momentum of f_old are sum f* and
+// sum c f* - M g / 2 (u carries the half-force shift, src/ComputeMacro.jl:26-45
+// plus the forcing).  One pass over the stored f* gives them, with no
+// per-cell moment arrays: a fixed grid of kConsBlocks blocks strides over the
+// cells, each block writes its four partial sums (every block writes, so
+// nothing needs clearing), and one block adds the partials in block order.
+// The partials and the result live in `scratch`, a file-scope device array:
+// each GPU holds its own copy for the life of the process, and its device
+// address is looked up once per device when the first context is created
+// there.  Calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+constexpr int kConsBlocks = 1024;
+constexpr int kConsThreads = 256;
+constexpr int kScratchDoubles = kConsBlocks * 4 + 4;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (once per device).
+int scratch_init(lbm_ctx* c) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, LBM_ERR_ARG, "device index beyond the scratch table");
+    if (g_scratch[c->device]) return LBM_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(c, LBM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[c->device] = static_cast<double*>(p);
+    return LBM_OK;
+}
+
+double* scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+}
+
+struct ConsDirs {
+    int8_t cx[27], cy[27], cz[27];
+    float w[27];  // weights in FP32, for the shifted 16-bit formats
+};
+
+// One stored population as the storage format's load returns it, in FP64.
+template <int ST>
+__device__ __forceinline__ double stored_pop(const void* f, int64_t i, float w, float ik) {
+    if (ST == LBM_FP64) return static_cast<const double*>(f)[i];
+    if (ST == LBM_FP32) return (double)static_cast<const float*>(f)[i];
+    const uint16_t h = static_cast<const uint16_t*>(f)[i];
+    if (ST == LBM_FP16S) return (double)__fmaf_rn(__half2float(__ushort_as_half(h)), ik, w);
+    return (double)(__uint_as_float(uint32_t(h) << 16) + w);
+}
+
+template <int ST>
+__global__ __launch_bounds__(kConsThreads) void k_cons_partials(const void* __restrict__ f, int64_t P,
+                                                                const uint8_t* __restrict__ flags, int64_t N, int q,
+                                                                ConsDirs t, float ik, double* __restrict__ part) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t cell = (int64_t)blockIdx.x * kConsThreads + threadIdx.x; cell < N;
+         cell += (int64_t)gridDim.x * kConsThreads) {
+        if (flags[cell] & kFlagBody) continue;  // solid (the fix-up bit marks fluid boundary nodes)
+        double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+        for (int d = 0; d < q; ++d) {
+            const double v = stored_pop<ST>(f, (int64_t)d * P + cell, t.w[d], ik);
+            r += v;
+            jx += (double)t.cx[d] * v;
+            jy += (double)t.cy[d] * v;
+            jz += (double)t.cz[d] * v;
+        }
+        s[0] += r;
+        s[1] += jx;
+        s[2] += jy;
+        s[3] += jz;
+    }
+    __shared__ double lds[kConsThreads / 64][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double v = s[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) lds[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = lds[0][threadIdx.x];
+        for (int w = 1; w < kConsThreads / 64; ++w) v += lds[w][threadIdx.x];
+        part[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
+    }
+}
+
+// out = (mass, momentum - mass g / 2) from the partials, added in block order
+// (one wave; lanes 0..3 own the four sums)
+__global__ void k_cons_final(const double* __restrict__ part, int nblocks, double gx, double gy, double gz,
+                             double* __restrict__ out) {
+    const int k = threadIdx.x;
+    double v = 0.0;
+    if (k < 4)
+        for (int b = 0; b < nblocks; ++b) v += part[(int64_t)b * 4 + k];
+    const double mass = __shfl(v, 0, 64);
+    const double g = k == 1 ? gx : (k == 2 ? gy : gz);
+    if (k < 4) out[k] = k == 0 ? v : v - 0.5 * mass * g;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1109,7 +1222,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
         delete c;
         return LBM_ERR_HIP;
     }
-    int rc = LBM_OK;
+    int rc = scratch_init(c);
     for (int i = 0; i < 2 && rc == LBM_OK; ++i) {
         rc = dev_alloc(c, &c->fbuf_raw[i], c->fbytes + kFrontPad);
         if (rc == LBM_OK) c->fbuf[i] = (char*)c->fbuf_raw[i] + kFrontPad;
@@ -1530,6 +1643,63 @@ int lbm_conservation(lbm_ctx* c, double* mass, double* mom3) {
     dev_free(c, d_u, (size_t)3 * N * 8);
     dev_free(c, d_p, (size_t)nblk * 4 * 8);
     return rc;
+}
+
+int lbm_conservation_static(lbm_ctx* c, double* mass, double* mom3) {
+    if (!c) return LBM_ERR_ARG;
+    if (!c->initialised) return fail(c, LBM_ERR_STATE, "not initialised");
+    double* sp = scratch_of(c->device);
+    if (!sp) return fail(c, LBM_ERR_STATE, "no scratch address for the context's device");
+    int cur_dev = -1;
+    HIPCHK(c, hipGetDevice(&cur_dev));
+    if (cur_dev != c->device) return fail(c, LBM_ERR_STATE, "lbm_conservation_static called off the context's device");
+    ConsDirs t{};
+    for (int d = 0; d < c->q; ++d) {
+        t.cx[d] = (int8_t)(c->q == 27 ? q27::kX[d] : kCx[d]);
+        t.cy[d] = (int8_t)(c->q == 27 ? q27::kY[d] : kCy[d]);
+        t.cz[d] = (int8_t)(c->q == 27 ? q27::kZ[d] : kCz[d]);
+        const int n = (t.cx[d] != 0) + (t.cy[d] != 0) + (t.cz[d] != 0);
+        // the weights exactly as the storage formats' loads form them, in FP32
+        if (c->q == 27)
+            t.w[d] = n == 0 ? 8.0f / 27.0f : (n == 1 ? 2.0f / 27.0f : (n == 2 ? 1.0f / 54.0f : 1.0f / 216.0f));
+        else
+            t.w[d] = weight<float>(d);
+    }
+    const void* f = c->fbuf[c->cur];
+    const dim3 grid(kConsBlocks), block(kConsThreads);
+    switch (c->cfg.storage) {
+        case LBM_FP64:
+            hipLaunchKernelGGL(k_cons_partials<LBM_FP64>, grid, block, 0, c->stream, f, c->g.P, c->flags, c->g.N, c->q,
+                               t, c->g.sh.ik, sp);
+            break;
+        case LBM_FP32:
+            hipLaunchKernelGGL(k_cons_partials<LBM_FP32>, grid, block, 0, c->stream, f, c->g.P, c->flags, c->g.N, c->q,
+                               t, c->g.sh.ik, sp);
+            break;
+        case LBM_FP16S:
+            hipLaunchKernelGGL(k_cons_partials<LBM_FP16S>, grid, block, 0, c->stream, f, c->g.P, c->flags, c->g.N,
+                               c->q, t, c->g.sh.ik, sp);
+            break;
+        default:
+            hipLaunchKernelGGL(k_cons_partials<LBM_BF16S>, grid, block, 0, c->stream, f, c->g.P, c->flags, c->g.N,
+                               c->q, t, c->g.sh.ik, sp);
+            break;
+    }
+    HIPCHK(c, hipGetLastError());
+    double* res = sp + (size_t)kConsBlocks * 4;
+    hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(64), 0, c->stream, sp, kConsBlocks, c->pd.g[0], c->pd.g[1],
+                       c->pd.g[2], res);
+    HIPCHK(c, hipGetLastError());
+    double h[4];
+    HIPCHK(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (mass) *mass = h[0];
+    if (mom3) {
+        mom3[0] = h[1];
+        mom3[1] = h[2];
+        mom3[2] = h[3];
+    }
+    return LBM_OK;
 }
 
 int lbm_info(lbm_ctx* c, int64_t* ncells, int64_t* nbb, int64_t* nlinks, int64_t* steps, int64_t* device_bytes) {

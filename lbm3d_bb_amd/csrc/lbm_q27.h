@@ -7,6 +7,11 @@ namespace lbm {
 namespace q27 {
 
 constexpr int kQ = 27;
+// ex, ey, ez of src/Utilities.jl:962-964: six faces, twelve edges, eight
+// corners, rest last; moving directions pair as (2k, 2k + 1)
+constexpr int kX[kQ] = {1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 0};
+constexpr int kY[kQ] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1, 0};
+constexpr int kZ[kQ] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 0};
 
 // One launch of k27_step: a time step (MODE_STEP), the getters' replay of the
 // last step (MODE_GET) or of the start state (MODE_GET_RAW), or the collision

@@ -27,11 +27,6 @@ namespace lbm {
 namespace q27 {
 
 constexpr int QQ = kQ;
-// ex, ey, ez of src/Utilities.jl:962-964: six faces, twelve edges, eight
-// corners, rest last; moving directions pair as (2k, 2k + 1)
-constexpr int kX[QQ] = {1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 0};
-constexpr int kY[QQ] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1, 0};
-constexpr int kZ[QQ] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 0};
 
 // bbOpp (src/Utilities.jl:976)
 __host__ __device__ constexpr int opp(int d) { return d == QQ - 1 ? d : (d ^ 1); }
