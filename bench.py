@@ -19,7 +19,9 @@ around every 16th launch in the timed region; ``traffic`` is the HBM bytes per
 launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, when
 profiles/traffic_<workload>.json holds it.  ``cpu_baseline`` times the oracle (a
 plain C restatement, OpenMP) on the same workload for a bounded number of
-steps.
+steps.  ``config.drag_err_vs_fp64`` compares each body's drag, averaged over
+the last 100 steps of the run, with an untimed FP64-storage run of the same
+workload, start and step count.
 
 --copy-probe MiB runs, after the timed region, ten device-to-device copies of
 that many MiB (a plain copy kernel of known size, the calibration of the
@@ -195,6 +197,8 @@ def main() -> int:
     ap.add_argument("--params", default=None, help="U-Net parameters (.npy) for --start net; random init if absent")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
                     help="after the run, ten device-to-device copies of this many MiB (counter calibration)")
+    ap.add_argument("--no-fp64-ref", action="store_true",
+                    help="skip the FP64 run of the same steps that drag_err_vs_fp64 compares against")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): there is no multi-GPU path; the path "
@@ -215,8 +219,7 @@ def main() -> int:
     start = {"start": "uniform", "reason": "uniform start asked"}
     if args.start == "net":
         start = net_start(s, cfg, laws, bodies, args.params)
-    if args.warmup:
-        s.step(args.warmup)
+    warm_forces = s.step(args.warmup) if args.warmup else None
     s.sync()
     s.profile(not args.no_profile)
     t0 = time.perf_counter()
@@ -281,10 +284,37 @@ def main() -> int:
         out["start"] = start
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+    if not args.no_fp64_ref and forces is not None:
+        F = forces if warm_forces is None else np.concatenate([warm_forces, forces])
+        out["config"]["drag_err_vs_fp64"] = drag_err_vs_fp64(w, F, args, cfg, laws, bodies)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def drag_err_vs_fp64(w, F, args, cfg, laws, bodies) -> dict:
+    """The same workload, start and steps in FP64 storage (untimed, after the
+    timed run): relative error of each body's drag (x-force) averaged over the
+    last min(100, steps) steps, against the FP64 run's."""
+    from lbm3d_bb_amd.solver import Solver
+
+    n = F.shape[0]
+    win = min(100, n)
+    if w["storage"] == "fp64":
+        return {"max": 0.0, "total": 0.0, "steps": n, "window": win, "note": "this run is FP64"}
+    cfg64 = make_case(dict(w, storage="fp64"))[0]
+    r = Solver(cfg64)
+    setup(r, laws, bodies)
+    if args.start == "net":
+        net_start(r, cfg64, laws, bodies, args.params)
+    F64 = r.step(n)
+    r.close()
+    d = F[-win:, :, 0].mean(axis=0)
+    d64 = F64[-win:, :, 0].mean(axis=0)
+    e = np.abs(d - d64) / np.abs(d64)
+    return {"max": float(e.max()), "median": float(np.median(e)), "total": float(abs(d.sum() - d64.sum()) /
+            abs(d64.sum())), "steps": n, "window": win}
 
 
 def net_start(s, cfg, laws, bodies, params) -> dict:

@@ -765,6 +765,18 @@ int q27_step_launch(lbm_ctx* c, const q27::StepArgs& a) {
 // D3Q19 step geometry takes
 static_assert(sizeof(StepGeom) >= LBM_MAX_BODIES * 6 * sizeof(int32_t), "step record too small for the flag boxes");
 
+// Seed of step n's stochastic rounding (16-bit storage): distinct per step,
+// fixed by n alone, so a run repeats bitwise.
+uint32_t step_seed(int64_t n) {
+    uint32_t x = (uint32_t)n * 0x9e3779b9u + (uint32_t)(n >> 32) + 0x85ebca6bu;
+    x ^= x >> 16;
+    x *= 0x21f0aaadu;
+    x ^= x >> 15;
+    x *= 0xd35a2d97u;
+    x ^= x >> 15;
+    return x;
+}
+
 int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
     // block partials of every step of the chunk: at most 1 GiB of them
     {
@@ -865,6 +877,7 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
             else
                 GEOCHK(c, launch_flags_full(c->flags, dn, nb, c->g, c->stream));
         }
+        c->g.sh.seed = step_seed(c->step + 1 + s);
         q27::StepArgs a = q27_args(c, MODE_STEP, c->fbuf[c->cur], c->fbuf[1 - c->cur]);
         a.fl_old = mv ? c->flags_old : c->flags;
         a.bnew = dn;
@@ -938,6 +951,7 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
             e1 = next_event(c);
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
+        c->g.sh.seed = step_seed(c->step + 1 + s);
         int lr = launch_step(c, MODE_STEP, grids[s], fin, fout, nullptr, nullptr, nullptr, rec, 1,
                              c->d_partials + (size_t)s * stride);
         if (lr) return lr;
@@ -966,6 +980,7 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
 }
 
 int collide_only(lbm_ctx* c) {
+    c->g.sh.seed = step_seed(c->step);
     if (c->q == 27) return q27_step_launch(c, q27_args(c, MODE_COLLIDE_ONLY, c->fbuf[1 - c->cur], c->fbuf[c->cur]));
     return launch_bulk(c, MODE_COLLIDE_ONLY, c->fbuf[1 - c->cur], c->fbuf[c->cur], nullptr, nullptr, nullptr, nullptr);
 }

@@ -279,6 +279,105 @@ __device__ inline void collide(C f[Q], const Phys<C>& p, C& rho_out, C u_out[3])
     }
 }
 
+// The same collision on deviations delta_d = f_d - w_d (the shifted 16-bit
+// formats), in FP32.  The populations stay at the size of their deviation
+// through the arithmetic, so its rounding is relative to that, not to w_d; and
+// the equilibrium is normalised with the sums of the FP32 weights
+// (rho_w = sum w_d, s2 = sum w_d c_dx^2) so that it carries exactly the cell's
+// mass and momentum: with exact weights every coefficient reduces to the
+// reference's (1, 3, 4.5, 9), while the FP32 weights, which do not sum to 1,
+// would otherwise add (rho_w - 1) omega of mass every step.
+//   f_eq - w = w [rm + rh (a2 (c.u)^2 - 1.5 u^2)] + w c3 c.(rho u),
+//   rm = sum delta / rho_w, rh = 1 + rm, c3 = 1 / s2, a2 = 1.5 rho_w / s2;
+// Guo source w [kp (a9 (c.u)(c.F) - 3 u.F) + km c3 c.F], a9 = 3 rho_w / s2.
+template <int COLL, bool FORCED>
+__device__ inline void collide_dev(float f[Q], const Phys<float>& p) {
+    constexpr float rw = (float)kRhoW19, irw = (float)(1.0 / kRhoW19);
+    constexpr float c3 = (float)(1.0 / kS2_19), a2 = (float)(1.5 * kRhoW19 / kS2_19),
+                    a9 = (float)(3.0 * kRhoW19 / kS2_19);
+    float sd, jx, jy, jz;
+    moments<float>(f, sd, jx, jy, jz);
+    const float rho = rw + sd;
+    const float rm = sd * irw, rh = 1.0f + rm;
+    const float inv = 1.0f / rho;
+    float ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    float Fx = 0.f, Fy = 0.f, Fz = 0.f, uF = 0.f;
+    if (FORCED) {
+        ux += 0.5f * p.g[0];
+        uy += 0.5f * p.g[1];
+        uz += 0.5f * p.g[2];
+        Fx = rho * p.g[0];
+        Fy = rho * p.g[1];
+        Fz = rho * p.g[2];
+        uF = ux * Fx + uy * Fy + uz * Fz;
+    }
+    const float u2 = ux * ux + uy * uy + uz * uz;
+    const float mx = rho * ux, my = rho * uy, mz = rho * uz;
+    const float wp = p.wp;
+    const float kp = 1.0f - 0.5f * wp;
+    if (COLL == LBM_TRT) {
+        const float wm = p.wm;
+        const float km = 1.0f - 0.5f * wm;
+#pragma unroll
+        for (int d = 0; d < 18; d += 2) {
+            const float w = weight<float>(d);
+            const float cu = float(kCx[d]) * ux + float(kCy[d]) * uy + float(kCz[d]) * uz;
+            const float cm = float(kCx[d]) * mx + float(kCy[d]) * my + float(kCz[d]) * mz;
+            const float dp = w * (rm + rh * (a2 * cu * cu - 1.5f * u2));
+            const float dm = w * (c3 * cm);
+            const float fp = 0.5f * (f[d] + f[d + 1]);
+            const float fm = 0.5f * (f[d] - f[d + 1]);
+            float fpn = fp - wp * (fp - dp);
+            float fmn = fm - wm * (fm - dm);
+            if (FORCED) {
+                const float cF = float(kCx[d]) * Fx + float(kCy[d]) * Fy + float(kCz[d]) * Fz;
+                fpn += kp * (w * (a9 * cu * cF - 3.0f * uF));
+                fmn += km * (w * (c3 * cF));
+            }
+            f[d] = fpn + fmn;
+            f[d + 1] = fpn - fmn;
+        }
+        const float w0 = weight<float>(18);
+        const float d0 = w0 * (rm - rh * (1.5f * u2));
+        float r = f[18] - wp * (f[18] - d0);
+        if (FORCED) r += kp * (w0 * (-3.0f * uF));
+        f[18] = r;
+    } else {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            const float w = weight<float>(d);
+            const float cu = float(kCx[d]) * ux + float(kCy[d]) * uy + float(kCz[d]) * uz;
+            const float cm = float(kCx[d]) * mx + float(kCy[d]) * my + float(kCz[d]) * mz;
+            const float de = w * (rm + rh * (a2 * cu * cu - 1.5f * u2) + c3 * cm);
+            float r = f[d] - wp * (f[d] - de);
+            if (FORCED) {
+                const float cF = float(kCx[d]) * Fx + float(kCy[d]) * Fy + float(kCz[d]) * Fz;
+                r += kp * (w * (c3 * cF + a9 * cu * cF - 3.0f * uF));
+            }
+            f[d] = r;
+        }
+    }
+}
+
+// collision in the storage's register form
+template <class S, int COLL, bool FORCED>
+__device__ inline void collide_s(typename S::C f[Q], const Phys<typename S::C>& p) {
+    if constexpr (S::kDev) {
+        collide_dev<COLL, FORCED>(f, p);
+    } else {
+        typename S::C rho, u[3];
+        collide<typename S::C, COLL, FORCED>(f, p, rho, u);
+    }
+}
+
+// rest level of direction d in the storage's register form: w_d for the
+// deviations of the shifted formats, 0 for absolute populations
+template <class S>
+__device__ inline double rest_level(int d) {
+    if constexpr (S::kDev) return (double)weight<float>(d);
+    return 0.0;
+}
+
 // Moments of a pulled (pre-collision) state for the getters.
 template <typename C, bool FORCED>
 __device__ inline void macros_of(const C f[Q], const Phys<C>& p, C& rho, C u[3]) {
@@ -302,18 +401,30 @@ __device__ inline void finish_cell(typename S::C f[Q], const Phys<typename S::C>
                                    double* __restrict__ rho_out, double* __restrict__ u_out, const Shift& sh) {
     using C = typename S::C;
     if (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY) {
-        C rho, u[3];
-        collide<C, COLL, FORCED>(f, p, rho, u);
+        collide_s<S, COLL, FORCED>(f, p);
+        const uint32_t h = sr_cell(cell, sh.seed);
 #pragma unroll
-        for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * P + cell, d, f[d], sh);
+        for (int d = 0; d < Q; ++d) S::store_sr(fout + (int64_t)d * P + cell, d, f[d], sh, sr_dir(h, d));
     } else {
         if (gout) {
 #pragma unroll
-            for (int d = 0; d < Q; ++d) gout[(int64_t)d * N + cell] = (double)f[d];
+            for (int d = 0; d < Q; ++d) gout[(int64_t)d * N + cell] = (double)f[d] + rest_level<S>(d);
         }
         if (rho_out) {
             C rho, u[3];
-            macros_of<C, FORCED>(f, p, rho, u);
+            if constexpr (S::kDev) {
+                // rho = rho_w + sum delta, u = j / rho (+ g/2): the weights carry no momentum
+                C sd, jx, jy, jz;
+                moments<C>(f, sd, jx, jy, jz);
+                rho = C(kRhoW19) + sd;
+                u[0] = jx / rho;
+                u[1] = jy / rho;
+                u[2] = jz / rho;
+                if (FORCED)
+                    for (int a = 0; a < 3; ++a) u[a] += C(0.5) * p.g[a];
+            } else {
+                macros_of<C, FORCED>(f, p, rho, u);
+            }
             rho_out[cell] = (double)rho;
             u_out[cell] = (double)u[0];
             u_out[N + cell] = (double)u[1];
@@ -455,15 +566,24 @@ __device__ inline double face_u_at(const Grid& g, int i, int c) { return g.face_
 // Face rules for direction d whose source lies outside face fidx:
 // VELOCITY src/OtherBC.jl:143-148, PRESSURE_ABB src/OtherBC.jl:156-165.
 // uf: the face's wall / inlet velocity (read by the caller ahead of its loads)
-template <typename C>
-__device__ inline C face_value(int kind, const C uf[3], int d, C own_opp, C rho_prev, const C up[3], C uu2) {
+// For the shifted formats (S::kDev) own_opp is a deviation and rho_prev holds
+// rho - 1 (own_moments, the face slots): the rule in deviation form,
+// delta_in = -delta_opp + 2 w [(rho - 1) + rho (4.5 (c.u)^2 - 1.5 u^2)] for
+// anti-bounce-back, the same expression as f for the velocity rule.
+template <class S>
+__device__ inline typename S::C face_value(int kind, const typename S::C uf[3], int d, typename S::C own_opp,
+                                           typename S::C rho_prev, const typename S::C up[3], typename S::C uu2) {
+    using C = typename S::C;
     const C w = weight<C>(d);
+    const C rho = S::kDev ? C(1) + rho_prev : rho_prev;
     if (kind == LBM_FACE_VELOCITY) {
         const C cu = C(kCx[d]) * (-uf[0]) + C(kCy[d]) * (-uf[1]) + C(kCz[d]) * (-uf[2]);
-        return own_opp - C(2) * w * rho_prev * cu * C(3);
+        return own_opp - C(2) * w * rho * cu * C(3);
     }
     const C cu = C(kCx[d]) * up[0] + C(kCy[d]) * up[1] + C(kCz[d]) * up[2];
-    return -own_opp + C(2) * w * rho_prev * (C(1) + C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3));
+    if constexpr (S::kDev)
+        return -own_opp + C(2) * w * (rho_prev + rho * (C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3)));
+    return -own_opp + C(2) * w * rho * (C(1) + C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3));
 }
 
 template <class S, bool FORCED>
@@ -472,9 +592,13 @@ __device__ inline void own_moments(const typename S::C own[Q], const Phys<typena
     using C = typename S::C;
     C jx, jy, jz;
     moments<C>(own, rho, jx, jy, jz);
-    up[0] = jx / rho;
-    up[1] = jy / rho;
-    up[2] = jz / rho;
+    // shifted formats: rho holds sum delta; the density is rho_w + sum delta,
+    // and rho - 1 is what the rules take (face_value)
+    const C dens = S::kDev ? C(kRhoW19) + rho : rho;
+    if (S::kDev) rho = C(kRhoW19 - 1.0) + rho;
+    up[0] = jx / dens;
+    up[1] = jy / dens;
+    up[2] = jz / dens;
     if (FORCED) {
         up[0] -= C(0.5) * p.g[0];
         up[1] -= C(0.5) * p.g[1];
@@ -592,7 +716,7 @@ __device__ __forceinline__ int face_pull(const typename S::T* __restrict__ fin, 
             const typename S::T* a = fin + (int64_t)d * g.P + (src + ca * axis_stride);
             const typename S::T* b = fin + (int64_t)kOpp[d] * g.P + cell;
             const C v = S::load_pick(zg ? a : b, kOpp[d], d, zg, g.sh);
-            const C fv = face_value<C>(kind, uf, d, v, rho_prev, up, uu2);
+            const C fv = face_value<S>(kind, uf, d, v, rho_prev, up, uu2);
             f[d] = zg ? v : fv;
         } else {
             f[d] = S::load(fin + (int64_t)d * g.P + src, d, g.sh);
@@ -629,22 +753,23 @@ __device__ inline void finish_group(typename S::C (&f)[V][Q], const bool (&act)[
     using C = typename S::C;
     if (MODE == MODE_STEP || MODE == MODE_COLLIDE_ONLY) {
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            C rho, u[3];
-            collide<C, COLL, FORCED>(f[v], p, rho, u);
-        }
+        for (int v = 0; v < V; ++v) collide_s<S, COLL, FORCED>(f[v], p);
         bool all = true;
 #pragma unroll
         for (int v = 0; v < V; ++v) all = all && act[v];
         if (V == 2 && all) {
+            const uint32_t h = sr_cell(cell0, g.sh.seed);
 #pragma unroll
-            for (int d = 0; d < Q; ++d) S::store2(fout + (int64_t)d * g.P + cell0, d, f[0][d], f[V - 1][d], g.sh);
+            for (int d = 0; d < Q; ++d)
+                S::store2_sr(fout + (int64_t)d * g.P + cell0, d, f[0][d], f[V - 1][d], g.sh, sr_dir(h, d));
         } else {
 #pragma unroll
             for (int v = 0; v < V; ++v)
                 if (act[v]) {
+                    const uint32_t h = sr_cell(cell0 + v, g.sh.seed);
 #pragma unroll
-                    for (int d = 0; d < Q; ++d) S::store(fout + (int64_t)d * g.P + cell0 + v, d, f[v][d], g.sh);
+                    for (int d = 0; d < Q; ++d)
+                        S::store_sr(fout + (int64_t)d * g.P + cell0 + v, d, f[v][d], g.sh, sr_dir(h, d));
                 }
         }
     } else {
@@ -753,7 +878,7 @@ __device__ __forceinline__ void face_row_pair(const typename S::T* __restrict__ 
         const bool out = SIDE == 0 ? (ca > 0) : (ca < 0);
         if (!out) continue;
 #pragma unroll
-        for (int v = 0; v < 2; ++v) f[v][d] = face_value<C>(kind, uf, d, f[v][d], rho_prev[v], up[v], uu2[v]);
+        for (int v = 0; v < 2; ++v) f[v][d] = face_value<S>(kind, uf, d, f[v][d], rho_prev[v], up[v], uu2[v]);
     }
     bool act[2];
 #pragma unroll
@@ -864,7 +989,7 @@ __device__ __forceinline__ void interior_row(const typename S::T* __restrict__ f
             for (int d = 0; d < Q; ++d) {
                 if (kCx[d] <= 0) continue;
                 const int k = (d == 0) ? 0 : (d == 6 ? 1 : (d == 8 ? 2 : (d == 12 ? 3 : 4)));
-                f[0][d] = face_value<C>(kind, uf_lo, d, ex[k], rho_prev, up, uu2);
+                f[0][d] = face_value<S>(kind, uf_lo, d, ex[k], rho_prev, up, uu2);
             }
         }
         if (rr) {
@@ -873,7 +998,7 @@ __device__ __forceinline__ void interior_row(const typename S::T* __restrict__ f
             for (int d = 0; d < Q; ++d) {
                 if (kCx[d] >= 0) continue;
                 const int k = (d == 1) ? 0 : (d == 7 ? 1 : (d == 9 ? 2 : (d == 13 ? 3 : 4)));
-                f[V - 1][d] = face_value<C>(kind, uf_hi, d, ex[k], rho_prev, up, uu2);
+                f[V - 1][d] = face_value<S>(kind, uf_hi, d, ex[k], rho_prev, up, uu2);
             }
         }
     }
@@ -1010,11 +1135,12 @@ __global__ __launch_bounds__(kBlock) void k_bulk(const typename S::T* __restrict
 
 // ---------------------------------------------------------------------------
 // Force accumulation: per thread one body in registers; a link of a fluid node
-// to a second body goes straight into its wave's LDS slots, one lane at a time
-// in lane order (wave_ordered_add); per wave a deterministic segmented shuffle
-// reduction into the same slots; per block a fixed-order sum over the waves.
-// Every addition happens in an order fixed by the cell layout alone, so the
-// force of a step is bitwise reproducible.
+// to a second body goes into its wave's own LDS slots (wave_slot_add); per
+// wave a deterministic segmented shuffle reduction into the same slots; per
+// block a fixed-order sum over the waves.  No two waves share a slot, one
+// wave's LDS operations complete in program order, and within one atomic
+// instruction the LDS applies the lanes that hit one address in a fixed
+// order, so the force of a step is bitwise reproducible.
 // ---------------------------------------------------------------------------
 struct ForceAcc {
     int b0;
@@ -1027,20 +1153,10 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// Adds v[6] of every active lane with `on` set to lds_wave[key * 6 ...], lane
-// by lane in ascending lane order.  The slots belong to this wave alone and a
-// wave's LDS operations complete in program order, so the sums are fixed.
-__device__ __forceinline__ void wave_ordered_add(bool on, int key, const double v[6], double* lds_wave) {
-    const int lane = threadIdx.x & 63;
-    uint64_t pending = __ballot(on);
-    while (pending) {
-        const int l = __ffsll((unsigned long long)pending) - 1;
-        if (lane == l) {
+// v[6] of a link to body `key` into this wave's slots (see above)
+__device__ __forceinline__ void wave_slot_add(int key, const double v[6], double* lds_wave) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) atomicAdd(&lds_wave[key * 6 + c], v[c]);
-        }
-        pending &= pending - 1;
-    }
+    for (int c = 0; c < 6; ++c) atomicAdd(&lds_wave[key * 6 + c], v[c]);
 }
 
 __device__ inline void wave_reduce_to_lds(int key, const double vals[6], double* lds_wave /*[nb][6]*/) {
@@ -1244,7 +1360,10 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             for (int e = 0; e < Q; ++e) own[e] = S::load(fin + (int64_t)e * g.P + nc, e, g.sh);
             C r, mx, my, mz;
             moments<C>(own, r, mx, my, mz);
-            C ux = mx / r, uy = my / r, uz = mz / r;
+            // shifted formats: r is sum delta; average rho - 1 there (precise), rho elsewhere
+            const C dens = S::kDev ? C(kRhoW19) + r : r;
+            if (S::kDev) r = C(kRhoW19 - 1.0) + r;
+            C ux = mx / dens, uy = my / dens, uz = mz / dens;
             if (FORCED) {
                 ux -= C(0.5) * p.g[0];
                 uy -= C(0.5) * p.g[1];
@@ -1256,7 +1375,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             sw += uz;
             ++cnt;
         }
-        C r = C(g.rho0), ux = C(0), uy = C(0), uz = C(0);
+        C r = S::kDev ? C(g.rho0 - 1.0) : C(g.rho0), ux = C(0), uy = C(0), uz = C(0);
         if (cnt > 0) {
             const C n = C(cnt);
             r = sr / n;
@@ -1268,7 +1387,10 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
             const C cu = C(kCx[d]) * ux + C(kCy[d]) * uy + C(kCz[d]) * uz;
-            f[d] = weight<C>(d) * r * (C(1) + cu * C(3) + C(0.5) * cu * cu * C(9) - C(0.5) * u2 * C(3));
+            if constexpr (S::kDev)  // f_eq - w with r = rho - 1
+                f[d] = weight<C>(d) * (r + (C(1) + r) * (cu * C(3) + C(0.5) * cu * cu * C(9) - C(0.5) * u2 * C(3)));
+            else
+                f[d] = weight<C>(d) * r * (C(1) + cu * C(3) + C(0.5) * cu * cu * C(9) - C(0.5) * u2 * C(3));
         }
         finish_face_cell<S, COLL, FORCED, MODE>(f, p, cell, g, fout, gout, rho_out, u_out, x, y, z, -1);
         return;
@@ -1389,7 +1511,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
         if (kind_d == DIR_FACE) {
             // face velocities from LDS: a global load here would wait once per face direction
             const C uf[3] = {C(lds_face_u[3 * aux]), C(lds_face_u[3 * aux + 1]), C(lds_face_u[3 * aux + 2])};
-            f[d] = face_value<C>(face_kind_at(g, aux), uf, d, f[d], rho_prev, up, uu2);
+            f[d] = face_value<S>(face_kind_at(g, aux), uf, d, f[d], rho_prev, up, uu2);
         } else if (kind_d == DIR_LINK) {
             // boundary link (x, dl), dl = opp(d): the solid node is x + c_dl
             const int dl = kOpp[d];
@@ -1415,7 +1537,7 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             const C uwz = C(B.v[2] + (B.om[0] * r[1] - B.om[1] * r[0]));
             const C cuw = C(kCx[dl]) * uwx + C(kCy[dl]) * uwy + C(kCz[dl]) * uwz;
             const C fin_dl = f[d];
-            const C corr = C(2) * weight<C>(dl) * rho_prev * cuw * C(3);
+            const C corr = C(2) * weight<C>(dl) * (S::kDev ? C(1) + rho_prev : rho_prev) * cuw * C(3);
             C val = fin_dl - corr;
             if (IBB && rule != 0) {
                 // rule 1: f*_dl at the second fluid node x - c_dl; rule 2: the
@@ -1439,7 +1561,8 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
             if (MODE == MODE_STEP) {
                 // momentum exchange of this link (src/MEA.jl:102-107), FP64, arm to the
                 // wall point x + c_dl/2 (SBB) or x + q c_dl (IBB)
-                const double t = (double)(fin_dl + val);
+                const double t = S::kDev ? (double)fin_dl + (double)val + 2.0 * rest_level<S>(dl)
+                                         : (double)(fin_dl + val);
                 const double P[3] = {t * kCx[dl], t * kCy[dl], t * kCz[dl]};
                 const double ha = rule == 0 ? 0.5 : qw;
                 double ra[3];
@@ -1455,12 +1578,12 @@ __device__ __forceinline__ void general_cell(const typename S::T* __restrict__ f
                 }
                 const double v6[6] = {P[0], P[1], P[2], ra[1] * P[2] - ra[2] * P[1], ra[2] * P[0] - ra[0] * P[2],
                                       ra[0] * P[1] - ra[1] * P[0]};
-                const bool first = acc.b0 < 0 || acc.b0 == aux;
-                if (first) {
+                if (acc.b0 < 0 || acc.b0 == aux) {
                     acc.b0 = aux;
                     for (int c = 0; c < 6; ++c) acc.a0[c] += v6[c];
+                } else {
+                    wave_slot_add(aux, v6, lds_wave);
                 }
-                wave_ordered_add(!first, aux, v6, lds_wave);
             }
         }
         // the interpolated rule's extra loads and FP64 sums: one direction at a time

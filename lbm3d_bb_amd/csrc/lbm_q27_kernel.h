@@ -39,30 +39,50 @@ __device__ inline C w27(int d) {
 }
 
 // 16-bit population storage on D3Q27: the deviation from the rest state
-// f_d - w_d (rho0 = 1) with this lattice's weights, times 2^12 in FP16 (RNE) or
-// as is in BF16 (RNE); FP32 arithmetic.
+// f_d - w_d (rho0 = 1) with this lattice's weights, times 2^12 in FP16 or as is
+// in BF16; the step kernels hold and collide the deviations (kDev), loads
+// return them and step stores take them (stochastic rounding, lbm_lattice.h);
+// the start-state store takes f and rounds to nearest.
 struct StoreF16S27 {
     using T = __half;
     using C = float;
-    __device__ static inline C load(const T* p, int d, const Shift& sh) {
-        return __fmaf_rn(__half2float(*p), sh.ik, w27<float>(d));
-    }
+    static constexpr bool kDev = true;
+    __device__ static inline C load(const T* p, int, const Shift& sh) { return __half2float(*p) * sh.ik; }
     __device__ static inline void store(T* p, int d, C v, const Shift& sh) {
         const __half h = __float2half_rn((v - w27<float>(d)) * sh.k);
         __builtin_nontemporal_store(__half_as_ushort(h), reinterpret_cast<unsigned short*>(p));
+    }
+    __device__ static inline void store_sr(T* p, int, C dv, const Shift& sh, uint32_t r) {
+        const uint32_t h = f16x2_sr(dv * sh.k, 0.0f, r);
+        __builtin_nontemporal_store((unsigned short)(h & 0xffffu), reinterpret_cast<unsigned short*>(p));
     }
 };
 
 struct StoreBF16S27 {
     using T = uint16_t;
     using C = float;
-    __device__ static inline C load(const T* p, int d, const Shift&) {
-        return StoreBF16S::bf16_to_f32(*p) + w27<float>(d);
-    }
+    static constexpr bool kDev = true;
+    __device__ static inline C load(const T* p, int, const Shift&) { return StoreBF16S::bf16_to_f32(*p); }
     __device__ static inline void store(T* p, int d, C v, const Shift&) {
         __builtin_nontemporal_store(StoreBF16S::f32_to_bf16_rne(v - w27<float>(d)), p);
     }
+    __device__ static inline void store_sr(T* p, int, C dv, const Shift&, uint32_t r) {
+        __builtin_nontemporal_store((uint16_t)bf16_sr(dv, r), p);
+    }
 };
+
+// Sums of the FP32 D3Q27 weights (rho_w = sum w_d, s2 = sum w_d c_dx^2; 1 and
+// 1/3 for the exact weights), as for D3Q19 (collide_dev, lbm_device.h).
+constexpr double kRhoW27 = (double)(8.0f / 27.0f) + 6.0 * (double)(2.0f / 27.0f) + 12.0 * (double)(1.0f / 54.0f) +
+                           8.0 * (double)(1.0f / 216.0f);
+constexpr double kS2_27 = 2.0 * (double)(2.0f / 27.0f) + 8.0 * (double)(1.0f / 54.0f) + 8.0 * (double)(1.0f / 216.0f);
+
+// rest level of direction d in the register form of S (w_d for deviations)
+template <class S>
+__device__ inline double rest27(int d) {
+    if constexpr (S::kDev) return (double)w27<float>(d);
+    return 0.0;
+}
 
 template <typename C>
 __device__ inline void moments27(const C f[QQ], C& rho, C& jx, C& jy, C& jz) {
@@ -81,10 +101,20 @@ __device__ inline void moments27(const C f[QQ], C& rho, C& jx, C& jy, C& jz) {
 }
 
 // Previous-step rho, u of a cell from its post-collision populations: the
-// collision keeps rho and adds F to the momentum, so u = j*/rho - g/2.
-template <typename C, bool FORCED>
-__device__ inline void prev_moments(C rho, C jx, C jy, C jz, const Phys<C>& p, C& rho_out, C u[3], C& uu2) {
-    rho_out = rho;
+// collision keeps rho and adds F to the momentum, so u = j*/rho - g/2.  For the
+// shifted formats rho is the sum of the deviations and rho_out receives
+// rho - 1, what their rules take.
+template <class S, bool FORCED>
+__device__ inline void prev_moments(typename S::C rho, typename S::C jx, typename S::C jy, typename S::C jz,
+                                    const Phys<typename S::C>& p, typename S::C& rho_out, typename S::C u[3],
+                                    typename S::C& uu2) {
+    using C = typename S::C;
+    if constexpr (S::kDev) {
+        rho_out = C(kRhoW27 - 1.0) + rho;
+        rho = C(kRhoW27) + rho;
+    } else {
+        rho_out = rho;
+    }
     u[0] = jx / rho;
     u[1] = jy / rho;
     u[2] = jz / rho;
@@ -154,6 +184,78 @@ __device__ inline void collide27(C f[QQ], const Phys<C>& p) {
             if (FORCED) {
                 const C cF = C(kX[d]) * Fx + C(kY[d]) * Fy + C(kZ[d]) * Fz;
                 r += kp * (w * (C(3) * cF + C(9) * cu * cF - C(3) * uF));
+            }
+            f[d] = r;
+        }
+    }
+}
+
+// collide27 on deviations delta = f - w for the shifted formats: the same
+// normalised equilibrium and source as collide_dev (lbm_device.h) with the
+// D3Q27 weights, so the collision keeps the cell's mass and momentum exactly.
+template <int COLL, bool FORCED>
+__device__ inline void collide27_dev(float f[QQ], const Phys<float>& p) {
+    constexpr float rw = (float)kRhoW27, irw = (float)(1.0 / kRhoW27);
+    constexpr float c3 = (float)(1.0 / kS2_27), a2 = (float)(1.5 * kRhoW27 / kS2_27),
+                    a9 = (float)(3.0 * kRhoW27 / kS2_27);
+    float sd, jx, jy, jz;
+    moments27<float>(f, sd, jx, jy, jz);
+    const float rho = rw + sd;
+    const float rm = sd * irw, rh = 1.0f + rm;
+    const float inv = 1.0f / rho;
+    float ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    float Fx = 0.f, Fy = 0.f, Fz = 0.f, uF = 0.f;
+    if (FORCED) {
+        ux += 0.5f * p.g[0];
+        uy += 0.5f * p.g[1];
+        uz += 0.5f * p.g[2];
+        Fx = rho * p.g[0];
+        Fy = rho * p.g[1];
+        Fz = rho * p.g[2];
+        uF = ux * Fx + uy * Fy + uz * Fz;
+    }
+    const float u2 = ux * ux + uy * uy + uz * uz;
+    const float mx = rho * ux, my = rho * uy, mz = rho * uz;
+    const float wp = p.wp;
+    const float kp = 1.0f - 0.5f * wp;
+    if (COLL == LBM_TRT) {
+        const float wm = p.wm;
+        const float km = 1.0f - 0.5f * wm;
+#pragma unroll
+        for (int d = 0; d < QQ - 1; d += 2) {
+            const float w = w27<float>(d);
+            const float cu = float(kX[d]) * ux + float(kY[d]) * uy + float(kZ[d]) * uz;
+            const float cm = float(kX[d]) * mx + float(kY[d]) * my + float(kZ[d]) * mz;
+            const float dp = w * (rm + rh * (a2 * cu * cu - 1.5f * u2));
+            const float dm = w * (c3 * cm);
+            const float fp = 0.5f * (f[d] + f[d + 1]);
+            const float fm = 0.5f * (f[d] - f[d + 1]);
+            float fpn = fp - wp * (fp - dp);
+            float fmn = fm - wm * (fm - dm);
+            if (FORCED) {
+                const float cF = float(kX[d]) * Fx + float(kY[d]) * Fy + float(kZ[d]) * Fz;
+                fpn += kp * (w * (a9 * cu * cF - 3.0f * uF));
+                fmn += km * (w * (c3 * cF));
+            }
+            f[d] = fpn + fmn;
+            f[d + 1] = fpn - fmn;
+        }
+        const float w0 = w27<float>(QQ - 1);
+        const float d0 = w0 * (rm - rh * (1.5f * u2));
+        float r = f[QQ - 1] - wp * (f[QQ - 1] - d0);
+        if (FORCED) r += kp * (w0 * (-3.0f * uF));
+        f[QQ - 1] = r;
+    } else {
+#pragma unroll
+        for (int d = 0; d < QQ; ++d) {
+            const float w = w27<float>(d);
+            const float cu = float(kX[d]) * ux + float(kY[d]) * uy + float(kZ[d]) * uz;
+            const float cm = float(kX[d]) * mx + float(kY[d]) * my + float(kZ[d]) * mz;
+            const float de = w * (rm + rh * (a2 * cu * cu - 1.5f * u2) + c3 * cm);
+            float r = f[d] - wp * (f[d] - de);
+            if (FORCED) {
+                const float cF = float(kX[d]) * Fx + float(kY[d]) * Fy + float(kZ[d]) * Fz;
+                r += kp * (w * (c3 * cF + a9 * cu * cF - 3.0f * uF));
             }
             f[d] = r;
         }
@@ -304,15 +406,25 @@ __device__ inline int face_rule_of(const int q0[3], int d, const Grid& g, RuleFa
 
 // Face rules for direction d whose source lies beyond the chosen face:
 // VELOCITY src/OtherBC.jl:143-148, PRESSURE_ABB src/OtherBC.jl:156-165.
-template <typename C>
-__device__ inline C velocity_rule(int d, C own_opp, const C uf[3], C rho_prev) {
+// For the shifted formats own_opp is a deviation and rho_prev holds rho - 1
+// (prev_moments): the rules in deviation form.
+template <class S>
+__device__ inline typename S::C velocity_rule(int d, typename S::C own_opp, const typename S::C uf[3],
+                                              typename S::C rho_prev) {
+    using C = typename S::C;
+    const C rho = S::kDev ? C(1) + rho_prev : rho_prev;
     const C cu = C(kX[d]) * (-uf[0]) + C(kY[d]) * (-uf[1]) + C(kZ[d]) * (-uf[2]);
-    return own_opp - C(2) * w27<C>(d) * rho_prev * cu * C(3);
+    return own_opp - C(2) * w27<C>(d) * rho * cu * C(3);
 }
 
-template <typename C>
-__device__ inline C abb_rule(int d, C own_opp, const C up[3], C uu2, C rho_prev) {
+template <class S>
+__device__ inline typename S::C abb_rule(int d, typename S::C own_opp, const typename S::C up[3], typename S::C uu2,
+                                         typename S::C rho_prev) {
+    using C = typename S::C;
     const C cu = C(kX[d]) * up[0] + C(kY[d]) * up[1] + C(kZ[d]) * up[2];
+    if constexpr (S::kDev)
+        return -own_opp + C(2) * w27<C>(d) *
+                              (rho_prev + (C(1) + rho_prev) * (C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3)));
     return -own_opp + C(2) * w27<C>(d) * rho_prev * (C(1) + C(0.5) * cu * cu * C(9) - C(0.5) * uu2 * C(3));
 }
 
@@ -357,12 +469,13 @@ __device__ inline uint32_t face_rule_mask(const int q0[3], const Grid& g, RuleFa
 
 // The rule of face FC on its nine directions, for a cell on that face only
 // (f[d] holds the cell's own f*[opp(d)] the rule starts from).
-template <typename C, int FC>
-__device__ inline void one_face_rule(C f[QQ], bool velocity, const C uf[3], const C up[3], C uu2, C rho_prev) {
+template <class S, int FC>
+__device__ inline void one_face_rule(typename S::C f[QQ], bool velocity, const typename S::C uf[3],
+                                     const typename S::C up[3], typename S::C uu2, typename S::C rho_prev) {
 #pragma unroll
     for (int d = 0; d < QQ; ++d) {
         if (!((face_dirs(FC) >> d) & 1u)) continue;
-        f[d] = velocity ? velocity_rule<C>(d, f[d], uf, rho_prev) : abb_rule<C>(d, f[d], up, uu2, rho_prev);
+        f[d] = velocity ? velocity_rule<S>(d, f[d], uf, rho_prev) : abb_rule<S>(d, f[d], up, uu2, rho_prev);
     }
 }
 
@@ -510,14 +623,15 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                         if (kZ[e] < 0) jz -= v;
                     }
                     C rr, uu[3], u2;
-                    prev_moments<C, FORCED>(r, jx, jy, jz, p, rr, uu, u2);
+                    prev_moments<S, FORCED>(r, jx, jy, jz, p, rr, uu, u2);
                     sr += rr;
                     su += uu[0];
                     sv += uu[1];
                     sw += uu[2];
                     ++cnt;
                 }
-                C r = C(g.rho0), ux = C(0), uy = C(0), uz = C(0);
+                // (shifted formats: sr sums rho - 1, and f holds f_eq - w)
+                C r = S::kDev ? C(g.rho0 - 1.0) : C(g.rho0), ux = C(0), uy = C(0), uz = C(0);
                 if (cnt > 0) {
                     const C n = C(cnt);
                     r = sr / n;
@@ -526,7 +640,15 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                     uz = sw / n;
                 }
 #pragma unroll
-                for (int d = 0; d < QQ; ++d) f[d] = feq27<C>(d, r, ux, uy, uz);
+                for (int d = 0; d < QQ; ++d) {
+                    if constexpr (S::kDev) {
+                        const C cu = C(kX[d]) * ux + C(kY[d]) * uy + C(kZ[d]) * uz;
+                        const C u2 = ux * ux + uy * uy + uz * uz;
+                        f[d] = w27<C>(d) * (r + (C(1) + r) * (cu * C(3) + C(0.5) * (cu * cu) * C(9) - C(0.5) * u2 * C(3)));
+                    } else {
+                        f[d] = feq27<C>(d, r, ux, uy, uz);
+                    }
+                }
             } else {
                 const uint32_t rmask = fmask | lmask;
                 // A cell on one rule face and with no link (an x-face cell shares
@@ -550,11 +672,11 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                             if (kZ[d] > 0) jz += v;
                             if (kZ[d] < 0) jz -= v;
                         }
-                        prev_moments<C, FORCED>(r, jx, jy, jz, p, rho_prev, up, uu2);
+                        prev_moments<S, FORCED>(r, jx, jy, jz, p, rho_prev, up, uu2);
                     } else {
 #pragma unroll
                         for (int d = 0; d < QQ; ++d) r += S::load(fin + (int64_t)d * g.P + cell, d, g.sh);
-                        rho_prev = r;
+                        rho_prev = S::kDev ? C(kRhoW27 - 1.0) + r : r;  // rho - 1 for the shifted formats
                     }
                 }
                 // pulls; a rule direction loads the cell's own f*[opp(d)] instead,
@@ -575,12 +697,12 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                     const C uf[3] = {C(face_u[3 * fc]), C(face_u[3 * fc + 1]), C(face_u[3 * fc + 2])};
                     const bool vel = (rf.vel >> fc) & 1u;
                     switch (fc) {
-                        case 0: one_face_rule<C, 0>(f, vel, uf, up, uu2, rho_prev); break;
-                        case 1: one_face_rule<C, 1>(f, vel, uf, up, uu2, rho_prev); break;
-                        case 2: one_face_rule<C, 2>(f, vel, uf, up, uu2, rho_prev); break;
-                        case 3: one_face_rule<C, 3>(f, vel, uf, up, uu2, rho_prev); break;
-                        case 4: one_face_rule<C, 4>(f, vel, uf, up, uu2, rho_prev); break;
-                        default: one_face_rule<C, 5>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 0: one_face_rule<S, 0>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 1: one_face_rule<S, 1>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 2: one_face_rule<S, 2>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 3: one_face_rule<S, 3>(f, vel, uf, up, uu2, rho_prev); break;
+                        case 4: one_face_rule<S, 4>(f, vel, uf, up, uu2, rho_prev); break;
+                        default: one_face_rule<S, 5>(f, vel, uf, up, uu2, rho_prev); break;
                     }
                 } else if (rmask) {
                     // wall point of the moving-wall velocity: physical x + c_dl/2 or
@@ -607,8 +729,8 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                             bool vel;
                             const int ch = face_rule_of(q0, d, g, rf, &vel);
                             const C uf[3] = {C(face_u[3 * ch]), C(face_u[3 * ch + 1]), C(face_u[3 * ch + 2])};
-                            if (vel) f[d] = velocity_rule<C>(d, own_opp, uf, rho_prev);
-                            else f[d] = abb_rule<C>(d, own_opp, up, uu2, rho_prev);
+                            if (vel) f[d] = velocity_rule<S>(d, own_opp, uf, rho_prev);
+                            else f[d] = abb_rule<S>(d, own_opp, up, uu2, rho_prev);
                         } else {
                             // boundary link (x, dl), dl = opp(d), solid node x + c_dl
                             // (SBB!, src/SolidBC.jl:187-208)
@@ -639,7 +761,7 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                             const C uwy = C(v[1] + (om[2] * r[0] - om[0] * r[2]));
                             const C uwz = C(v[2] + (om[0] * r[1] - om[1] * r[0]));
                             const C cuw = C(cdl[0]) * uwx + C(cdl[1]) * uwy + C(cdl[2]) * uwz;
-                            const C corr = C(2) * w * rho_prev * cuw * C(3);
+                            const C corr = C(2) * w * (S::kDev ? C(1) + rho_prev : rho_prev) * cuw * C(3);
                             C val = own_opp - corr;
                             if (IBB && rule != 0) {
                                 // rule 1: f*_dl at the second fluid node x - c_dl; rule 2: the
@@ -656,7 +778,8 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                             if (MODE == MODE_STEP) {
                                 // momentum exchange (src/MEA.jl:102-107) in FP64, torque arm to the
                                 // wall point x + c_dl/2 (simple) or x + q c_dl (interpolated)
-                                const double t = (double)(own_opp + val);
+                                const double t = S::kDev ? (double)own_opp + (double)val + 2.0 * rest27<S>(d)
+                                                         : (double)(own_opp + val);
                                 const double P[3] = {t * cdl[0], t * cdl[1], t * cdl[2]};
                                 const double ha = rule == 0 ? 0.5 : qw;
                                 double ra[3];
@@ -665,13 +788,13 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                                     ra[a] = wrap_arm((double)q0[a] + 0.5 + ha * (double)cdl[a] - ch[a], a, g);
                                 const double v6[6] = {P[0], P[1], P[2], ra[1] * P[2] - ra[2] * P[1],
                                                       ra[2] * P[0] - ra[0] * P[2], ra[0] * P[1] - ra[1] * P[0]};
-                                const bool first = acc.b0 < 0 || acc.b0 == bi;
-                                if (first) {
+                                if (acc.b0 < 0 || acc.b0 == bi) {
                                     acc.b0 = bi;
 #pragma unroll
                                     for (int c = 0; c < 6; ++c) acc.a0[c] += v6[c];
+                                } else {
+                                    wave_slot_add(bi, v6, s_wave);
                                 }
-                                wave_ordered_add(!first, bi, v6, s_wave);
                             }
                         }
                     }
@@ -682,6 +805,7 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
             // f_old and its moments u = sum f c / rho + g/2 (src/ComputeMacro.jl:26-45)
             C rho, jx, jy, jz;
             moments27<C>(f, rho, jx, jy, jz);
+            if (S::kDev) rho = C(kRhoW27) + rho;  // the weights carry no momentum
             C ux = jx / rho, uy = jy / rho, uz = jz / rho;
             if (FORCED) {
                 ux += C(0.5) * p.g[0];
@@ -690,7 +814,7 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
             }
             if (gout) {
 #pragma unroll
-                for (int d = 0; d < QQ; ++d) gout[(int64_t)d * N + cell] = (double)f[d];
+                for (int d = 0; d < QQ; ++d) gout[(int64_t)d * N + cell] = (double)f[d] + rest27<S>(d);
             }
             if (rho_out) rho_out[cell] = (double)rho;
             if (u_out) {
@@ -699,9 +823,11 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                 u_out[2 * N + cell] = (double)uz;
             }
         } else {
-            collide27<C, COLL, FORCED>(f, p);
+            if constexpr (S::kDev) collide27_dev<COLL, FORCED>(f, p);
+            else collide27<C, COLL, FORCED>(f, p);
+            const uint32_t h = sr_cell(cell, g.sh.seed);
 #pragma unroll
-            for (int d = 0; d < QQ; ++d) S::store(fout + (int64_t)d * g.P + cell, d, f[d], g.sh);
+            for (int d = 0; d < QQ; ++d) S::store_sr(fout + (int64_t)d * g.P + cell, d, f[d], g.sh, sr_dir(h, d));
         }
     }
     if (MODE == MODE_STEP && bnear) {

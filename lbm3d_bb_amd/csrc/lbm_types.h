@@ -60,7 +60,8 @@ struct Box {
 // rounding as before).
 struct Shift {
     float k;
-    float ik;  // 1 / k
+    float ik;       // 1 / k
+    uint32_t seed;  // per step: random stream of the stochastic rounding of stores
 };
 
 struct Grid {

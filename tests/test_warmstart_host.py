@@ -67,3 +67,26 @@ def test_drag_monitor_needs_consecutive_small_residuals():
     seq = [1.0, 1.5, 1.2, 1.20001, 1.200011, 1.3, 1.30001, 1.300011, 1.300012]
     hits = [mon.update(100 * (i + 1), v) for i, v in enumerate(seq)]
     assert hits[-1] and not any(hits[:-1])
+
+
+def test_offline_rule_matches_the_drag_monitor():
+    """warmstart_ceiling evaluates the residual rule on recorded per-step drags:
+    it must stop where DragMonitor, fed the same window means, stops."""
+    from lbm3d_bb_amd.convergence import DragMonitor
+    from lbm3d_bb_amd.warmstart_ceiling import smooth_noise, steps_to_converge
+
+    t = np.arange(40000)
+    rng = np.random.default_rng(1)
+    d = 1.0 + 0.3 * np.exp(-t / 3000.0) + 0.01 * np.exp(-t / 900.0) * np.sin(2 * np.pi * t / 443.0)
+    d += 1e-7 * rng.standard_normal(t.size)
+    for every in (100, 500):
+        m = DragMonitor(every=every)
+        stop = None
+        for i in range(1, t.size // every + 1):
+            if m.update(i * every, d[(i - 1) * every:i * every].mean()):
+                stop = i * every
+                break
+        assert stop is not None and steps_to_converge(d, every) == stop
+    assert steps_to_converge(np.ones(150), 100) is None  # one window: nothing to compare
+    a = smooth_noise((16, 16, 16), rng, 3.0)
+    assert abs(a.mean()) < 1e-12 and abs(a.std() - 1.0) < 1e-12
