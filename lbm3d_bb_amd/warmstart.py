@@ -377,22 +377,29 @@ def solve_steady(case, net: UNet | None = None, use_net: bool = False, tol: floa
     return res
 
 
-def generate_dataset(seeds, L: int, log=None, **kw) -> dict:
+def generate_dataset(seeds, L: int, log=None, budget_s: float | None = None, **kw) -> dict:
     """Steady FP32 fields of the packs ``seeds`` from uniform starts: network
-    inputs X (n, 2, L, L, L), targets Y (n, 4, L, L, L) and the plain
-    steps-to-converged-drag of every sample.  Packs whose run does not reach a
-    converged, finite drag (an unsteady wake) are left out of X and Y."""
+    inputs X (n, 2, L, L, L), targets Y (n, 4, L, L, L) (both held in FP16 on
+    the host) and the plain steps-to-converged-drag of every sample.  Packs
+    whose run does not reach a converged, finite drag (an unsteady wake) are
+    left out of X and Y.  With budget_s, generation stops once that many
+    seconds have passed."""
     from .packs import make_pack
 
     seeds = list(seeds)
     n = len(seeds)
-    X = np.empty((n, 2, L, L, L), dtype=np.float32)
-    Y = np.empty((n, 4, L, L, L), dtype=np.float32)
+    X = np.empty((n, 2, L, L, L), dtype=np.float16)
+    Y = np.empty((n, 4, L, L, L), dtype=np.float16)
     steps = np.zeros(n, dtype=np.int64)
     conv = np.zeros(n, dtype=bool)
     re = np.zeros(n)
     k = 0
+    t0 = time.perf_counter()
+    done = 0
     for i, sd in enumerate(seeds):
+        if budget_s is not None and time.perf_counter() - t0 > budget_s:
+            break
+        done = i + 1
         case = make_pack(int(sd), L)
         r = solve_steady(case, fields=True, **kw)
         ok = bool(r["converged"]) and bool(np.all(np.isfinite(r["rho"]))) and bool(np.all(np.isfinite(r["u"])))
@@ -404,7 +411,8 @@ def generate_dataset(seeds, L: int, log=None, **kw) -> dict:
         if log:
             log(f"sample {i + 1}/{n} seed {sd} Re {case.re:.2f} spheres {len(case.bodies)} "
                 f"steps {r['steps']} converged {r['converged']}{'' if ok else ' (left out)'}")
-    return {"X": X[:k], "Y": Y[:k], "steps": steps, "converged": conv, "re": re, "seeds": np.asarray(seeds)}
+    return {"X": X[:k], "Y": Y[:k], "steps": steps[:done], "converged": conv[:done], "re": re[:done],
+            "seeds": np.asarray(seeds[:done])}
 
 
 def augment(x: np.ndarray, y: np.ndarray, rng) -> tuple[np.ndarray, np.ndarray]:
@@ -444,11 +452,12 @@ def train(net: UNet, X: np.ndarray, Y: np.ndarray, epochs: int, batch: int, lr: 
         losses = []
         for k in range(per_epoch):
             idx = np.sort(order[k * batch:(k + 1) * batch])
-            xb, yb = X[idx], Y[idx]
             if augment_data:
-                pairs = [augment(X[i], Y[i], rng) for i in idx]
+                pairs = [augment(X[i].astype(np.float32), Y[i].astype(np.float32), rng) for i in idx]
                 xb = np.stack([p[0] for p in pairs])
                 yb = np.stack([p[1] for p in pairs])
+            else:
+                xb, yb = X[idx].astype(np.float32), Y[idx].astype(np.float32)
             net.set_lr(lr * 0.5 * (1.0 + np.cos(np.pi * t / total)) + 1e-6)
             losses.append(net.train_step(xb, yb))
             t += 1
@@ -458,27 +467,43 @@ def train(net: UNet, X: np.ndarray, Y: np.ndarray, epochs: int, batch: int, lr: 
     return hist
 
 
+def prediction_error(net: UNet, case, rho: np.ndarray, u: np.ndarray) -> dict:
+    """RMS error over the fluid cells of the network's prediction against a
+    steady field: velocity in units of the inlet speed (per component), and
+    rho - 1 relative to the rms of the true rho - 1."""
+    rp, up = predict(net, case)
+    fluid = ~case.mask().reshape(-1)
+    du = (up - u)[:, fluid] / case.u_in
+    dr = (rp - rho)[fluid]
+    return {"u_rms": float(np.sqrt((du * du).mean())),
+            "rho_rel_rms": float(np.sqrt((dr * dr).mean()) / max(np.sqrt(((rho[fluid] - 1.0) ** 2).mean()), 1e-300))}
+
+
 def evaluate(net: UNet, seeds, L: int, log=None, **kw) -> dict:
     """Steps to a converged drag from the uniform and from the network start on
-    held-out packs, and the relative difference of the two converged drags."""
+    held-out packs, the relative difference of the two converged drags, and the
+    network's prediction error against the converged field of the uniform run."""
     from .packs import make_pack
 
     rows = []
     for i, sd in enumerate(seeds):
         case = make_pack(int(sd), L)
-        ru = solve_steady(case, **kw)
+        ru = solve_steady(case, fields=True, **kw)
         rn = solve_steady(case, net=net, use_net=True, **kw)
+        pe = prediction_error(net, case, ru.pop("rho"), ru.pop("u"))
         d_rel = abs(rn["drag"] - ru["drag"]) / max(abs(ru["drag"]), 1e-300)
         rows.append({"seed": int(sd), "re": case.re, "spheres": len(case.bodies), "steps_uniform": ru["steps"],
                      "steps_net": rn["steps"], "converged_uniform": ru["converged"],
                      "converged_net": rn["converged"], "start": rn["start"], "drag_uniform": ru["drag"],
-                     "drag_net": rn["drag"], "drag_rel_diff": d_rel})
+                     "drag_net": rn["drag"], "drag_rel_diff": d_rel, "pred_u_rms": pe["u_rms"],
+                     "pred_rho_rel_rms": pe["rho_rel_rms"]})
         if log:
             log(f"eval {i + 1}/{len(seeds)} seed {sd} Re {case.re:.2f} steps uniform {ru['steps']} net {rn['steps']} "
-                f"({rn['start']}) drag diff {d_rel:.2e}")
+                f"({rn['start']}) drag diff {d_rel:.2e} pred err u {pe['u_rms']:.3f} rho {pe['rho_rel_rms']:.3f}")
     su = np.array([r["steps_uniform"] for r in rows], dtype=np.float64)
     sn = np.array([r["steps_net"] for r in rows], dtype=np.float64)
     return {"rows": rows, "mean_steps_uniform": float(su.mean()), "mean_steps_net": float(sn.mean()),
             "speedup_of_means": float(su.mean() / sn.mean()), "median_ratio": float(np.median(su / sn)),
+            "median_pred_u_rms": float(np.median([r["pred_u_rms"] for r in rows])),
             "max_drag_rel_diff": float(max(r["drag_rel_diff"] for r in rows)),
             "net_starts": int(sum(r["start"] == "net" for r in rows))}

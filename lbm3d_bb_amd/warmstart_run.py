@@ -1,6 +1,8 @@
 """The learned warm-start workload end to end on one GPU.
 
   python -m lbm3d_bb_amd.warmstart_run --size 128 --train 2000 --eval 200 --out DIR
+  python -m lbm3d_bb_amd.warmstart_run --size 128 --train 1000 --train-from 1000 --params P --eval 0 --out DIR
+  python -m lbm3d_bb_amd.warmstart_run --size 128 --train 0 --params P --eval 200 --out DIR
   python -m lbm3d_bb_amd.warmstart_run --size 64 --batch 4 --timing 10   (network timing only)
 
 1. generates steady FP32 fields of seeded sphere packs with the solver from
@@ -11,8 +13,10 @@
    and from the network's prediction, with the same residual rule, and reports
    the steps of both and the difference of the converged drags.
 
-Writes DIR/summary.json and DIR/unet_params.npy; prints one progress line per
-sample and epoch.
+Training can continue over several runs (--params from the last run,
+--train-from to take the next seeds), each generating its own fields; with
+--train 0 only the evaluation runs.  Writes DIR/summary.json and
+DIR/unet_params.npy; prints one progress line per sample and epoch.
 """
 from __future__ import annotations
 
@@ -33,8 +37,11 @@ HELD_OUT_BASE = 1_000_000
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--size", type=int, default=128)
-    ap.add_argument("--train", type=int, default=2000)
-    ap.add_argument("--eval", type=int, default=200)
+    ap.add_argument("--train", type=int, default=2000, help="training packs (seeds --train-from ...)")
+    ap.add_argument("--train-from", type=int, default=0, help="first training seed")
+    ap.add_argument("--gen-budget", type=float, default=None, metavar="S",
+                    help="stop generating training fields after S seconds")
+    ap.add_argument("--eval", type=int, default=200, help="held-out packs to evaluate (0: none)")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -56,37 +63,40 @@ def main(argv=None) -> int:
 
     L = args.size
     kw = dict(tol=args.tol, every=args.every, max_steps=args.max_steps)
-    t0 = time.perf_counter()
-    data = generate_dataset(range(args.train), L, log=log, **kw)
-    t_gen = time.perf_counter() - t0
     net = UNet((L, L, L), batch=args.batch, levels=4, base=16, seed=0, lr=args.lr)
     if args.params:
         net.load(args.params)
-    t0 = time.perf_counter()
-    hist = train(net, data["X"], data["Y"], args.epochs, args.batch, lr=args.lr, log=log)
-    t_train = time.perf_counter() - t0
-    ms, flops = net.last_timing()
-    net.save(os.path.join(args.out, "unet_params.npy"))
-    used = int(data["X"].shape[0])
-    del data
-    t0 = time.perf_counter()
-    ev = evaluate(net, range(HELD_OUT_BASE, HELD_OUT_BASE + args.eval), L, log=log, **kw)
-    t_eval = time.perf_counter() - t0
     summary = {
         "workload": f"{L}^3 sphere packs (4-32 spheres, r in [5,12]*{L}/128), Re uniform in "
                     f"[1,{re_max(L):.0f}], inlet u={U_IN}, zero-gradient outlet, periodic y/z, TRT 3/16, FP32; "
                     f"drag residual {args.tol} over {args.every}-step window means, 3 in a row",
-        "train_samples": args.train, "train_samples_used": used, "eval_samples": args.eval, "epochs": args.epochs,
-        "batch": args.batch,
-        "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
-        "train_step_ms": ms, "train_step_tflops": flops / ms / 1e9 if ms > 0 else None,
-        "seconds": {"generate": t_gen, "train": t_train, "evaluate": t_eval},
-        **{k: v for k, v in ev.items() if k != "rows"},
-        "rows": ev["rows"],
+        "params_in": args.params, "batch": args.batch, "epochs": args.epochs, "seconds": {},
     }
+    if args.train > 0:
+        t0 = time.perf_counter()
+        data = generate_dataset(range(args.train_from, args.train_from + args.train), L, log=log,
+                                budget_s=args.gen_budget, **kw)
+        summary["seconds"]["generate"] = time.perf_counter() - t0
+        used = int(data["X"].shape[0])
+        summary.update(train_seeds=[int(args.train_from), int(args.train_from + len(data["seeds"]))],
+                       train_samples_used=used, uniform_steps_median=float(np.median(data["steps"])))
+        t0 = time.perf_counter()
+        hist = train(net, data["X"], data["Y"], args.epochs, args.batch, lr=args.lr, log=log)
+        summary["seconds"]["train"] = time.perf_counter() - t0
+        ms, flops = net.last_timing()
+        summary.update(loss_first_epoch=hist[0], loss_last_epoch=hist[-1], loss_per_epoch=hist, train_step_ms=ms,
+                       train_step_tflops=flops / ms / 1e9 if ms > 0 else None)
+        del data
+        net.save(os.path.join(args.out, "unet_params.npy"))
+    if args.eval > 0:
+        t0 = time.perf_counter()
+        ev = evaluate(net, range(HELD_OUT_BASE, HELD_OUT_BASE + args.eval), L, log=log, **kw)
+        summary["seconds"]["evaluate"] = time.perf_counter() - t0
+        summary.update({k: v for k, v in ev.items() if k != "rows"})
+        summary["rows"] = ev["rows"]
     with open(os.path.join(args.out, "summary.json"), "w", encoding="utf-8") as fh:
         json.dump(summary, fh, indent=1)
-    log(json.dumps({k: v for k, v in summary.items() if k != "rows"}))
+    log(json.dumps({k: v for k, v in summary.items() if k not in ("rows", "loss_per_epoch")}))
     return 0
 
 
