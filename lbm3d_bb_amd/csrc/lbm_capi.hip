@@ -137,6 +137,15 @@ int fail(lbm_ctx* c, int code, const std::string& msg) {
             return fail((ctx), LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
     } while (0)
 
+// For calls made while temporaries are held: a failure is recorded in rc (the
+// first one wins) and the caller goes on to free what it allocated.
+#define HIPKEEP(ctx, rc, call)                                                                      \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess && (rc) == LBM_OK)                                                     \
+            (rc) = fail((ctx), LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
 int dev_alloc(lbm_ctx* c, void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes ? bytes : 16);
     if (e != hipSuccess) return fail(c, LBM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -581,11 +590,13 @@ int full_geometry(lbm_ctx* c, const std::vector<DevBody>& bodies) {
     int rc = dev_alloc(c, (void**)&d_b, std::max<size_t>(bodies.size(), 1) * sizeof(DevBody));
     if (rc) return rc;
     if (!bodies.empty())
-        HIPCHK(c, hipMemcpyAsync(d_b, bodies.data(), bodies.size() * sizeof(DevBody), hipMemcpyHostToDevice, c->stream));
-    GEOCHK(c, launch_flags_full(c->flags, d_b, (int)bodies.size(), c->g, c->stream));
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPKEEP(c, rc, hipMemcpyAsync(d_b, bodies.data(), bodies.size() * sizeof(DevBody), hipMemcpyHostToDevice,
+                                      c->stream));
+    if (rc == LBM_OK) HIPKEEP(c, rc, (hipError_t)launch_flags_full(c->flags, d_b, (int)bodies.size(), c->g, c->stream));
+    HIPKEEP(c, rc, hipGetLastError());
+    HIPKEEP(c, rc, hipStreamSynchronize(c->stream));
     dev_free(c, d_b, std::max<size_t>(bodies.size(), 1) * sizeof(DevBody));
+    if (rc) return rc;
     c->mask_bodies = bodies;
     c->mask_reach = c->reach;
     c->geom_valid = true;
@@ -1459,15 +1470,13 @@ int lbm_get_links(lbm_ctx* c, int32_t* cell, int8_t* dir, int64_t cap, int64_t* 
     int32_t* d_cell = nullptr;
     int8_t* d_dir = nullptr;
     rc = dev_alloc(c, (void**)&d_cell, (size_t)std::max<int64_t>(need, 1) * 4);
-    if (rc) return rc;
-    rc = dev_alloc(c, (void**)&d_dir, (size_t)std::max<int64_t>(need, 1));
-    if (rc) return rc;
+    if (rc == LBM_OK) rc = dev_alloc(c, (void**)&d_dir, (size_t)std::max<int64_t>(need, 1));
     int64_t have = need;
-    rc = build_list(c, d_cell, d_dir, &have);
+    if (rc == LBM_OK) rc = build_list(c, d_cell, d_dir, &have);
     if (rc == LBM_OK) {
-        if (cell && need) HIPCHK(c, hipMemcpyAsync(cell, d_cell, (size_t)need * 4, hipMemcpyDeviceToHost, c->stream));
-        if (dir && need) HIPCHK(c, hipMemcpyAsync(dir, d_dir, (size_t)need, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (cell && need) HIPKEEP(c, rc, hipMemcpyAsync(cell, d_cell, (size_t)need * 4, hipMemcpyDeviceToHost, c->stream));
+        if (dir && need) HIPKEEP(c, rc, hipMemcpyAsync(dir, d_dir, (size_t)need, hipMemcpyDeviceToHost, c->stream));
+        HIPKEEP(c, rc, hipStreamSynchronize(c->stream));
     }
     dev_free(c, d_cell, (size_t)std::max<int64_t>(need, 1) * 4);
     dev_free(c, d_dir, (size_t)std::max<int64_t>(need, 1));
@@ -1482,14 +1491,13 @@ int lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* u) {
     int rc = LBM_OK;
     if (rho) rc = dev_alloc(c, (void**)&d_rho, (size_t)N * 8);
     if (rc == LBM_OK && u) rc = dev_alloc(c, (void**)&d_u, (size_t)3 * N * 8);
-    if (rc) return rc;
-    if (rho) HIPCHK(c, hipMemcpyAsync(d_rho, rho, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
-    if (u) HIPCHK(c, hipMemcpyAsync(d_u, u, (size_t)3 * N * 8, hipMemcpyHostToDevice, c->stream));
-    if (c->q == 27) {
+    if (rc == LBM_OK && rho) HIPKEEP(c, rc, hipMemcpyAsync(d_rho, rho, (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+    if (rc == LBM_OK && u) HIPKEEP(c, rc, hipMemcpyAsync(d_u, u, (size_t)3 * N * 8, hipMemcpyHostToDevice, c->stream));
+    if (rc == LBM_OK && c->q == 27) {
         const int e = q27_launch_equilibrium(c->cfg.storage, c->g, c->stream, c->fbuf[1 - c->cur], d_rho, d_u,
                                              c->cfg.rho0, c->cfg.u0);
         if (e) rc = fail(c, LBM_ERR_HIP, std::string("equilibrium: ") + hipGetErrorString((hipError_t)e));
-    } else {
+    } else if (rc == LBM_OK) {
         LaunchReq r = base_req(c, K_EQUILIBRIUM, 0);
         r.fout = c->fbuf[1 - c->cur];
         r.src_rho = d_rho;
@@ -1499,7 +1507,7 @@ int lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* u) {
         rc = do_launch(c, r);
     }
     if (rc == LBM_OK) rc = collide_only(c);
-    if (rc == LBM_OK) HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPKEEP(c, rc, hipStreamSynchronize(c->stream));
     dev_free(c, d_rho, rho ? (size_t)N * 8 : 0);
     dev_free(c, d_u, u ? (size_t)3 * N * 8 : 0);
     if (rc) return rc;
@@ -1517,18 +1525,18 @@ int lbm_set_populations(lbm_ctx* c, const double* f) {
     const size_t bytes = (size_t)c->q * N * 8;
     int rc = dev_alloc(c, (void**)&d, bytes);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(d, f, bytes, hipMemcpyHostToDevice, c->stream));
-    if (c->q == 27) {
+    HIPKEEP(c, rc, hipMemcpyAsync(d, f, bytes, hipMemcpyHostToDevice, c->stream));
+    if (rc == LBM_OK && c->q == 27) {
         const int e = q27_launch_from_double(c->cfg.storage, c->g, c->stream, c->fbuf[1 - c->cur], d);
         if (e) rc = fail(c, LBM_ERR_HIP, std::string("populations: ") + hipGetErrorString((hipError_t)e));
-    } else {
+    } else if (rc == LBM_OK) {
         LaunchReq r = base_req(c, K_FROM_DOUBLE, 0);
         r.fout = c->fbuf[1 - c->cur];
         r.src_f = d;
         rc = do_launch(c, r);
     }
     if (rc == LBM_OK) rc = collide_only(c);
-    if (rc == LBM_OK) HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPKEEP(c, rc, hipStreamSynchronize(c->stream));
     dev_free(c, d, bytes);
     if (rc) return rc;
     c->step = 0;
@@ -1606,9 +1614,9 @@ int lbm_get_macros(lbm_ctx* c, double* rho, double* u) {
     if (rc == LBM_OK) rc = dev_alloc(c, (void**)&d_u, (size_t)3 * N * 8);
     if (rc == LBM_OK) rc = get_state(c, nullptr, d_rho, d_u);
     if (rc == LBM_OK) {
-        if (rho) HIPCHK(c, hipMemcpyAsync(rho, d_rho, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
-        if (u) HIPCHK(c, hipMemcpyAsync(u, d_u, (size_t)3 * N * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (rho) HIPKEEP(c, rc, hipMemcpyAsync(rho, d_rho, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
+        if (u) HIPKEEP(c, rc, hipMemcpyAsync(u, d_u, (size_t)3 * N * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPKEEP(c, rc, hipStreamSynchronize(c->stream));
     }
     dev_free(c, d_rho, (size_t)N * 8);
     dev_free(c, d_u, (size_t)3 * N * 8);
@@ -1623,8 +1631,8 @@ int lbm_get_populations(lbm_ctx* c, double* f) {
     int rc = dev_alloc(c, (void**)&d, bytes);
     if (rc == LBM_OK) rc = get_state(c, d, nullptr, nullptr);
     if (rc == LBM_OK) {
-        HIPCHK(c, hipMemcpyAsync(f, d, bytes, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPKEEP(c, rc, hipMemcpyAsync(f, d, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPKEEP(c, rc, hipStreamSynchronize(c->stream));
     }
     dev_free(c, d, bytes);
     return rc;
@@ -1641,9 +1649,11 @@ int lbm_conservation(lbm_ctx* c, double* mass, double* mom3) {
     if (rc == LBM_OK) rc = get_state(c, nullptr, d_rho, d_u);
     std::vector<double> h((size_t)nblk * 4);
     if (rc == LBM_OK) {
-        GEOCHK(c, launch_conservation(d_rho, d_u, c->flags, N, d_p, c->stream));
-        HIPCHK(c, hipMemcpyAsync(h.data(), d_p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPKEEP(c, rc, (hipError_t)launch_conservation(d_rho, d_u, c->flags, N, d_p, c->stream));
+        if (rc == LBM_OK) HIPKEEP(c, rc, hipMemcpyAsync(h.data(), d_p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPKEEP(c, rc, hipStreamSynchronize(c->stream));
+    }
+    if (rc == LBM_OK) {
         double s[4] = {0, 0, 0, 0};
         for (int64_t b = 0; b < nblk; ++b)
             for (int k = 0; k < 4; ++k) s[k] += h[(size_t)b * 4 + k];

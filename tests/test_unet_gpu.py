@@ -78,3 +78,31 @@ def test_training_reduces_loss():
     for _ in range(60):
         last = net.train_step(x, t)
     assert last < 0.5 * first, (first, last)
+
+
+@pytest.mark.parametrize("base", [32, 64])
+def test_wide_channel_tiles_match_torch(base):
+    """At 32^3 and batch 4 the finest level has 131072 rows (512 blocks of
+    256), so its convolutions keep the 32-channel (base 32) and 64-channel
+    (base 64) output tiles instead of narrowing to 16 channels; the smaller
+    shapes above only reach the 16-channel tile."""
+    import torch
+
+    from unet_reference import forward, loss_and_grad
+
+    shape = (32, 32, 32)
+    net = _net(shape, batch=4, levels=2, base=base, seed=11)
+    x, t = _data(shape, 4, seed=13)
+    y = net.forward(x)
+    p = torch.tensor(net.get_params(), dtype=torch.float32)
+    yr = forward(p, net.layers(), torch.tensor(x), 2).double().numpy()
+    assert np.abs(yr).max() > 1e-3
+    assert _rel(y, yr) < 2e-2, _rel(y, yr)
+    if base == 32:
+        loss, g = net.gradient(x, t)
+        lr, gr, _ = loss_and_grad(net.get_params(), net.layers(), x, t, 2)
+        assert abs(loss - lr) / lr < 1e-2
+        for i, L in enumerate(net.layers()):
+            nw = (8 if L.kind == 1 else 1) * L.cout * (27 if L.kind == 0 else 1) * L.cin
+            assert _rel(g[L.w_off:L.w_off + nw], gr[L.w_off:L.w_off + nw]) < 5e-2, (i, L)
+            assert _rel(g[L.b_off:L.b_off + L.cout], gr[L.b_off:L.b_off + L.cout]) < 5e-2, (i, L)
