@@ -318,10 +318,10 @@ def drag_err_vs_fp64(w, F, args, cfg, laws, bodies) -> dict:
 
 
 def net_start(s, cfg, laws, bodies, params) -> dict:
-    """The 4-level 16/32/64/128-channel U-Net on 128^3 tiles (halo 16) from
+    """The pack network (4 levels, 16/32/64/128 channels, warmstart.PACK_NET) on 128^3 tiles (halo 16) from
     the solver's solid flags; Re = |u0| D / nu with D the smallest diameter of
     the first body.  Returns the start taken and the inference time."""
-    from lbm3d_bb_amd.warmstart import UNet, warm_start_tiled
+    from lbm3d_bb_amd.warmstart import pack_net, warm_start_tiled
 
     u_ref = float(np.linalg.norm(cfg.u0))
     if u_ref == 0.0:
@@ -330,7 +330,7 @@ def net_start(s, cfg, laws, bodies, params) -> dict:
     nu = (1.0 / cfg.omega_plus - 0.5) / 3.0
     re = u_ref * 2.0 * min(semi) / nu
     tile = tuple(min(128, n) for n in (cfg.nz, cfg.ny, cfg.nx))
-    net = UNet(tile, batch=2, levels=4, base=16, seed=0)
+    net = pack_net(tile, batch=2, seed=0)
     try:
         if params:
             net.load(params)

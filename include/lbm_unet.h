@@ -35,6 +35,9 @@ typedef struct {
     int32_t levels;      /* 2..5 */
     uint64_t seed;       /* weight initialisation */
     float lr, beta1, beta2, eps;
+    int32_t periodic;    /* bit 0 x, bit 1 y, bit 2 z: the 3x3x3 convolutions wrap around along these axes
+                            instead of padding with zeros (a periodic domain); needs every level at least
+                            16 x 4 x 4 voxels (x, y, z), else lbm_unet_create fails */
 } lbm_unet_config;
 
 typedef struct lbm_unet lbm_unet;

@@ -143,6 +143,7 @@ int conv(lbm_unet* n, int geom, const Buf& x, int xco, int cin, const bf16* w, i
     a.in = in;
     a.out = out;
     a.m = (int64_t)rows.n * rows.d * rows.h * rows.w;
+    a.wrap = geom == G_CONV3 ? (n->cfg.periodic & 7) : 0;
     // k_conv indexes voxels in 32 bits and takes a same-size 3x3x3 convolution's rows on the input grid
     const int64_t nin = (int64_t)in.n * in.d * in.h * in.w;
     if (nin >= (int64_t)1 << 31 || a.m >= (int64_t)1 << 31) return fail(n, -2, "convolution tensor of 2^31 voxels or more");
@@ -155,6 +156,34 @@ int conv(lbm_unet* n, int geom, const Buf& x, int xco, int cin, const bf16* w, i
     // nt in {1, 2, 4} divides coutp / 16, so the grid's y extent covers every channel
     int nt = 4;
     while (nt > 1 && (coutp / 16) % nt != 0) nt >>= 1;
+    // same-size 3x3x3 convolutions on grids at least one brick wide take the
+    // LDS-staged kernel when the input channels fit its chunking (8, 16 or a
+    // multiple of 32)
+    const int ck = cin == 8 ? 8 : (cin == 16 ? 16 : (cin % 32 == 0 ? 32 : 0));
+    if (geom == G_CONV3 && ck && in.w >= kBx && in.h >= kBy && in.d >= kBz) {
+        const int nbx = (in.w + kBx - 1) / kBx, nby = (in.h + kBy - 1) / kBy, nbz = (in.d + kBz - 1) / kBz;
+        const int64_t bricks = (int64_t)nbx * nby * nbz * in.n;
+        if (bricks >= (int64_t)1 << 31) return fail(n, -2, "convolution grid too large");
+        while (nt > 1 && bricks * (coutp / (16 * nt)) < 512) nt >>= 1;
+        const dim3 grid3((unsigned)bricks, (unsigned)(coutp / (16 * nt)), 1), block3(kThreads);
+#define LAUNCH_C3(CK, NTV) hipLaunchKernelGGL((k_conv3_lds<CK, NTV>), grid3, block3, 0, n->stream, a, nbx, nby, nbz)
+#define C3_BY_NT(CK)                     \
+    if (nt == 1) LAUNCH_C3(CK, 1);       \
+    else if (nt == 2) LAUNCH_C3(CK, 2);  \
+    else LAUNCH_C3(CK, 4);
+        if (ck == 8) {
+            C3_BY_NT(8);
+        } else if (ck == 16) {
+            C3_BY_NT(16);
+        } else {
+            C3_BY_NT(32);
+        }
+#undef C3_BY_NT
+#undef LAUNCH_C3
+        UCHK(n, hipGetLastError());
+        return 0;
+    }
+    if (a.wrap) return fail(n, -2, "periodic padding needs 3x3x3 convolutions of at least 16 x 4 x 4 voxels");
     while (nt > 1 && (int64_t)nblk(a.m) * (coutp / (16 * nt)) * npar < 512) nt >>= 1;
     const dim3 grid(nblk(a.m), (unsigned)(coutp / (16 * nt)), (unsigned)npar), block(kThreads);
 #define LAUNCH_CONV(G, NTV) hipLaunchKernelGGL((k_conv<G, NTV>), grid, block, 0, n->stream, a)
@@ -194,6 +223,7 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     a.out = out;
     a.m = (int64_t)rows.n * rows.d * rows.h * rows.w;
     a.chunk = std::max<int64_t>(512, rup(a.m / 512 + 1, 128));
+    a.wrap = L.geom == G_CONV3 ? (n->cfg.periodic & 7) : 0;
     // channel tile 16*N: the smallest that holds both channel counts, at most 64
     const int cmax = std::max(L.cinp, L.coutp);
     const int N = cmax <= 16 ? 1 : (cmax <= 32 ? 2 : 4);
@@ -404,10 +434,12 @@ int lbm_unet_create(const lbm_unet_config* cfg, lbm_unet** out) {
     if (!cfg || !out) return -1;
     *out = nullptr;
     if (cfg->levels < 2 || cfg->levels > 5 || cfg->cin < 1 || cfg->cin > 8 || cfg->cout < 1 || cfg->cout > 16 ||
-        cfg->base < 16 || cfg->base % 16 || cfg->batch < 1)
+        cfg->base < 16 || cfg->base % 16 || cfg->batch < 1 || (cfg->periodic & ~7))
         return -1;
     const int f = 1 << (cfg->levels - 1);
     if (cfg->nx % f || cfg->ny % f || cfg->nz % f || cfg->nx < f || cfg->ny < f || cfg->nz < f) return -1;
+    // wrapped convolutions run only in the brick kernel, which needs 16 x 4 x 4 voxels at the coarsest level
+    if (cfg->periodic && (cfg->nx / f < 16 || cfg->ny / f < 4 || cfg->nz / f < 4)) return -1;
     // the kernels index voxels (not elements) in 32 bits
     if ((int64_t)cfg->batch * cfg->nx * cfg->ny * cfg->nz >= (int64_t)1 << 31) return -1;
     lbm_unet* n = new lbm_unet();

@@ -60,14 +60,24 @@ struct ConvArgs {
     Dims in;                // input dims
     Dims out;               // output dims
     int64_t m;              // number of rows
+    int wrap;               // 3x3x3 taps wrap around along x (bit 0), y (1), z (2) instead of reading zeros
 };
+
+// Coordinate c + o of an axis of size n under the padding rule: wrapped when
+// the axis is periodic, -1 (zero padding) when it falls outside otherwise.
+__device__ inline int pad_coord(int c, int n, bool wrap) {
+    if (c >= 0 && c < n) return c;
+    if (!wrap) return -1;
+    return c < 0 ? c + n : c - n;
+}
 
 // Input voxel of row (n,z,y,x) through tap t; false for zero padding.
 template <int GEOM>
 __device__ inline bool in_voxel(const ConvArgs& a, int n, int z, int y, int x, int t, int64_t& v) {
     if (GEOM == G_CONV3) {
-        const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
-        if (zz < 0 || zz >= a.in.d || yy < 0 || yy >= a.in.h || xx < 0 || xx >= a.in.w) return false;
+        const int zz = pad_coord(z + t / 9 - 1, a.in.d, a.wrap & 4), yy = pad_coord(y + (t / 3) % 3 - 1, a.in.h, a.wrap & 2),
+                  xx = pad_coord(x + t % 3 - 1, a.in.w, a.wrap & 1);
+        if (zz < 0 || yy < 0 || xx < 0) return false;
         v = vox(a.in, n, zz, yy, xx);
     } else if (GEOM == G_DOWN2) {
         v = vox(a.in, n, 2 * z + (t >> 2), 2 * y + ((t >> 1) & 1), 2 * x + (t & 1));
@@ -204,6 +214,158 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
     }
 }
 
+// Same-size 3x3x3 convolution with the input staged in LDS.  A block computes
+// a brick of 4 (z) x 4 (y) x 16 (x) output voxels of one sample, 256 rows, for
+// 16*NT output channels: wave w takes the z-plane w of the brick, its m-tile mt
+// the y-row mt, lane row r16 the voxel x0 + r16.  The input brick with its
+// one-voxel halo (6 x 6 x 18 voxels) is loaded once per chunk of CK input
+// channels (16-byte pieces, zeros outside the grid) and read by every tap from
+// LDS: the A fragment of tap (dz, dy, dx) for row (w, mt, r16) is brick voxel
+// (w + dz, mt + dy, r16 + dx); halo voxels outside the grid are zeros, or the
+// wrapped-around voxels along the axes a.wrap marks periodic.  The reduction index is k = t * cin + ci as in
+// the weight layout: for CK = 8 or 16 (cin = CK, one chunk) a 32-wide k-step
+// spans 32 / CK taps, for CK = 32 it is one tap of one 32-channel chunk (cin a
+// multiple of 32).  B fragments come from global memory (the weights stay in
+// L2), one k-step ahead of the MFMAs that use them.
+constexpr int kBz = 4, kBy = 4, kBx = 16;
+constexpr int kHz = kBz + 2, kHy = kBy + 2, kHx = kBx + 2;
+constexpr int kHvox = kHz * kHy * kHx;
+
+template <int CK>
+struct BrickLds {
+    // voxel stride in LDS, in elements (a multiple of 8, so every 16-byte read
+    // is aligned): 40 for CK = 32 and 24 for CK = 8 or 16, so that the 16 row
+    // lanes of an A read cover 16 disjoint groups of 4 of the 64 banks
+    static constexpr int S = CK == 32 ? 40 : 24;
+};
+
+template <int CK, int NT>
+__global__ __launch_bounds__(kThreads) void k_conv3_lds(ConvArgs a, int nbx, int nby, int nbz) {
+    constexpr int S = BrickLds<CK>::S;
+    constexpr int PPV = CK / 8;  // 16-byte pieces per voxel
+    __shared__ __attribute__((aligned(16))) bf16 s_in[kHvox * S];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int co_base = blockIdx.y * 16 * NT;
+    // brick coordinates: x fastest, then y, z, sample
+    uint32_t b = blockIdx.x;
+    const int bx = (int)(b % (uint32_t)nbx);
+    b /= (uint32_t)nbx;
+    const int by = (int)(b % (uint32_t)nby);
+    b /= (uint32_t)nby;
+    const int bz = (int)(b % (uint32_t)nbz);
+    const int n = (int)(b / (uint32_t)nbz);
+    const int x0 = bx * kBx, y0 = by * kBy, z0 = bz * kBz;
+    const Dims in = a.in;
+    const bf16* xb = a.x + a.xco;
+
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int nchunks = CK == 32 ? a.cin / 32 : 1;
+    constexpr int kTapsPerStep = 32 / CK;
+    constexpr int kSteps = CK == 32 ? 27 : (27 + kTapsPerStep - 1) / kTapsPerStep;
+    const int rowbase = wave * (kHy * kHx) + r16;  // brick voxel of (w, mt = 0, r16), tap (0, 0, 0)
+    const bf16* wrow[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) wrow[nt] = a.wt + (int64_t)(co_base + nt * 16 + r16) * a.kpad;
+
+    for (int j = 0; j < nchunks; ++j) {
+        if (j) __syncthreads();
+        // stage the brick: piece p = (voxel, 8-channel group)
+        for (int p = threadIdx.x; p < kHvox * PPV; p += kThreads) {
+            const int v = p / PPV, q = p - v * PPV;
+            const int hz = v / (kHy * kHx), rem = v - hz * (kHy * kHx);
+            const int hy = rem / kHx, hx = rem - hy * kHx;
+            const int zz = pad_coord(z0 - 1 + hz, in.d, a.wrap & 4), yy = pad_coord(y0 - 1 + hy, in.h, a.wrap & 2),
+                      xx = pad_coord(x0 - 1 + hx, in.w, a.wrap & 1);
+            bf16x8 val = zero8();
+            // (a brick that overhangs the grid's far end by more than the halo reads only zeros there)
+            if (zz >= 0 && yy >= 0 && xx >= 0 && zz < in.d && yy < in.h && xx < in.w)
+                val = load8(xb + (int64_t)vox(in, n, zz, yy, xx) * a.xcs + j * 32 + q * 8);
+            *reinterpret_cast<bf16x8*>(&s_in[v * S + q * 8]) = val;
+        }
+        __syncthreads();
+        // k-step s: lane group g covers taps / channels (tg, cg)
+        auto kpos = [&](int s, int& tg, int& cg) {
+            if (CK == 32) {
+                tg = s;
+                cg = g * 8;
+            } else {
+                const int kk = s * 32 + g * 8;
+                tg = kk / CK;
+                cg = kk - tg * CK;
+            }
+        };
+        bf16x8 Bc[NT];
+        {
+            int tg, cg;
+            kpos(0, tg, cg);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) Bc[nt] = load8(wrow[nt] + tg * a.cin + j * 32 + cg);
+        }
+        for (int s = 0; s < kSteps; ++s) {
+            int tg, cg;
+            kpos(s, tg, cg);
+            bf16x8 Bn[NT];
+            if (s + 1 < kSteps) {
+                int tn, cn;
+                kpos(s + 1, tn, cn);
+                // a k-step past the last tap reads the zero padding of the weight row
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) Bn[nt] = load8(wrow[nt] + tn * a.cin + j * 32 + cn);
+            }
+            bf16x8 A[4];
+            if (tg < 27) {
+                const int dz = tg / 9, dy = (tg / 3) % 3, dx = tg % 3;
+                const int base = rowbase + dz * (kHy * kHx) + dy * kHx + dx;
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) A[mt] = load8(&s_in[(base + mt * kHx) * S + cg]);
+            } else {
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) A[mt] = zero8();
+            }
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], Bc[nt], acc[mt][nt], 0, 0, 0);
+            if (s + 1 < kSteps) {
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) Bc[nt] = Bn[nt];
+            }
+        }
+    }
+    // epilogue: lane holds x = x0 + 4g + i of row (z0 + w, y0 + mt), column r16 of each tile
+    const int z = z0 + wave;
+    if (z >= a.out.d) return;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int y = y0 + mt;
+        if (y >= a.out.h) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + 4 * g + i;
+            if (x >= a.out.w) continue;
+            const int64_t ov = vox(a.out, n, z, y, x);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = co_base + nt * 16 + r16;
+                if (co >= a.cstore) continue;
+                float v = acc[mt][nt][i];
+                if (a.bias) v += a.bias[co];
+                if (a.relu) v = v > 0.f ? v : 0.f;
+                if (a.aux && !((float)a.aux[ov * a.acs + a.aco + co] > 0.f)) v = 0.f;
+                if (a.yf) a.yf[ov * a.cstore + co] = v;
+                else a.y[ov * a.ycs + a.yco + co] = (bf16)v;
+            }
+        }
+    }
+}
+
 // Weight gradient: dw[par][co][t*cin + ci] += sum_rows dy[out(row)][co] * x[in(row,t)][ci].
 // Block: one tap (blockIdx.y), a TC x TC channel tile with TC = 16*N
 // (blockIdx.z), a chunk of rows (blockIdx.x).  The 4 waves are N ci sub-tiles
@@ -223,6 +385,7 @@ struct WgradArgs {
     int64_t m;
     int64_t chunk;         // rows per block
     int nci_tiles;         // ceil(cin / TC)
+    int wrap;              // periodic axes of a 3x3x3 convolution (as ConvArgs::wrap)
 };
 
 template <int GEOM, int N>
@@ -246,6 +409,7 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
 
     ConvArgs ca;
     ca.in = a.in;
+    ca.wrap = a.wrap;
     // staging role: row rr of the step, channels cc..cc+7 of the tile
     const int rr = threadIdx.x % R, cc = (threadIdx.x / R) * 8;
     for (int64_t r0 = rbeg; r0 < rend; r0 += R) {

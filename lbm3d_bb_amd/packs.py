@@ -5,7 +5,12 @@ a 128^3 lattice (both scaled with the lattice size L / 128), centres drawn by
 rejection sampling from one numpy Generator seeded per sample.  The flow case
 is a channel along x: velocity inlet u_in = 0.08 at x = 0, zero-gradient
 outlet at x = L - 1, periodic y and z, D3Q19 TRT with Lambda = 3/16, FP32
-populations.  Its Reynolds number Re = u_in D_ref / nu with D_ref =
+populations.  Neither this outlet nor the reference's anti-bounce-back one
+(src/OtherBC.jl:137-166, which takes the node's own density) fixes the
+density level: the steady state's mass, and the drag with it, carries the
+history of the start, and a uniform start keeps drifting by ~0.1 % per 30k
+steps after its drag residual first falls below 1e-4.  (The anti-bounce-back
+outlet also diverged on these packs within 2k steps at Re 22.)  Its Reynolds number Re = u_in D_ref / nu with D_ref =
 2 * 8.5 * L / 128 (the mean diameter of the radius range) fixes nu, hence
 tau+ = 1/2 + 3 nu.  Re is drawn uniformly in [1, 100], capped where nu
 would fall below NU_MIN (tau+ < 0.53, where the scheme is no longer stable

@@ -24,7 +24,7 @@ class UNetConfig(ct.Structure):
     _fields_ = [("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32), ("batch", ct.c_int32),
                 ("cin", ct.c_int32), ("cout", ct.c_int32), ("base", ct.c_int32), ("levels", ct.c_int32),
                 ("seed", ct.c_uint64), ("lr", ct.c_float), ("beta1", ct.c_float), ("beta2", ct.c_float),
-                ("eps", ct.c_float)]
+                ("eps", ct.c_float), ("periodic", ct.c_int32)]
 
 
 _F = ct.POINTER(ct.c_float)
@@ -71,11 +71,13 @@ class Layer:
 
 
 class UNet:
-    def __init__(self, shape, batch=1, cin=2, cout=4, base=16, levels=4, seed=0, lr=1e-3):
+    def __init__(self, shape, batch=1, cin=2, cout=4, base=16, levels=4, seed=0, lr=1e-3, periodic=0):
+        """periodic: bit 0 x, bit 1 y, bit 2 z -- axes along which the 3x3x3
+        convolutions wrap around instead of padding with zeros."""
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
-        self.cin, self.cout, self.batch = cin, cout, batch
-        cfg = UNetConfig(nx, ny, nz, batch, cin, cout, base, levels, seed, lr, 0.9, 0.999, 1e-8)
+        self.cin, self.cout, self.batch, self.periodic = cin, cout, batch, periodic
+        cfg = UNetConfig(nx, ny, nz, batch, cin, cout, base, levels, seed, lr, 0.9, 0.999, 1e-8, periodic)
         self.lib = _lib()
         self._h = ct.c_void_p()
         rc = self.lib.lbm_unet_create(ct.byref(cfg), ct.byref(self._h))
@@ -124,6 +126,9 @@ class UNet:
 
     def forward(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 5 or x.shape[1:] != (self.cin,) + self.shape or x.shape[0] > self.batch:
+            raise ValueError(f"input of shape {x.shape} for a network of {self.cin} channels on {self.shape}, "
+                             f"batch <= {self.batch}")
         b = x.shape[0]
         y = np.empty((b, self.cout) + self.shape, dtype=np.float32)
         self._chk(self.lib.lbm_unet_forward(self._h, _fp(x), b, _fp(y)))
@@ -165,11 +170,29 @@ class UNet:
 # --------------------------------------------------------------------------
 LOG_RE_MAX = float(np.log(100.0))
 
+# The pack network: 4 levels of 16/32/64/128 channels; inputs solid mask,
+# ln(Re)/ln(100) and the position along the channel x/(nx - 1) (the inlet is
+# at x = 0, the outlet at nx - 1: the pressure falls along x, and a cell far
+# from both ends could not tell its distance to either from the mask alone);
+# 3x3x3 convolutions wrap around in y and z like the packs.
+PACK_NET = dict(cin=3, cout=4, base=16, levels=4, periodic=6)
 
-def encode_input(mask: np.ndarray, re: float) -> np.ndarray:
-    """(solid mask, ln(Re)/ln(100)) channels, shape (2, nz, ny, nx)."""
+
+def pack_net(shape, batch: int = 1, seed: int = 0, lr: float = 1e-3) -> "UNet":
+    return UNet(shape, batch=batch, seed=seed, lr=lr, **PACK_NET)
+
+
+def encode_input(mask: np.ndarray, re: float, channels: int = 2) -> np.ndarray:
+    """Network input, shape (channels, nz, ny, nx): the solid mask,
+    ln(Re)/ln(100) and, for channels = 3, x/(nx - 1)."""
     m = (np.asarray(mask) != 0).astype(np.float32)
-    return np.stack([m, np.full_like(m, np.log(re) / LOG_RE_MAX)])
+    ch = [m, np.full_like(m, np.log(re) / LOG_RE_MAX)]
+    if channels >= 3:
+        nx = m.shape[2]
+        ch.append(np.broadcast_to(np.linspace(0.0, 1.0, nx, dtype=np.float32), m.shape).copy())
+    if channels not in (2, 3):
+        raise ValueError("the network input has 2 or 3 channels")
+    return np.stack(ch)
 
 
 def encode_target(rho: np.ndarray, u: np.ndarray, u_ref: float, shape) -> np.ndarray:
@@ -246,7 +269,7 @@ def predict(net: UNet, case) -> tuple[np.ndarray, np.ndarray]:
     """(rho, u) predicted for a pack case; u is (3, N), solid cells at rest."""
     L = case.L
     mask = case.mask()
-    y = net.forward(encode_input(mask, case.re)[None])[0]
+    y = net.forward(encode_input(mask, case.re, net.cin)[None])[0]
     rho, u = decode_output(y, case.u_in)
     solid = mask.reshape(-1)
     u[:, solid] = 0.0
@@ -289,7 +312,7 @@ def predict_tiled(net, mask: np.ndarray, re: float, u_ref: float, halo: int = 16
     u = np.empty((3, nz, ny, nx), dtype=np.float64)
     for i in range(0, len(tiles), net.batch):
         group = tiles[i:i + net.batch]
-        x = np.stack([encode_input(mask[sz:sz + tz, sy:sy + ty, sx:sx + tx], re)
+        x = np.stack([encode_input(mask[sz:sz + tz, sy:sy + ty, sx:sx + tx], re, net.cin)
                       for (sz, _, _), (sy, _, _), (sx, _, _) in group])
         y = net.forward(x)
         for b, ((sz, lz, hz), (sy, ly, hy), (sx, lx, hx)) in enumerate(group):
@@ -377,9 +400,10 @@ def solve_steady(case, net: UNet | None = None, use_net: bool = False, tol: floa
     return res
 
 
-def generate_dataset(seeds, L: int, log=None, budget_s: float | None = None, **kw) -> dict:
+def generate_dataset(seeds, L: int, log=None, budget_s: float | None = None, cin: int = PACK_NET["cin"],
+                     **kw) -> dict:
     """Steady FP32 fields of the packs ``seeds`` from uniform starts: network
-    inputs X (n, 2, L, L, L), targets Y (n, 4, L, L, L) (both held in FP16 on
+    inputs X (n, cin, L, L, L), targets Y (n, 4, L, L, L) (both held in FP16 on
     the host) and the plain steps-to-converged-drag of every sample.  Packs
     whose run does not reach a converged, finite drag (an unsteady wake) are
     left out of X and Y.  With budget_s, generation stops once that many
@@ -388,7 +412,7 @@ def generate_dataset(seeds, L: int, log=None, budget_s: float | None = None, **k
 
     seeds = list(seeds)
     n = len(seeds)
-    X = np.empty((n, 2, L, L, L), dtype=np.float16)
+    X = np.empty((n, cin, L, L, L), dtype=np.float16)
     Y = np.empty((n, 4, L, L, L), dtype=np.float16)
     steps = np.zeros(n, dtype=np.int64)
     conv = np.zeros(n, dtype=bool)
@@ -405,7 +429,7 @@ def generate_dataset(seeds, L: int, log=None, budget_s: float | None = None, **k
         ok = bool(r["converged"]) and bool(np.all(np.isfinite(r["rho"]))) and bool(np.all(np.isfinite(r["u"])))
         steps[i], conv[i], re[i] = r["steps"], ok, case.re
         if ok:
-            X[k] = encode_input(case.mask(), case.re)
+            X[k] = encode_input(case.mask(), case.re, cin)
             Y[k] = encode_target(r["rho"], r["u"], case.u_in, (L, L, L))
             k += 1
         if log:
@@ -416,7 +440,7 @@ def generate_dataset(seeds, L: int, log=None, budget_s: float | None = None, **k
 
 
 def augment(x: np.ndarray, y: np.ndarray, rng) -> tuple[np.ndarray, np.ndarray]:
-    """An exact symmetry of the pack flow applied to one sample (x (2, z, y, x),
+    """An exact symmetry of the pack flow applied to one sample (x (cin, z, y, x),
     y (4, z, y, x)): a periodic shift in y and z, reflections of y and z (the
     velocity component flips sign) and the y <-> z exchange."""
     sy, sz = (int(v) for v in rng.integers(0, x.shape[2], size=2))
@@ -437,33 +461,47 @@ def augment(x: np.ndarray, y: np.ndarray, rng) -> tuple[np.ndarray, np.ndarray]:
 
 
 def train(net: UNet, X: np.ndarray, Y: np.ndarray, epochs: int, batch: int, lr: float = 1e-3, seed: int = 0,
-          log=None, augment_data: bool = True) -> list[float]:
-    """Adam on the mean-squared error with a cosine learning-rate decay; the
-    samples are reshuffled every epoch and, by default, each draw is a random
-    symmetry image of its sample (augment).  Returns the mean loss per epoch."""
+          log=None, augment_data: bool = True, warmup: int = 200, on_epoch=None) -> list[float]:
+    """Adam on the mean-squared error with a linear warm-up over `warmup`
+    steps and a cosine decay after it; the samples are reshuffled every epoch
+    and, by default, each draw is a random symmetry image of its sample
+    (augment).  The next batch is assembled on a host thread while the device
+    runs the current step.  on_epoch(epoch, loss), if given, runs after every
+    epoch (a checkpoint).  Returns the mean loss per epoch."""
+    from concurrent.futures import ThreadPoolExecutor
+
     rng = np.random.default_rng(seed)
     n = X.shape[0]
     per_epoch = max(n // batch, 1)
     total = epochs * per_epoch
+    order = np.concatenate([rng.permutation(n)[:per_epoch * batch] for _ in range(epochs)]) if n else np.zeros(0, int)
+
+    def make(t: int):
+        idx = np.sort(order[t * batch:(t + 1) * batch])
+        if augment_data:
+            pairs = [augment(X[i], Y[i], rng) for i in idx]
+            return (np.stack([p[0] for p in pairs]).astype(np.float32),
+                    np.stack([p[1] for p in pairs]).astype(np.float32))
+        return X[idx].astype(np.float32), Y[idx].astype(np.float32)
+
     hist = []
-    t = 0
-    for ep in range(epochs):
-        order = rng.permutation(n)
-        losses = []
-        for k in range(per_epoch):
-            idx = np.sort(order[k * batch:(k + 1) * batch])
-            if augment_data:
-                pairs = [augment(X[i].astype(np.float32), Y[i].astype(np.float32), rng) for i in idx]
-                xb = np.stack([p[0] for p in pairs])
-                yb = np.stack([p[1] for p in pairs])
-            else:
-                xb, yb = X[idx].astype(np.float32), Y[idx].astype(np.float32)
-            net.set_lr(lr * 0.5 * (1.0 + np.cos(np.pi * t / total)) + 1e-6)
+    losses = []
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        nxt = pool.submit(make, 0) if total else None
+        for t in range(total):
+            xb, yb = nxt.result()
+            if t + 1 < total:
+                nxt = pool.submit(make, t + 1)
+            f = (t + 1) / warmup if t < warmup else 0.5 * (1.0 + np.cos(np.pi * (t - warmup) / max(total - warmup, 1)))
+            net.set_lr(lr * f + 1e-6)
             losses.append(net.train_step(xb, yb))
-            t += 1
-        hist.append(float(np.mean(losses)))
-        if log:
-            log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.5f}")
+            if (t + 1) % per_epoch == 0:
+                hist.append(float(np.mean(losses)))
+                losses = []
+                if log:
+                    log(f"epoch {len(hist)}/{epochs} loss {hist[-1]:.5f}")
+                if on_epoch:
+                    on_epoch(len(hist), hist[-1])
     return hist
 
 

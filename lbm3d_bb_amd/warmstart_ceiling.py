@@ -17,6 +17,14 @@ again from starts built from that state:
                 relative rms e on rho - 1 (a network of that accuracy), with
                 the non-equilibrium part
 
+  rho1          the steady u with rho = 1 everywhere (a network that predicts
+                the velocity only)
+  rho_mean1     the steady fields with rho shifted to a fluid mean of 1
+                (the pressure field without its level)
+  net, net_rho1, net_u_exact_rho
+                with --params: the network's (rho, u); its u with rho = 1;
+                its u with the steady rho
+
 and the uniform start itself.  Every run records the drag (x-force summed over
 the bodies) of every step, so the steps to convergence of any residual rule
 (window length, tolerance, patience) are evaluated afterwards on the same
@@ -82,8 +90,9 @@ def evaluate_rules(drag: np.ndarray) -> dict:
     return {f"every{e}_tol{t:g}": steps_to_converge(drag, e, t) for e, t in RULES}
 
 
-def run_case(seed: int, L: int, settle: int, steps: int, noise: list[float], log) -> dict:
+def run_case(seed: int, L: int, settle: int, steps: int, noise: list[float], log, net=None) -> dict:
     from .solver import Solver
+    from .warmstart import predict
 
     case = make_pack(seed, L)
     shape = (L, L, L)
@@ -100,16 +109,36 @@ def run_case(seed: int, L: int, settle: int, steps: int, noise: list[float], log
     res = {"seed": seed, "re": case.re, "spheres": len(case.bodies), "tau_plus": case.tau_plus,
            "drag_steady": float(d_uni[-1000:].mean()),
            "uniform": evaluate_rules(d_uni)}
-    starts = {"populations": ("pop", None), "fields_ce": ("ce", 0.0), "fields_eq": ("eq", 0.0)}
+    fluid = ~solid
+    starts = {"populations": ("pop", None), "fields_ce": ("ce", 0.0), "fields_eq": ("eq", 0.0),
+              "rho1": ("ce", "rho1"), "rho_mean1": ("ce", "rho_mean1")}
     for e in noise:
         starts[f"noise_{e:g}"] = ("ce", e)
+    if net is not None:
+        rn, un = predict(net, case)
+        starts.update({"net": ("ce", "net"), "net_rho1": ("ce", "net_rho1"), "net_u_exact_rho": ("ce", "net_u")})
+        du = (un - u)[:, fluid] / case.u_in
+        dr = (rn - rho)[fluid]
+        res["net_pred"] = {"u_rms": float(np.sqrt((du * du).mean())),
+                           "rho_rel_rms": float(np.sqrt((dr * dr).mean()) / np.sqrt(((rho[fluid] - 1.0) ** 2).mean())),
+                           "rho_mean_pred": float(rn[fluid].mean()), "rho_mean_steady": float(rho[fluid].mean())}
     rng = np.random.default_rng(seed)
     for name, (kind, e) in starts.items():
         if kind == "pop":
             s.set_populations(f_ss)
         else:
             r2, u2 = rho.copy(), u.copy()
-            if e:
+            if e == "rho1":
+                r2[:] = 1.0
+            elif e == "rho_mean1":
+                r2 = rho - rho[fluid].mean() + 1.0
+            elif e == "net":
+                r2, u2 = rn.copy(), un.copy()
+            elif e == "net_rho1":
+                r2, u2 = np.ones_like(rho), un.copy()
+            elif e == "net_u":
+                u2 = un.copy()
+            elif e:
                 # errors smooth on the scale of a sphere: rms e u_in on each velocity
                 # component, rms e on the relative deviation of rho from the rest state
                 r2 = 1.0 + (rho - 1.0) * (1.0 + e * smooth_noise(shape, rng, 4.0).reshape(-1))
@@ -135,15 +164,22 @@ def main(argv=None) -> int:
     ap.add_argument("--settle", type=int, default=60000, help="uniform-start steps that define the steady state")
     ap.add_argument("--steps", type=int, default=30000, help="steps of every other start")
     ap.add_argument("--noise", default="0.01,0.03,0.1")
+    ap.add_argument("--params", default=None, help="U-Net parameters (.npy): add the network's starts")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
+    net = None
+    if a.params:
+        from .warmstart import pack_net
+
+        net = pack_net((a.size,) * 3, batch=1)
+        net.load(a.params)
 
     def log(m):
         print(m, flush=True)
 
     rows = []
     for sd in [int(x) for x in a.seeds.split(",") if x]:
-        rows.append(run_case(sd, a.size, a.settle, a.steps, [float(x) for x in a.noise.split(",") if x], log))
+        rows.append(run_case(sd, a.size, a.settle, a.steps, [float(x) for x in a.noise.split(",") if x], log, net))
         with open(a.out, "w", encoding="utf-8") as fh:
             json.dump({"size": a.size, "settle": a.settle, "steps": a.steps, "rules": RULES, "rows": rows}, fh,
                       indent=1)

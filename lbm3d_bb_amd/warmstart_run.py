@@ -29,7 +29,7 @@ import time
 import numpy as np
 
 from .packs import U_IN, re_max
-from .warmstart import UNet, evaluate, generate_dataset, train
+from .warmstart import PACK_NET, evaluate, generate_dataset, pack_net, train
 
 HELD_OUT_BASE = 1_000_000
 
@@ -63,14 +63,15 @@ def main(argv=None) -> int:
 
     L = args.size
     kw = dict(tol=args.tol, every=args.every, max_steps=args.max_steps)
-    net = UNet((L, L, L), batch=args.batch, levels=4, base=16, seed=0, lr=args.lr)
+    net = pack_net((L, L, L), batch=args.batch, seed=0, lr=args.lr)
     if args.params:
         net.load(args.params)
     summary = {
         "workload": f"{L}^3 sphere packs (4-32 spheres, r in [5,12]*{L}/128), Re uniform in "
                     f"[1,{re_max(L):.0f}], inlet u={U_IN}, zero-gradient outlet, periodic y/z, TRT 3/16, FP32; "
                     f"drag residual {args.tol} over {args.every}-step window means, 3 in a row",
-        "params_in": args.params, "batch": args.batch, "epochs": args.epochs, "seconds": {},
+        "network": PACK_NET, "params_in": args.params, "batch": args.batch, "epochs": args.epochs, "lr": args.lr,
+        "seconds": {},
     }
     if args.train > 0:
         t0 = time.perf_counter()
@@ -81,13 +82,14 @@ def main(argv=None) -> int:
         summary.update(train_seeds=[int(args.train_from), int(args.train_from + len(data["seeds"]))],
                        train_samples_used=used, uniform_steps_median=float(np.median(data["steps"])))
         t0 = time.perf_counter()
-        hist = train(net, data["X"], data["Y"], args.epochs, args.batch, lr=args.lr, log=log)
+        ckpt = os.path.join(args.out, "unet_params.npy")
+        hist = train(net, data["X"], data["Y"], args.epochs, args.batch, lr=args.lr, log=log,
+                     on_epoch=lambda ep, loss: net.save(ckpt))
         summary["seconds"]["train"] = time.perf_counter() - t0
         ms, flops = net.last_timing()
         summary.update(loss_first_epoch=hist[0], loss_last_epoch=hist[-1], loss_per_epoch=hist, train_step_ms=ms,
                        train_step_tflops=flops / ms / 1e9 if ms > 0 else None)
         del data
-        net.save(os.path.join(args.out, "unet_params.npy"))
     if args.eval > 0:
         t0 = time.perf_counter()
         ev = evaluate(net, range(HELD_OUT_BASE, HELD_OUT_BASE + args.eval), L, log=log, **kw)
@@ -106,12 +108,13 @@ def timing(L: int, batch: int, k: int) -> int:
     k calls after one untimed call of each."""
     rng = np.random.default_rng(0)
     shape = (L, L, L)
-    net = UNet(shape, batch=batch, levels=4, base=16, seed=0)
-    x = np.zeros((batch, 2) + shape, dtype=np.float32)
+    net = pack_net(shape, batch=batch, seed=0)
+    x = np.zeros((batch, net.cin) + shape, dtype=np.float32)
     x[:, 0] = rng.random((batch,) + shape) < 0.2
-    x[:, 1] = 0.5
+    x[:, 1:] = 0.5
     t = (0.1 * rng.standard_normal((batch, 4) + shape)).astype(np.float32)
-    out = {"workload": f"U-Net 4 levels 16/32/64/128 channels, {L}^3, batch {batch}", "calls": k}
+    out = {"workload": f"U-Net 4 levels 16/32/64/128 channels, {net.cin} input channels, y/z wrap, {L}^3, "
+                       f"batch {batch}", "calls": k}
     for name, fn in (("forward", lambda: net.forward(x)), ("train_step", lambda: net.train_step(x, t))):
         fn()
         best, flops = float("inf"), 0.0

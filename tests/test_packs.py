@@ -52,6 +52,9 @@ def test_encode_decode_roundtrip():
     assert np.abs(r2 - rho).max() < 1e-7 and np.abs(u2 - u).max() < 1e-8
     x = encode_input(rng.random(shape) < 0.2, 100.0)
     assert x.shape == (2,) + shape and np.all(x[1] == 1.0)
+    x3 = encode_input(rng.random(shape) < 0.2, 10.0, 3)
+    assert x3.shape == (3,) + shape and np.allclose(x3[1], 0.5)
+    assert np.all(x3[2, :, :, 0] == 0.0) and np.all(x3[2, :, :, -1] == 1.0) and np.all(np.diff(x3[2], axis=2) > 0)
 
 
 def test_garbage_prediction_falls_back():

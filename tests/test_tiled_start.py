@@ -46,7 +46,7 @@ class LocalNet:
     and the Re channel times (1 + the mask shifted by +1 along z)."""
 
     def __init__(self, shape, r=2, batch=3):
-        self.shape, self.r, self.batch = tuple(shape), r, batch
+        self.shape, self.r, self.batch, self.cin = tuple(shape), r, batch, 2
         self.calls = 0
 
     def forward(self, x):

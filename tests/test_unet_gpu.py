@@ -106,3 +106,38 @@ def test_wide_channel_tiles_match_torch(base):
             nw = (8 if L.kind == 1 else 1) * L.cout * (27 if L.kind == 0 else 1) * L.cin
             assert _rel(g[L.w_off:L.w_off + nw], gr[L.w_off:L.w_off + nw]) < 5e-2, (i, L)
             assert _rel(g[L.b_off:L.b_off + L.cout], gr[L.b_off:L.b_off + L.cout]) < 5e-2, (i, L)
+
+
+@pytest.mark.parametrize("periodic", [6, 7])
+def test_periodic_padding_matches_torch(periodic):
+    """Convolutions that wrap around along y and z (and x) instead of padding
+    with zeros, forward and gradient, against torch with circular padding."""
+    import torch
+
+    from unet_reference import forward, loss_and_grad
+    from lbm3d_bb_amd.warmstart import UNet
+
+    shape = (8, 8, 32)
+    net = UNet(shape, batch=2, levels=2, base=16, seed=17, periodic=periodic)
+    x, t = _data(shape, 2, seed=19)
+    y = net.forward(x)
+    p = torch.tensor(net.get_params(), dtype=torch.float64)
+    yr = forward(p, net.layers(), torch.tensor(x, dtype=torch.float64), 2, periodic=periodic).numpy()
+    y0 = forward(p, net.layers(), torch.tensor(x, dtype=torch.float64), 2, periodic=0).numpy()
+    assert _rel(y0, yr) > 0.05  # the wrap changes the answer
+    assert _rel(y, yr) < 2e-2, _rel(y, yr)
+    loss, g = net.gradient(x, t)
+    lr, gr, _ = loss_and_grad(net.get_params(), net.layers(), x, t, 2, periodic=periodic)
+    assert abs(loss - lr) / lr < 1e-2
+    for i, L in enumerate(net.layers()):
+        nw = (8 if L.kind == 1 else 1) * L.cout * (27 if L.kind == 0 else 1) * L.cin
+        assert _rel(g[L.w_off:L.w_off + nw], gr[L.w_off:L.w_off + nw]) < 5e-2, (i, L)
+        assert _rel(g[L.b_off:L.b_off + L.cout], gr[L.b_off:L.b_off + L.cout]) < 5e-2, (i, L)
+
+
+def test_periodic_padding_on_a_grid_too_small_fails():
+    from lbm3d_bb_amd import _ffi
+    from lbm3d_bb_amd.warmstart import UNet
+
+    with pytest.raises(_ffi.LbmError):
+        UNet((8, 8, 16), batch=1, levels=2, base=16, seed=1, periodic=6)  # level 1 is 8 voxels wide

@@ -29,13 +29,30 @@ def layer_weights(params: torch.Tensor, L) -> tuple[torch.Tensor, torch.Tensor]:
     return w, b
 
 
-def forward(params: torch.Tensor, layers, x: torch.Tensor, levels: int, bf16: bool = True) -> torch.Tensor:
+def pad3(h: torch.Tensor, periodic: int) -> torch.Tensor:
+    """One voxel of padding on every side of (B, C, D, H, W): wrapped around
+    along the axes periodic marks (bit 0 x = W, bit 1 y = H, bit 2 z = D),
+    zeros along the others."""
+    hp = F.pad(h, (1, 1, 1, 1, 1, 1), mode="circular")
+    keep = torch.ones(hp.shape[2:], dtype=hp.dtype)
+    for bit, ax in ((1, 2), (2, 1), (4, 0)):
+        if not periodic & bit:
+            idx = [slice(None)] * 3
+            idx[ax] = 0
+            keep[tuple(idx)] = 0
+            idx[ax] = -1
+            keep[tuple(idx)] = 0
+    return hp * keep
+
+
+def forward(params: torch.Tensor, layers, x: torch.Tensor, levels: int, bf16: bool = True,
+            periodic: int = 0) -> torch.Tensor:
     """x: (B, cin, D, H, W); the layer order matches the library's table."""
     it = iter(layers)
 
     def conv3(h, L, relu=True):
         w, b = layer_weights(params, L)
-        y = F.conv3d(h, _bf(w, bf16), b, padding=1)
+        y = F.conv3d(pad3(h, periodic), _bf(w, bf16), b)
         return _bf(F.relu(y) if relu else y, bf16)
 
     enc = []
@@ -61,11 +78,11 @@ def forward(params: torch.Tensor, layers, x: torch.Tensor, levels: int, bf16: bo
     return F.conv3d(h, _bf(w, bf16), b)
 
 
-def loss_and_grad(params_np: np.ndarray, layers, x_np, t_np, levels, bf16=True):
+def loss_and_grad(params_np: np.ndarray, layers, x_np, t_np, levels, bf16=True, periodic=0):
     p = torch.tensor(params_np, dtype=torch.float64, requires_grad=True)
     x = torch.tensor(x_np, dtype=torch.float64)
     t = torch.tensor(t_np, dtype=torch.float64)
-    y = forward(p, layers, x, levels, bf16=bf16)
+    y = forward(p, layers, x, levels, bf16=bf16, periodic=periodic)
     loss = ((y - t) ** 2).mean()
     loss.backward()
     return float(loss.detach()), p.grad.numpy(), y.detach().numpy()
