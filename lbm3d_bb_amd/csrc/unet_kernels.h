@@ -222,11 +222,14 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
 // channels (16-byte pieces, zeros outside the grid) and read by every tap from
 // LDS: the A fragment of tap (dz, dy, dx) for row (w, mt, r16) is brick voxel
 // (w + dz, mt + dy, r16 + dx); halo voxels outside the grid are zeros, or the
-// wrapped-around voxels along the axes a.wrap marks periodic.  The reduction index is k = t * cin + ci as in
-// the weight layout: for CK = 8 or 16 (cin = CK, one chunk) a 32-wide k-step
-// spans 32 / CK taps, for CK = 32 it is one tap of one 32-channel chunk (cin a
-// multiple of 32).  B fragments come from global memory (the weights stay in
-// L2), one k-step ahead of the MFMAs that use them.
+// wrapped-around voxels along the axes a.wrap marks periodic.  The reduction
+// index is k = t * cin + ci as in the weight layout, t = 9 dz + 3 dy + dx: a
+// 32-wide k-step holds one tap column dx of 32 channels (CK = 32, cin a
+// multiple of 32, one chunk of 32 channels at a time), two columns of 16
+// channels (CK = 16) or three of 8 (CK = 8, the fourth lane group reading
+// zeros), and the steps of one (dz, columns) group share their A fragments
+// (the loop below).  B fragments come from global memory (the weights stay
+// in L2).
 constexpr int kBz = 4, kBy = 4, kBx = 16;
 constexpr int kHz = kBz + 2, kHy = kBy + 2, kHx = kBx + 2;
 constexpr int kHvox = kHz * kHy * kHx;
@@ -289,53 +292,52 @@ __global__ __launch_bounds__(kThreads) void k_conv3_lds(ConvArgs a, int nbx, int
             *reinterpret_cast<bf16x8*>(&s_in[v * S + q * 8]) = val;
         }
         __syncthreads();
-        // k-step s: lane group g covers taps / channels (tg, cg)
-        auto kpos = [&](int s, int& tg, int& cg) {
-            if (CK == 32) {
-                tg = s;
-                cg = g * 8;
-            } else {
-                const int kk = s * 32 + g * 8;
-                tg = kk / CK;
-                cg = kk - tg * CK;
-            }
-        };
-        bf16x8 Bc[NT];
-        {
-            int tg, cg;
-            kpos(0, tg, cg);
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) Bc[nt] = load8(wrow[nt] + tg * a.cin + j * 32 + cg);
+        // k-steps grouped by (dz, xg): the lanes of group g take the tap
+        // column dx(g) and channels c(g) of the group (CK = 32: dx = xg, all
+        // 32 channels; CK = 16: dx = 2 xg + g/2, 16 channels; CK = 8: dx = g,
+        // 8 channels; a dx past 2 reads zeros).  The six brick rows
+        // hy = 0..5 of that column serve the three taps dy = 0..2 of all four
+        // m-tiles (row mt + dy), so each A fragment feeds two MFMAs on average.
+        constexpr int XG = CK == 32 ? 3 : (CK == 16 ? 2 : 1);
+        int dxg, cgl;
+        if (CK == 32) {
+            dxg = 0;
+            cgl = g * 8;
+        } else if (CK == 16) {
+            dxg = g >> 1;
+            cgl = (g & 1) * 8;
+        } else {
+            dxg = g;
+            cgl = 0;
         }
-        for (int s = 0; s < kSteps; ++s) {
-            int tg, cg;
-            kpos(s, tg, cg);
-            bf16x8 Bn[NT];
-            if (s + 1 < kSteps) {
-                int tn, cn;
-                kpos(s + 1, tn, cn);
-                // a k-step past the last tap reads the zero padding of the weight row
+        for (int dz = 0; dz < 3; ++dz) {
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) Bn[nt] = load8(wrow[nt] + tn * a.cin + j * 32 + cn);
-            }
-            bf16x8 A[4];
-            if (tg < 27) {
-                const int dz = tg / 9, dy = (tg / 3) % 3, dx = tg % 3;
-                const int base = rowbase + dz * (kHy * kHx) + dy * kHx + dx;
+            for (int xg = 0; xg < XG; ++xg) {
+                const int dx = (CK == 32 ? xg : (CK == 16 ? 2 * xg : 0)) + dxg;
+                // weight row offset of tap (dz, dy = 0, the group's first column) for this lane
+                const int kw = (dz * 9) * a.cin + (CK == 32 ? xg * a.cin + j * 32 : (CK == 16 ? 32 * xg : 0)) + g * 8;
+                bf16x8 B[3][NT];
 #pragma unroll
-                for (int mt = 0; mt < 4; ++mt) A[mt] = load8(&s_in[(base + mt * kHx) * S + cg]);
-            } else {
+                for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-                for (int mt = 0; mt < 4; ++mt) A[mt] = zero8();
-            }
+                    for (int nt = 0; nt < NT; ++nt) B[dy][nt] = load8(wrow[nt] + kw + dy * 3 * a.cin);
+                bf16x8 F[kHy];
+                if (dx <= 2) {
+                    const int base = (rowbase + dz * (kHy * kHx) + dx) * S + cgl;
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
+                    for (int hy = 0; hy < kHy; ++hy) F[hy] = load8(&s_in[base + hy * kHx * S]);
+                } else {
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], Bc[nt], acc[mt][nt], 0, 0, 0);
-            if (s + 1 < kSteps) {
+                    for (int hy = 0; hy < kHy; ++hy) F[hy] = zero8();
+                }
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) Bc[nt] = Bn[nt];
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[mt][nt] =
+                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[mt + dy], B[dy][nt], acc[mt][nt], 0, 0, 0);
             }
         }
     }

@@ -6,11 +6,15 @@ formats, against the FP64 run of the same configuration.
 
 Each storage runs the configuration from the same start for the same steps
 (the FP64 run is the reference: its step is pinned to the oracle at the FP64
-bar).  The force of a body is the momentum-exchange sum of src/MEA.jl:76-116
+bar).  `--start net` starts every run from the tile-wise prediction of the
+trained pack network instead of the uniform state (bench.net_start); the
+output records the start taken.  The force of a body is the momentum-exchange sum of src/MEA.jl:76-116
 (FP64 accumulation, with torque).  Reported per body: the mean x-force (drag)
 and the mean torque vector over the last `--window` steps, and for every
 non-FP64 storage the relative drag error |D - D64| / |D64| and the torque error
-|T - T64| / max_bodies |T64|.  Prints a progress line every `--chunk` steps.
+|T - T64| / max_bodies |T64|; and the history: per-body force and torque
+averaged over every `--chunk` steps.  Prints a progress line every `--chunk`
+steps.
 """
 from __future__ import annotations
 
@@ -47,6 +51,7 @@ def run_storage(w: dict, storage: str, steps: int, window: int, chunk: int, log,
     nb = len(laws) if laws is not None else len(bodies)
     tail = np.zeros((0, nb, 6))
     drag_hist = []
+    hist = []
     t0 = time.perf_counter()
     done = 0
     while done < steps:
@@ -57,6 +62,7 @@ def run_storage(w: dict, storage: str, steps: int, window: int, chunk: int, log,
             raise FloatingPointError(f"{storage}: non-finite force by step {done}")
         tail = np.concatenate([tail, F])[-window:]
         drag_hist.append([done, F[-1, :, 0].sum()])
+        hist.append([done, F.mean(axis=0).tolist()])
         el = time.perf_counter() - t0
         log(f"{storage}: step {done}/{steps}, {cfg.ncells * done / el / 1e6:.0f} MLUPS, "
             f"total drag {F[-1, :, 0].sum():.6e}")
@@ -66,7 +72,8 @@ def run_storage(w: dict, storage: str, steps: int, window: int, chunk: int, log,
     return {"storage": storage, "steps": steps, "seconds": el, "mlups": cfg.ncells * steps / el / 1e6,
             "drag": tail[:, :, 0].mean(axis=0).tolist(), "force": tail[:, :, :3].mean(axis=0).tolist(),
             "torque": tail[:, :, 3:].mean(axis=0).tolist(), "rho_min": float(rho.min()),
-            "rho_max": float(rho.max()), "drag_total_history": drag_hist, "start": info, "workload": desc}
+            "rho_max": float(rho.max()), "drag_total_history": drag_hist, "chunk_mean_history": hist,
+            "start": info, "workload": desc}
 
 
 def compare(ref: dict, run: dict) -> dict:
@@ -92,6 +99,8 @@ def main(argv=None) -> int:
     ap.add_argument("--chunk", type=int, default=2000, help="steps between progress lines")
     ap.add_argument("--storages", default="fp64,fp32,fp16s,bf16s")
     ap.add_argument("--ref", default=None, help="JSON of an earlier FP64 run of the same workload and steps")
+    ap.add_argument("--start", default="uniform", choices=["uniform", "net"],
+                    help="net: the trained pack network's tile-wise start (lbm3d_bb_amd/data/unet_pack128.npy)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -106,14 +115,20 @@ def main(argv=None) -> int:
         runs["fp64"] = prev["runs"]["fp64"]
         if runs["fp64"]["steps"] != a.steps:
             raise SystemExit("the reference run has a different step count")
+    start = None
+    if a.start == "net":
+        from .warmstart import PACK_PARAMS
+
+        def start(s, cfg, laws, bodies):
+            return bench.net_start(s, cfg, laws, bodies, PACK_PARAMS)
     for st in [x for x in a.storages.split(",") if x]:
         if st in runs:
             continue
-        runs[st] = run_storage(w, st, a.steps, a.window, a.chunk, log)
+        runs[st] = run_storage(w, st, a.steps, a.window, a.chunk, log, start=start)
         with open(a.out, "w", encoding="utf-8") as fh:  # partial results survive a cut-off run
             json.dump({"workload": a.workload, "steps": a.steps, "window": a.window, "runs": runs}, fh)
     out = {"workload": a.workload, "description": next(iter(runs.values()))["workload"], "steps": a.steps,
-           "window": a.window, "runs": runs}
+           "window": a.window, "start": a.start, "runs": runs}
     if "fp64" in runs:
         out["errors_vs_fp64"] = {st: compare(runs["fp64"], r) for st, r in runs.items() if st != "fp64"}
         for st, e in out["errors_vs_fp64"].items():

@@ -12,6 +12,7 @@ component: it always starts from a uniform equilibrium
 from __future__ import annotations
 
 import ctypes as ct
+import os
 import time
 from dataclasses import dataclass
 
@@ -178,8 +179,18 @@ LOG_RE_MAX = float(np.log(100.0))
 PACK_NET = dict(cin=3, cout=4, base=16, levels=4, periodic=6)
 
 
-def pack_net(shape, batch: int = 1, seed: int = 0, lr: float = 1e-3) -> "UNet":
-    return UNet(shape, batch=batch, seed=seed, lr=lr, **PACK_NET)
+# Parameters of the pack network trained at 128^3 (1042 packs, seeds 0-1041,
+# 24 epochs; stored in FP16, the kernels round them to BF16 anyway).
+PACK_PARAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_pack128.npy")
+
+
+def pack_net(shape, batch: int = 1, seed: int = 0, lr: float = 1e-3, trained: bool = False) -> "UNet":
+    """The pack network; with trained=True it holds the committed 128^3
+    parameters (any shape the 4-level network accepts)."""
+    net = UNet(shape, batch=batch, seed=seed, lr=lr, **PACK_NET)
+    if trained:
+        net.set_params(np.load(PACK_PARAMS, allow_pickle=False).astype(np.float32))
+    return net
 
 
 def encode_input(mask: np.ndarray, re: float, channels: int = 2) -> np.ndarray:
@@ -263,6 +274,24 @@ def plausible(rho: np.ndarray, u: np.ndarray, u_ref: float) -> tuple[bool, str]:
     if vmax >= max(0.3, 6.0 * u_ref):
         return False, f"speed {vmax:.3g} too large"
     return True, "ok"
+
+
+def flux_correct(rho: np.ndarray, u: np.ndarray, shape, u_in: float, solid: np.ndarray) -> np.ndarray:
+    """u with its x-component rescaled slice by slice so that every x-slice
+    carries the inlet's mass flux u_in * sum_{x=0 slice} rho, as a steady flow
+    through the channel must (continuity).  A predicted field whose slices
+    carry too much or too little flux would otherwise start a channel-long
+    pressure wave that the drag residual has to wait out.  rho (N,), u (3, N),
+    solid (N,) bool; shape = (nz, ny, nx)."""
+    nz, ny, nx = shape
+    R = rho.reshape(nz, ny, nx)
+    U = u.copy().reshape(3, nz, ny, nx)
+    fl = ~solid.reshape(nz, ny, nx)
+    target = u_in * R[:, :, 0].sum()
+    flux = (R * U[0] * fl).sum(axis=(0, 1))
+    scale = np.where(flux > 0.0, target / np.where(flux > 0.0, flux, 1.0), 1.0)
+    U[0] *= scale[None, None, :]
+    return U.reshape(3, -1)
 
 
 def predict(net: UNet, case) -> tuple[np.ndarray, np.ndarray]:

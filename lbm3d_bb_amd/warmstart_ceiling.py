@@ -21,9 +21,11 @@ again from starts built from that state:
                 the velocity only)
   rho_mean1     the steady fields with rho shifted to a fluid mean of 1
                 (the pressure field without its level)
-  net, net_rho1, net_u_exact_rho
+  net, net_rho1, net_u_exact_rho, net_flux, net_flux_eq
                 with --params: the network's (rho, u); its u with rho = 1;
-                its u with the steady rho
+                its u with the steady rho; its fields with every x-slice's
+                flux set to the inlet's (flux_correct), with and without the
+                non-equilibrium part
 
 and the uniform start itself.  Every run records the drag (x-force summed over
 the bodies) of every step, so the steps to convergence of any residual rule
@@ -41,7 +43,7 @@ import time
 import numpy as np
 
 from .packs import make_pack
-from .warmstart import populations_from_fields
+from .warmstart import flux_correct, populations_from_fields
 
 
 def steps_to_converge(drag: np.ndarray, every: int, tol: float = 1e-4, patience: int = 3) -> int | None:
@@ -116,7 +118,8 @@ def run_case(seed: int, L: int, settle: int, steps: int, noise: list[float], log
         starts[f"noise_{e:g}"] = ("ce", e)
     if net is not None:
         rn, un = predict(net, case)
-        starts.update({"net": ("ce", "net"), "net_rho1": ("ce", "net_rho1"), "net_u_exact_rho": ("ce", "net_u")})
+        starts.update({"net": ("ce", "net"), "net_rho1": ("ce", "net_rho1"), "net_u_exact_rho": ("ce", "net_u"),
+                       "net_flux": ("ce", "net_flux"), "net_flux_eq": ("eq", "net_flux")})
         du = (un - u)[:, fluid] / case.u_in
         dr = (rn - rho)[fluid]
         res["net_pred"] = {"u_rms": float(np.sqrt((du * du).mean())),
@@ -138,6 +141,8 @@ def run_case(seed: int, L: int, settle: int, steps: int, noise: list[float], log
                 r2, u2 = np.ones_like(rho), un.copy()
             elif e == "net_u":
                 u2 = un.copy()
+            elif e == "net_flux":
+                r2, u2 = rn.copy(), flux_correct(rn, un, shape, case.u_in, solid)
             elif e:
                 # errors smooth on the scale of a sphere: rms e u_in on each velocity
                 # component, rms e on the relative deviation of rho from the rest state
