@@ -15,7 +15,7 @@ launch per time step: bulk rows, faces and body regions; k27_step for the
 D3Q27 workloads): its algorithmic bytes per launch (2 x q x s bytes per cell,
 q = 19 or 27 populations, s the storage size, all cells)
 over its mean launch time from HIP events recorded on the solver's stream
-around every 16th launch in the timed region; ``traffic`` is the HBM bytes per
+around every 4th launch in the timed region (``--sample-stride``); ``traffic`` is the HBM bytes per
 launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, when
 profiles/traffic_<workload>.json holds it.  ``cpu_baseline`` times the oracle (a
 plain C restatement, OpenMP) on the same workload for a bounded number of
@@ -192,6 +192,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-launch HIP events (no roofline)")
+    ap.add_argument("--sample-stride", type=int, default=4,
+                    help="time every n-th step launch with HIP events (an event pair costs ~3 us of the step)")
     ap.add_argument("--start", default="uniform", choices=["uniform", "net"],
                     help="net: start from the 128^3 U-Net applied tile-wise (untimed, before the warm-up)")
     ap.add_argument("--params", default=None, help="U-Net parameters (.npy) for --start net; random init if absent")
@@ -221,13 +223,13 @@ def main() -> int:
         start = net_start(s, cfg, laws, bodies, args.params)
     warm_forces = s.step(args.warmup) if args.warmup else None
     s.sync()
-    s.profile(not args.no_profile)
+    s.profile(0 if args.no_profile else max(1, args.sample_stride))
     t0 = time.perf_counter()
     forces = s.step(args.steps)
     s.sync()
     el = time.perf_counter() - t0
     prof = s.profile_read()
-    s.profile(False)
+    s.profile(0)
     info = s.info()
     rho, u = s.macros()
     s.close()

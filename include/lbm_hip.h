@@ -238,7 +238,9 @@ int lbm_conservation_static(lbm_ctx* ctx, double* mass, double* mom3);
  * completed steps, device bytes held. */
 int lbm_info(lbm_ctx* ctx, int64_t* ncells, int64_t* nbb_nodes, int64_t* nlinks,
              int64_t* steps, int64_t* device_bytes);
-/* Per-kernel HIP-event timing on the context stream (0 = off). */
+/* Per-kernel HIP-event timing on the context stream: an event pair around every
+ * enable-th step kernel launch (0 = off, 1 = every launch), at most 4096 pairs
+ * per lbm_step call. */
 int lbm_profile(lbm_ctx* ctx, int32_t enable);
 /* Totals since the last enable: summed duration (ms) and launch count of the
  * fused stream-collide kernel and of the boundary-link kernel. */

@@ -42,16 +42,31 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
-def build_lib(verbose: bool = False) -> str:
-    """Compiles each translation unit in parallel, then links liblbm_hip.so."""
+def _headers_mtime() -> float:
+    heads = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    inc = os.path.join(REPO, "include")
+    heads += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    return max(os.path.getmtime(h) for h in heads)
+
+
+def build_lib(verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles each translation unit in parallel, then links liblbm_hip.so.
+
+    incremental: skip a unit whose object is newer than its source and every
+    header (a development shortcut; the default rebuilds everything)."""
     import concurrent.futures as cf
 
     obj_dir = os.path.join(PKG_DIR, "_build")
     os.makedirs(obj_dir, exist_ok=True)
     cc = hipcc()
+    hdr_t = _headers_mtime()
 
     def compile_one(src: str) -> str:
         obj = os.path.join(obj_dir, src.replace(".hip", ".o"))
+        if incremental and os.path.exists(obj):
+            t = os.path.getmtime(obj)
+            if t > hdr_t and t > os.path.getmtime(os.path.join(CSRC, src)):
+                return obj
         cmd = [cc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -73,4 +88,4 @@ def build_lib(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build_lib(verbose=True))
+    print(build_lib(verbose=True, incremental="--incremental" in sys.argv[1:]))

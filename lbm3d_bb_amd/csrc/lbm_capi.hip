@@ -25,11 +25,11 @@ using namespace lbm;
 
 namespace {
 
+constexpr size_t kMaxProfileEvents = 8192;  // event pairs held per lbm_step call (4096 samples)
 constexpr int kChunk = 64;          // steps whose poses are uploaded with one copy
 // Bytes in front of the first population plane: the bulk pass's shifted pair
 // loads read one element before a row start (the value is replaced for x = 0).
 constexpr size_t kFrontPad = 256;
-constexpr int kProfileStride = 16;  // lbm_profile times every 16th step
 
 }  // namespace
 
@@ -113,7 +113,7 @@ struct lbm_ctx {
     bool initialised = false;
     bool raw_state = false;  // step 0: fbuf[1-cur] holds f_old_0
 
-    bool profiling = false;
+    int profile_stride = 0;  // lbm_profile: time every profile_stride-th step (0 = off)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     double bulk_ms = 0.0, link_ms = 0.0;
@@ -896,7 +896,8 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
         a.refill = (c->cfg.refill && mv) ? 1 : 0;
         a.partials = c->d_partials + (size_t)s * stride;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        const bool sample = c->profiling && ((c->step + s) % kProfileStride == 0);
+        const bool sample = c->profile_stride > 0 && ((c->step + s) % c->profile_stride == 0) &&
+                            c->ev_used + 2 <= kMaxProfileEvents;
         if (sample) {
             e0 = next_event(c);
             e1 = next_event(c);
@@ -956,7 +957,8 @@ int run_chunk(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
         void* fin = c->fbuf[c->cur];
         void* fout = c->fbuf[1 - c->cur];
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        const bool sample = c->profiling && ((c->step + s) % kProfileStride == 0);
+        const bool sample = c->profile_stride > 0 && ((c->step + s) % c->profile_stride == 0) &&
+                            c->ev_used + 2 <= kMaxProfileEvents;
         if (sample) {
             e0 = next_event(c);
             e1 = next_event(c);
@@ -1588,7 +1590,7 @@ int lbm_step(lbm_ctx* c, int32_t nsteps, double* force_out) {
     }();
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->profiling) {
+    if (c->profile_stride > 0) {
         // sampled event pairs around the step kernel
         for (size_t i = 0; i + 2 <= c->ev_used; i += 2) {
             float a = 0.f;
@@ -1739,7 +1741,7 @@ int lbm_info(lbm_ctx* c, int64_t* ncells, int64_t* nbb, int64_t* nlinks, int64_t
 
 int lbm_profile(lbm_ctx* c, int32_t enable) {
     if (!c) return LBM_ERR_ARG;
-    c->profiling = enable != 0;
+    c->profile_stride = enable > 0 ? enable : 0;
     c->bulk_ms = c->link_ms = 0.0;
     c->bulk_launches = c->link_launches = 0;
     return LBM_OK;

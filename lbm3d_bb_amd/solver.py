@@ -243,8 +243,9 @@ class Solver:
         self._call("info", *[ct.byref(x) for x in v])
         return dict(zip(("ncells", "nbb_nodes", "nlinks", "steps", "device_bytes"), (x.value for x in v)))
 
-    def profile(self, enable: bool) -> None:
-        self._call("profile", 1 if enable else 0)
+    def profile(self, stride: int) -> None:
+        """Times every `stride`-th step kernel launch with HIP events (0 = off)."""
+        self._call("profile", int(stride))
 
     def profile_read(self) -> dict:
         bm, lm = ct.c_double(0.0), ct.c_double(0.0)
