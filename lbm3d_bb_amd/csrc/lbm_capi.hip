@@ -12,6 +12,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -112,6 +113,9 @@ struct lbm_ctx {
     int64_t step = 0;
     bool initialised = false;
     bool raw_state = false;  // step 0: fbuf[1-cur] holds f_old_0
+    // D3Q27 16-bit storage, even nx >= 4: the split step (k27_pair + k27_step);
+    // LBM_Q27_SPLIT=0 in the environment keeps the single per-cell launch
+    bool q27_split = false;
 
     int profile_stride = 0;  // lbm_profile: time every profile_stride-th step (0 = off)
     std::vector<hipEvent_t> ev_pool;
@@ -760,7 +764,118 @@ q27::StepArgs q27_args(const lbm_ctx* c, int mode, const void* fin, void* fout) 
     a.fl_old = c->flags;
     a.nb = c->nb;
     a.face_u = c->d_face_u;
+    a.g.side_in = side_of(c, fin);
+    a.g.side_out = side_of(c, fout);
+    a.split = c->q27_split ? 1 : 0;
     return a;
+}
+
+// Boxes of the bodies of a split 16-bit step (q27::SplitRows): for every body
+// the cells with |x_a + 1/2 - c_a| <= ext_a on each axis in the new and in the
+// old pose (the device's near test, minimum image on periodic axes), one cell
+// wider on each side, the x bounds widened to an even first and an odd last
+// cell.  n = -1 (no split this step): a body without an analytic extent
+// (explicit flags), a non-finite pose or more boxes than fit.
+q27::SplitRows split_rows(const lbm_ctx* c, const DevBody* bnew, const DevBody* bold, int nb) {
+    q27::SplitRows r;
+    std::memset(&r, 0, sizeof(r));
+    const int L[3] = {c->g.nx, c->g.ny, c->g.nz};
+    auto fail_split = [&r]() {
+        r.n = -1;
+        return r;
+    };
+    for (int b = 0; b < nb; ++b) {
+        const DevBody* poses[2] = {&bnew[b], bold ? &bold[b] : nullptr};
+        // the two poses in one box unless a periodic wrap lies between them
+        bool merge = poses[1] != nullptr;
+        for (int a = 0; a < 3 && merge; ++a)
+            if (((c->g.per >> a) & 1) && std::fabs((double)poses[0]->c[a] - (double)poses[1]->c[a]) > 0.25 * L[a])
+                merge = false;
+        for (int k = 0; k < 2; ++k) {
+            if (poses[k] == nullptr || (k == 1 && merge)) break;
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+                double lf = 1e300, hf = -1e300;
+                for (int m = 0; m < 2; ++m) {
+                    const DevBody* B = (m == 0 || merge) ? poses[m == 0 ? k : 1] : nullptr;
+                    if (B == nullptr) continue;
+                    if (B->reach < 0.0f) return fail_split();
+                    lf = std::min(lf, (double)B->c[a] - 0.5 - (double)B->ext[a]);
+                    hf = std::max(hf, (double)B->c[a] - 0.5 + (double)B->ext[a]);
+                }
+                if (!std::isfinite(lf) || !std::isfinite(hf)) return fail_split();
+                lo[a] = (int)std::floor(lf) - 1;
+                hi[a] = (int)std::ceil(hf) + 1;
+            }
+            // x: pair-aligned; a range that wraps on a periodic x is two boxes
+            lo[0] = lo[0] >= 0 ? lo[0] & ~1 : -((-lo[0] + 1) & ~1);
+            hi[0] |= 1;
+            int xs[2][2];
+            int nxr = 0;
+            if (hi[0] - lo[0] + 1 >= L[0] || !(c->g.per & 1)) {
+                xs[0][0] = std::max(lo[0], 0);
+                xs[0][1] = std::min(hi[0], L[0] - 1);
+                if (hi[0] - lo[0] + 1 >= L[0]) {
+                    xs[0][0] = 0;
+                    xs[0][1] = L[0] - 1;
+                }
+                nxr = xs[0][1] >= xs[0][0] ? 1 : 0;
+            } else {
+                const int l = ((lo[0] % L[0]) + L[0]) % L[0], h = l + (hi[0] - lo[0]);
+                if (h < L[0]) {
+                    xs[0][0] = l;
+                    xs[0][1] = h;
+                    nxr = 1;
+                } else {
+                    xs[0][0] = l;
+                    xs[0][1] = L[0] - 1;
+                    xs[1][0] = 0;
+                    xs[1][1] = h - L[0];
+                    nxr = 2;
+                }
+            }
+            // y, z: wrapped start and extent (periodic) or clamped (else)
+            int y0 = 0, ey = 0, z0 = 0, ez = 0;
+            for (int a = 1; a < 3; ++a) {
+                int l = lo[a], h = hi[a], e;
+                if ((c->g.per >> a) & 1) {
+                    e = std::min(h - l + 1, L[a]);
+                    l = ((l % L[a]) + L[a]) % L[a];
+                } else {
+                    l = std::max(l, 0);
+                    h = std::min(h, L[a] - 1);
+                    e = std::max(h - l + 1, 0);
+                }
+                (a == 1 ? y0 : z0) = l;
+                (a == 1 ? ey : ez) = e;
+            }
+            if (ey == 0 || ez == 0) continue;  // entirely beyond a face
+            for (int i = 0; i < nxr; ++i) {
+                if (r.n == q27::kSplitBoxes) return fail_split();
+                r.x0[r.n] = xs[i][0];
+                r.ex[r.n] = xs[i][1] - xs[i][0] + 1;
+                r.y0[r.n] = y0;
+                r.ey[r.n] = ey;
+                r.z0[r.n] = z0;
+                r.ez[r.n] = ez;
+                // blocks an x range of ex cells can span in one row
+                r.bpx[r.n] = (r.ex[r.n] + kBlock - 2) / kBlock + 1;
+                ++r.n;
+            }
+        }
+    }
+    // enumeration of the per-cell blocks: face rows, then each box's rows
+    const bool yf = !((c->g.per >> 1) & 1), zf = !((c->g.per >> 2) & 1);
+    r.bpr = L[0] % kBlock == 0 ? L[0] / kBlock : L[0] / kBlock + 2;
+    r.nface = (zf ? 2 * L[1] : 0) + (yf ? 2 * (zf ? L[2] - 2 : L[2]) : 0);
+    int64_t v = (int64_t)r.nface * r.bpr;
+    for (int k = 0; k < r.n; ++k) {
+        r.base[k] = (int)v;
+        v += (int64_t)r.ey[k] * r.ez[k] * r.bpx[k];
+    }
+    r.base[r.n] = (int)v;
+    if (v >= ((int64_t)1 << 31)) r.n = -1;
+    return r;
 }
 
 int q27_step_launch(lbm_ctx* c, const q27::StepArgs& a) {
@@ -895,6 +1010,11 @@ int run_chunk27(lbm_ctx* c, int nsteps, double* force_out, int64_t* done) {
         a.bold = mv ? dn + nb : nullptr;
         a.refill = (c->cfg.refill && mv) ? 1 : 0;
         a.partials = c->d_partials + (size_t)s * stride;
+        if (a.split) {
+            const DevBody* hb = (const DevBody*)(c->h_rec[par] + (size_t)s * c->rec_bytes);
+            a.rows = split_rows(c, hb, mv ? hb + nb : nullptr, nb);
+            if (a.rows.n < 0) a.split = 0;
+        }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         const bool sample = c->profile_stride > 0 && ((c->step + s) % c->profile_stride == 0) &&
                             c->ev_used + 2 <= kMaxProfileEvents;
@@ -1192,6 +1312,11 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     lbm_ctx* c = new lbm_ctx();
     c->cfg = *cfg;
     c->cfg.q = q;
+    {
+        const char* e = std::getenv("LBM_Q27_SPLIT");
+        c->q27_split = q == 27 && (cfg->storage == LBM_FP16S || cfg->storage == LBM_BF16S) && cfg->nx % 2 == 0 &&
+                       cfg->nx >= 4 && !(e && e[0] == '0');
+    }
     c->q = q;
     c->g.nx = cfg->nx;
     c->g.ny = cfg->ny;

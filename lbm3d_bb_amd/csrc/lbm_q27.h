@@ -13,6 +13,31 @@ constexpr int kX[kQ] = {1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 
 constexpr int kY[kQ] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1, 0};
 constexpr int kZ[kQ] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 0};
 
+// Cells of a split 16-bit step that the per-cell blocks take besides the rows
+// on non-periodic y/z faces: per body, a box around its cells within reach in
+// the old and the new pose (host-computed from the step's bodies, a cell wider
+// on each side than the device's near test, x bounds on even / odd cells so
+// that no pair of the pair blocks is cut).  y0/z0 in [0, L) (they wrap on
+// periodic axes); x0 + ex <= nx (a box that would wrap in x is two boxes);
+// cell (x, y, z) lies in box k when x - x0[k] < ex[k], (y - y0[k]) mod ny < ey[k]
+// and (z - z0[k]) mod nz < ez[k].
+// The per-cell blocks of a split step are enumerated from these values alone:
+// first the rows on the non-periodic y/z faces (nface rows, bpr block slots
+// per row: the blocks a row of nx cells can span), then the rows of each box
+// (bpx[k] slots per row: the blocks its x range can span); base[k] is the
+// first slot of box k, base[n] the total.  A block reached
+// twice (rows that share a block, overlapping boxes) computes the same values
+// twice, which is harmless.
+constexpr int kSplitBoxes = 32;
+struct SplitRows {
+    int n;
+    int y0[kSplitBoxes], ey[kSplitBoxes], z0[kSplitBoxes], ez[kSplitBoxes];
+    int x0[kSplitBoxes], ex[kSplitBoxes], bpx[kSplitBoxes];
+    int bpr;
+    int nface;
+    int base[kSplitBoxes + 1];
+};
+
 // One launch of k27_step: a time step (MODE_STEP), the getters' replay of the
 // last step (MODE_GET) or of the start state (MODE_GET_RAW), or the collision
 // of the start state (MODE_COLLIDE_ONLY).
@@ -39,6 +64,11 @@ struct StepArgs {
     double* partials;  // MODE_STEP: [blocks][nb][6], written by the blocks within reach of a body
     int refill;
     const double* face_u;  // [6][3]
+    // 16-bit storage, even nx >= 4: MODE_STEP runs k27_split (pair blocks over
+    // the plain rows, per-cell blocks over the rest, `rows`); every mode that
+    // stores f* writes the x-face slots
+    int split;
+    SplitRows rows;
 };
 
 }  // namespace q27

@@ -53,8 +53,18 @@ struct StoreF16S27 {
         __builtin_nontemporal_store(__half_as_ushort(h), reinterpret_cast<unsigned short*>(p));
     }
     __device__ static inline void store_sr(T* p, int, C dv, const Shift& sh, uint32_t r) {
-        const uint32_t h = f16x2_sr(dv * sh.k, 0.0f, r);
-        __builtin_nontemporal_store((unsigned short)(h & 0xffffu), reinterpret_cast<unsigned short*>(p));
+        __builtin_nontemporal_store((unsigned short)sr_bits(dv, sh, r), reinterpret_cast<unsigned short*>(p));
+    }
+    // the 16 bits store_sr writes, and the deviation they hold
+    __device__ static inline uint32_t sr_bits(C dv, const Shift& sh, uint32_t r) {
+        return f16x2_sr(dv * sh.k, 0.0f, r) & 0xffffu;
+    }
+    __device__ static inline C dec(uint32_t h, const Shift& sh) {
+        return __half2float(__ushort_as_half((unsigned short)(h & 0xffffu))) * sh.ik;
+    }
+    // two cells in one word (low: a), each rounded as store_sr rounds it
+    __device__ static inline uint32_t sr_pair(C a, C b, const Shift& sh, uint32_t ra, uint32_t rb) {
+        return f16x2_sr(a * sh.k, b * sh.k, (ra & 0x1fffu) | ((rb & 0x1fffu) << 16));
     }
 };
 
@@ -68,6 +78,11 @@ struct StoreBF16S27 {
     }
     __device__ static inline void store_sr(T* p, int, C dv, const Shift&, uint32_t r) {
         __builtin_nontemporal_store((uint16_t)bf16_sr(dv, r), p);
+    }
+    __device__ static inline uint32_t sr_bits(C dv, const Shift&, uint32_t r) { return bf16_sr(dv, r) & 0xffffu; }
+    __device__ static inline C dec(uint32_t h, const Shift&) { return __uint_as_float(h << 16); }
+    __device__ static inline uint32_t sr_pair(C a, C b, const Shift&, uint32_t ra, uint32_t rb) {
+        return (bf16_sr(a, ra) & 0xffffu) | (bf16_sr(b, rb) << 16);
     }
 };
 
@@ -308,17 +323,22 @@ __device__ inline RuleFaces rule_faces_of(const Grid& g) {
     return r;
 }
 
-// Whether the rows (y, z) of this block's cells come within reach of a body,
-// from block-uniform values: the y and z intervals of the rows, against each
-// body's extent (min image on periodic axes; an interval of half the axis or
-// more counts as near).
-__device__ inline bool interval_near(int lo, int hi, float c, float ext, bool per, int L) {
-    const float half = 0.5f * (float)(hi - lo + 1);
-    if (per && 2 * (hi - lo + 1) >= L) return true;
-    const float m = 0.5f * (float)(lo + hi + 1);
-    return fabsf(min_image(m - c, per, L)) <= ext + half;
+// Distance from the point c to the cell centres x + 1/2, x in [xa, xb], along
+// an axis of length L (minimum image on a periodic axis).
+__device__ inline float span_dist(int xa, int xb, float c, bool per, int L) {
+    float best = 3.0e38f;
+    for (int k = per ? -1 : 0; k <= (per ? 1 : 0); ++k) {
+        const float q = c + (float)(k * L);
+        const float d = fmaxf(fmaxf((float)xa + 0.5f - q, q - ((float)xb + 0.5f)), 0.0f);
+        best = fminf(best, d);
+    }
+    return best;
 }
 
+// Whether a cell of this block comes within reach of a body (near_body), row
+// by row (a block spans at most 256 / nx + 2 rows) with the block's x span in
+// each: a block is near only when one of its cells is, and the split step's
+// host-computed boxes contain every such cell.
 __device__ inline bool block_near(uint32_t block, const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
                                   int nb, const Grid& g) {
     if (nb <= 0) return false;
@@ -326,20 +346,24 @@ __device__ inline bool block_near(uint32_t block, const DevBody* __restrict__ bn
     const uint32_t c0 = block * (uint32_t)kBlock;
     const uint32_t c1 = (c0 + kBlock < (uint32_t)g.N ? c0 + kBlock : (uint32_t)g.N) - 1u;
     const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
-    const uint32_t r0 = c0 / nx, r1 = c1 / nx;  // rows y + ny z
-    const int z0 = (int)(r0 / ny), z1 = (int)(r1 / ny);
-    const int y0 = z0 == z1 ? (int)(r0 - (uint32_t)z0 * ny) : 0;
-    const int y1 = z0 == z1 ? (int)(r1 - (uint32_t)z1 * ny) : g.ny - 1;
-    bool hit = false;
-    for (int b = 0; b < nb && !hit; ++b) {
-        for (int k = 0; k < 2 && !hit; ++k) {
-            if (k == 1 && bold == nullptr) break;
-            const DevBody& B = k == 0 ? bnew[b] : bold[b];
-            hit = B.reach < 0.0f || (interval_near(y0, y1, B.c[1], B.ext[1], g.per & 2, g.ny) &&
-                                     interval_near(z0, z1, B.c[2], B.ext[2], g.per & 4, g.nz));
+    for (uint32_t r = c0 / nx; r <= c1 / nx; ++r) {
+        const float py = (float)(r % ny) + 0.5f, pz = (float)(r / ny) + 0.5f;
+        const int xa = r == c0 / nx ? (int)(c0 - r * nx) : 0;
+        const int xb = r == c1 / nx ? (int)(c1 - r * nx) : g.nx - 1;
+        for (int b = 0; b < nb; ++b) {
+            for (int k = 0; k < 2; ++k) {
+                if (k == 1 && bold == nullptr) break;
+                const DevBody& B = k == 0 ? bnew[b] : bold[b];
+                if (B.reach < 0.0f) return true;
+                const float dy = min_image(py - B.c[1], g.per & 2, g.ny);
+                const float dz = min_image(pz - B.c[2], g.per & 4, g.nz);
+                if (fabsf(dy) <= B.ext[1] && fabsf(dz) <= B.ext[2] &&
+                    span_dist(xa, xb, B.c[0], g.per & 1, g.nx) <= B.ext[0])
+                    return true;
+            }
         }
     }
-    return hit;
+    return false;
 }
 
 // Pull sources x - c of a cell for the offsets c = -1, 0, 1 on each axis
@@ -500,30 +524,118 @@ __device__ inline double wrap_arm(double r, int a, const Grid& g) {
     return r;
 }
 
+// ---------------------------------------------------------------------------
+// The split 16-bit step: two launches per step.  k27_pair takes every cell of a
+// row off the non-periodic y/z faces, in a block no body comes near
+// (plain_pair), two x-neighbours per thread with 4-byte pair loads and stores:
+// a one-cell-per-thread 16-bit step issues one 2-byte access per lane and
+// direction, as many instructions as FP32 for half the bytes.  Its x-face
+// cells are part of their rows' waves: periodic x wraps with one 2-byte load,
+// a zero-gradient face clamps (a value select), a velocity / anti-bounce-back
+// face applies its rule from the cell's own f*[opp d] (one extra load per
+// direction pair) and the previous-step moments of the cell in its face slot
+// (one 16-byte load), which whichever kernel produced the cell's f* wrote
+// (q27_slot_write).  k27_step (split = 1) takes the rest: cells on the y/z
+// rule rows and every cell of a block within reach of a body, the same
+// partition by the same tests.
+// ---------------------------------------------------------------------------
+
+// A row on a non-periodic y or z face (the per-cell kernel's).
+__device__ inline bool face_row27(int y, int z, const Grid& g) {
+    return (!(g.per & 2) && (y == 0 || y == g.ny - 1)) || (!(g.per & 4) && (z == 0 || z == g.nz - 1));
+}
+
+// A cell of the per-cell blocks in a split step: on a y/z face row or in a
+// body's box (the same answer for both cells of a pair: boxes start on even x
+// and end on odd x).
+__device__ inline bool percell_cell(int x, int y, int z, const Grid& g, const SplitRows& r) {
+    if (face_row27(y, z, g)) return true;
+    bool in = false;
+    for (int k = 0; k < r.n && !in; ++k) {
+        int dy = y - r.y0[k], dz = z - r.z0[k];
+        if (dy < 0) dy += g.ny;
+        if (dz < 0) dz += g.nz;
+        in = (unsigned)(x - r.x0[k]) < (unsigned)r.ex[k] && dy < r.ey[k] && dz < r.ez[k];
+    }
+    return in;
+}
+
+// Slot of the x-face rule cell (x, y, z) on face f (0: x = 0, 1: x = nx - 1).
+__device__ inline int x_slot(const Grid& g, int f, int y, int z) { return g.face_base[f] + y + g.ny * z; }
+
+// The previous-step moments that an x-face rule reads, from a cell's
+// populations as stored (decoded deviations dv): what the per-cell kernel
+// computes from its own loads next step.
+// Sum over the 64 lanes of a wave, in a fixed butterfly order (every lane gets it).
+template <typename C>
+__device__ __forceinline__ C wave_sum_any(C v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Running sums of moments27 (the same additions in the same order), one
+// population at a time.
+template <typename C>
+__device__ __forceinline__ void moment_add(int d, C v, C& rho, C& jx, C& jy, C& jz) {
+    rho += v;
+    if (kX[d] > 0) jx += v;
+    if (kX[d] < 0) jx -= v;
+    if (kY[d] > 0) jy += v;
+    if (kY[d] < 0) jy -= v;
+    if (kZ[d] > 0) jz += v;
+    if (kZ[d] < 0) jz -= v;
+}
+
+template <class S, bool FORCED>
+__device__ inline void q27_slot_put(const Grid& g, typename S::C rho, typename S::C jx, typename S::C jy,
+                                    typename S::C jz, const Phys<typename S::C>& p, int slot) {
+    using C = typename S::C;
+    C rp, up[3], uu2;
+    prev_moments<S, FORCED>(rho, jx, jy, jz, p, rp, up, uu2);
+    C* o = reinterpret_cast<C*>(g.side_out) + 4 * (int64_t)slot;
+    o[0] = rp;
+    o[1] = up[0];
+    o[2] = up[1];
+    o[3] = up[2];
+}
+
 // IBB: the interpolated bounce-back (IBB!, src/SolidBC.jl:253-350) on the links
 // instead of the simple one: wall fraction q from the body's level set at the
 // cell and at the solid node (FP32, clamped to [kEpsQ, 1 - kEpsQ]); rule 1
 // (q < 1/2 with a fluid second node x - c_dl), rule 2 (q >= 1/2, the wall
 // term over 2q), else the simple rule; wall velocity and torque arm at x + q c_dl.
 template <class S, int COLL, bool FORCED, int MODE, bool IBB>
-__global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                                   double* __restrict__ gout, double* __restrict__ rho_out,
-                                                   double* __restrict__ u_out, Grid g, Phys<typename S::C> p,
-                                                   const uint8_t* __restrict__ fl_new, const uint8_t* __restrict__ fl_old,
-                                                   const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
-                                                   int nb, double* __restrict__ partials, int refill,
-                                                   const double* __restrict__ face_u) {
+__device__ __forceinline__ void step_block(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                           double* __restrict__ gout, double* __restrict__ rho_out,
+                                           double* __restrict__ u_out, const Grid& g, const Phys<typename S::C>& p,
+                                           const uint8_t* __restrict__ fl_new, const uint8_t* __restrict__ fl_old,
+                                           const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold, int nb,
+                                           double* __restrict__ partials, int refill,
+                                           const double* __restrict__ face_u, int split, const SplitRows& rows,
+                                           uint32_t blk) {
     using C = typename S::C;
     extern __shared__ double lds_dyn[];
+    const int64_t cell = (int64_t)blk * kBlock + threadIdx.x;
+    // split step: the pair blocks take the cells off the y/z face rows and
+    // outside the bodies' boxes (percell_cell, from kernel arguments alone)
+    bool mine = true;
+    if (MODE == MODE_STEP && split && cell < g.N) {
+        int x, y, z;
+        xyz_of(cell, g, x, y, z);
+        mine = percell_cell(x, y, z, g, rows);
+    }
     const int n6 = nb * 6;
     double* s_wave = lds_dyn + (threadIdx.x >> 6) * n6;  // this wave's force slots
     // IBB: the wall fraction of each link direction, one column per thread
     float* lds_q = reinterpret_cast<float*>(lds_dyn + kWaves * n6) + threadIdx.x;
     // Only blocks within reach of a body can hold links: they alone reduce
     // forces and write their partials (the sum visits the same blocks).
-    const bool bnear = MODE != MODE_COLLIDE_ONLY && MODE != MODE_GET_RAW && block_near(blockIdx.x, bnew, bold, nb, g);
+    const bool bnear = MODE != MODE_COLLIDE_ONLY && MODE != MODE_GET_RAW && block_near(blk, bnew, bold, nb, g);
+    __shared__ uint32_t s_waves_done;
     if (MODE == MODE_STEP && bnear) {
         for (int i = threadIdx.x; i < kWaves * n6; i += kBlock) lds_dyn[i] = 0.0;
+        if (threadIdx.x == 0) s_waves_done = 0u;
         __syncthreads();
     }
     ForceAcc acc;
@@ -531,8 +643,58 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc.a0[c] = 0.0;
     const int64_t N = g.N;
-    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (cell < N) {
+    // Fresh nodes (refill): the mean previous-step rho, u of the neighbours
+    // x + c_d that are fluid and not fresh, gathered by the whole wave, one
+    // neighbour per lane (two memory round trips per fresh node), summed in a
+    // fixed shuffle order.  rf_*: the sums for this lane's node, rf_n the count.
+    bool fresh_l = false;
+    C rf_r = C(0), rf_x = C(0), rf_y = C(0), rf_z = C(0);
+    int rf_n = 0;
+    if ((MODE == MODE_STEP || MODE == MODE_GET) && bnear && refill) {
+        int x = 0, y = 0, z = 0;
+        if (cell < N && mine) {
+            xyz_of(cell, g, x, y, z);
+            bool nr = false;
+            for (int b = 0; b < nb && !nr; ++b)
+                nr = near_body(bnew[b], x, y, z, g) || (bold != nullptr && near_body(bold[b], x, y, z, g));
+            fresh_l = nr && (fl_new[cell] & kFlagBody) == 0 && (fl_old[cell] & kFlagBody) != 0;
+        }
+        const int lane = (int)(threadIdx.x & 63u);
+        uint64_t fm = __ballot(fresh_l);
+        while (fm) {
+            const int L = __ffsll((unsigned long long)fm) - 1;
+            const int q0[3] = {__shfl(x, L, 64), __shfl(y, L, 64), __shfl(z, L, 64)};
+            int64_t nc = -1;
+            if (lane < QQ - 1) {
+#pragma unroll
+                for (int d = 0; d < QQ - 1; ++d)
+                    if (d == lane) nc = in_domain_source(q0, opp(d), g);
+            }
+            bool ok = nc >= 0;
+            if (ok) ok = (fl_new[nc] & kFlagBody) == 0 && (fl_old[nc] & kFlagBody) == 0;
+            C rr = C(0), uu[3] = {C(0), C(0), C(0)};
+            if (ok) {
+                C r = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+                for (int e = 0; e < QQ; ++e)
+                    moment_add<C>(e, S::load(fin + (int64_t)e * g.P + nc, e, g.sh), r, jx, jy, jz);
+                C u2;
+                prev_moments<S, FORCED>(r, jx, jy, jz, p, rr, uu, u2);
+            }
+            const C sr = wave_sum_any<C>(rr), sx = wave_sum_any<C>(uu[0]), sy = wave_sum_any<C>(uu[1]),
+                    sz = wave_sum_any<C>(uu[2]);
+            const int n = __popcll(__ballot(ok));
+            if (lane == L) {
+                rf_r = sr;
+                rf_x = sx;
+                rf_y = sy;
+                rf_z = sz;
+                rf_n = n;
+            }
+            fm &= fm - 1;
+        }
+    }
+    if (cell < N && mine) {
         C f[QQ];
         if (MODE == MODE_COLLIDE_ONLY || MODE == MODE_GET_RAW) {
 #pragma unroll
@@ -553,7 +715,7 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                     near = near_body(bnew[b], x, y, z, g) || (bold != nullptr && near_body(bold[b], x, y, z, g));
             }
             const int self = near ? (fl_new[cell] & kFlagBody) : 0;
-            const bool fresh = near && refill && self == 0 && (fl_old[cell] & kFlagBody) != 0;
+            const bool fresh = fresh_l;  // (near && refill && self == 0 && solid in the old flags)
             // links to a solid source (lmask)
             uint32_t lmask = 0u;
             int lbody = -1;       // body of the links (the last one met)
@@ -601,35 +763,10 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                 }
             }
             if (fresh) {
-                // equilibrium of the mean previous-step rho, u of the neighbours
-                // x + c_d that are fluid and not fresh (refill, extension)
-                // (a rare path: one neighbour at a time keeps the registers of
-                // the step's other paths low)
-                C sr = C(0), su = C(0), sv = C(0), sw = C(0);
-                int cnt = 0;
-                for (int d = 0; d < QQ - 1; ++d) {
-                    const int64_t nc = in_domain_source(q0, opp(d), g);
-                    if (nc < 0 || (fl_new[nc] & kFlagBody) != 0 || (fl_old[nc] & kFlagBody) != 0) continue;
-                    C r = C(0), jx = C(0), jy = C(0), jz = C(0);
-#pragma unroll
-                    for (int e = 0; e < QQ; ++e) {
-                        const C v = S::load(fin + (int64_t)e * g.P + nc, e, g.sh);
-                        r += v;
-                        if (kX[e] > 0) jx += v;
-                        if (kX[e] < 0) jx -= v;
-                        if (kY[e] > 0) jy += v;
-                        if (kY[e] < 0) jy -= v;
-                        if (kZ[e] > 0) jz += v;
-                        if (kZ[e] < 0) jz -= v;
-                    }
-                    C rr, uu[3], u2;
-                    prev_moments<S, FORCED>(r, jx, jy, jz, p, rr, uu, u2);
-                    sr += rr;
-                    su += uu[0];
-                    sv += uu[1];
-                    sw += uu[2];
-                    ++cnt;
-                }
+                // equilibrium of the neighbours' mean previous-step rho, u
+                // (gathered above by the wave)
+                const C sr = rf_r, su = rf_x, sv = rf_y, sw = rf_z;
+                const int cnt = rf_n;
                 // (shifted formats: sr sums rho - 1, and f holds f_eq - w)
                 C r = S::kDev ? C(g.rho0 - 1.0) : C(g.rho0), ux = C(0), uy = C(0), uz = C(0);
                 if (cnt > 0) {
@@ -828,16 +965,275 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
             const uint32_t h = sr_cell(cell, g.sh.seed);
 #pragma unroll
             for (int d = 0; d < QQ; ++d) S::store_sr(fout + (int64_t)d * g.P + cell, d, f[d], g.sh, sr_dir(h, d));
+            if constexpr (S::kDev) {
+                // split step: the face slot of an x-face rule cell, for k27_pair's next step
+                if (split) {
+                    const uint32_t xc = (uint32_t)(cell % g.nx);
+                    const RuleFaces rf = rule_faces_of(g);
+                    const int fx = xc == 0u && ((rf.vel | rf.abb) & 1u) ? 0
+                                   : (xc == (uint32_t)g.nx - 1u && ((rf.vel | rf.abb) & 2u) ? 1 : -1);
+                    if (fx >= 0) {
+                        const uint32_t row = (uint32_t)(cell / g.nx);
+                        const int y = (int)(row % (uint32_t)g.ny), z = (int)(row / (uint32_t)g.ny);
+                        C sr = C(0), sx = C(0), sy = C(0), sz = C(0);
+#pragma unroll
+                        for (int d = 0; d < QQ; ++d)
+                            moment_add<C>(d, S::dec(S::sr_bits(f[d], g.sh, sr_dir(h, d)), g.sh), sr, sx, sy, sz);
+                        q27_slot_put<S, FORCED>(g, sr, sx, sy, sz, p, x_slot(g, fx, y, z));
+                    }
+                }
+            }
         }
     }
     if (MODE == MODE_STEP && bnear) {
+        // the last wave of the block to finish adds the waves' slots in a fixed
+        // order (no barrier: the other waves leave as soon as they are done)
         wave_reduce_to_lds(acc.b0, acc.a0, s_wave);
-        __syncthreads();
-        for (int i = threadIdx.x; i < n6; i += kBlock) {
-            double s = lds_dyn[i];
-            for (int w = 1; w < kWaves; ++w) s += lds_dyn[w * n6 + i];
-            partials[(int64_t)blockIdx.x * n6 + i] = s;
+        __threadfence_block();
+        uint32_t done = 0u;
+        if ((threadIdx.x & 63u) == 0u) done = atomicAdd(&s_waves_done, 1u);
+        done = (uint32_t)__shfl((int)done, 0, 64);
+        if (done == (uint32_t)kWaves - 1u) {
+            __threadfence_block();
+            for (int i = (int)(threadIdx.x & 63u); i < n6; i += 64) {
+                double s = lds_dyn[i];
+                for (int w = 1; w < kWaves; ++w) s += lds_dyn[w * n6 + i];
+                partials[(int64_t)blk * n6 + i] = s;
+            }
         }
+    }
+}
+
+// index of a direction with c_x = +1 among the nine (-1 otherwise)
+__host__ __device__ constexpr int xpos_index(int d) {
+    int k = 0;
+    for (int e = 0; e < QQ; ++e) {
+        if (kX[e] <= 0) continue;
+        if (e == d) return k;
+        ++k;
+    }
+    return -1;
+}
+
+__device__ inline uint32_t load_word(const void* p) {
+    uint32_t v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+__device__ inline uint32_t load_half(const void* p) { return *reinterpret_cast<const uint16_t*>(p); }
+
+// x treatment of a wave of pairs
+enum { PX_NONE = 0, PX_WRAP = 1, PX_CLAMP = 2, PX_RULE = 3 };
+
+template <class S, int COLL, bool FORCED, int XM>
+__device__ __forceinline__ void pair_cells(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                           const Grid& g, const Phys<float>& p, const double* __restrict__ face_u,
+                                           int x0, int y, int z) {
+    using T = typename S::T;
+    const int nx = g.nx, nxy = (int)g.nxy;
+    const int row = nx * y + nxy * z;
+    const bool left = x0 == 0, right = x0 + 2 == nx;
+    const RuleFaces rf = rule_faces_of(g);
+    const bool rule0 = ((rf.vel | rf.abb) & 1u) != 0u, rule1 = ((rf.vel | rf.abb) & 2u) != 0u;
+    const bool rl = XM == PX_RULE && left && rule0;
+    const bool rr = XM == PX_RULE && right && rule1;
+    // periodic y/z wraps by index arithmetic (rows on non-periodic y/z faces are not here)
+    const int wym = (y == 0) ? nxy : 0, wyp = (y == g.ny - 1) ? -nxy : 0;
+    const int wzm = (z == 0) ? (int)g.N : 0, wzp = (z == g.nz - 1) ? -(int)g.N : 0;
+    float rho_prev = 0.f, up[3] = {0.f, 0.f, 0.f}, uu2 = 0.f;
+    if (XM == PX_RULE) {
+        // every lane reads a valid slot; only rule lanes use it
+        int slot = rule0 ? x_slot(g, 0, y, z) : x_slot(g, 1, y, z);
+        if (rr && rule0) slot = x_slot(g, 1, y, z);
+        const float4 sv = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(g.side_in) + 4 * (int64_t)slot);
+        rho_prev = sv.x;
+        up[0] = sv.y;
+        up[1] = sv.z;
+        up[2] = sv.w;
+        uu2 = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
+    }
+    // w[d]: the pulled pair, cell x0 in the low half and x0 + 1 in the high
+    // half, as stored (16 bits each); decoded one cell at a time
+    uint32_t w[QQ];
+#pragma unroll
+    for (int d = 0; d < QQ; ++d) {
+        int srow = row - (kY[d] * nx + kZ[d] * nxy);
+        if (kY[d] > 0) srow += wym;
+        if (kY[d] < 0) srow += wyp;
+        if (kZ[d] > 0) srow += wzm;
+        if (kZ[d] < 0) srow += wzp;
+        const T* plane = fin + (int64_t)d * g.P + srow;
+        if (kX[d] == 0) {
+            w[d] = load_word(plane + x0);
+        } else if (kX[d] > 0) {
+            // sources (x0 - 1, x0); the left pair loads (0, 1) so that no load
+            // leaves the row: x = -1 wraps, clamps onto x = 0 or takes the rule
+            const uint32_t v = load_word((XM != PX_NONE && left) ? plane : plane + x0 - 1);
+            if (XM == PX_NONE) {
+                w[d] = v;
+            } else {
+                const uint32_t lo = XM == PX_WRAP ? load_half(left ? plane + nx - 1 : plane + x0 - 1) : (v & 0xffffu);
+                w[d] = left ? (lo | (v << 16)) : ((v & 0xffff0000u) | lo);
+            }
+        } else {
+            // sources (x0 + 1, x0 + 2); the right pair loads (nx - 2, nx - 1)
+            const uint32_t v = load_word((XM != PX_NONE && right) ? plane + x0 : plane + x0 + 1);
+            if (XM == PX_NONE) {
+                w[d] = v;
+            } else {
+                const uint32_t hi = XM == PX_WRAP ? load_half(right ? plane : plane + x0 + 2) : (v >> 16);
+                w[d] = right ? ((v >> 16) | (hi << 16)) : ((v & 0xffffu) | (hi << 16));
+            }
+        }
+    }
+    const int64_t c0 = (int64_t)row + x0;
+    float f[QQ];
+    // cell x0: decode, face rule (x = 0), collide; its rounded bits replace the
+    // low halves
+    {
+#pragma unroll
+        for (int d = 0; d < QQ; ++d) f[d] = S::dec(w[d], g.sh);
+        if (XM == PX_RULE && rl) {
+            // d leaves x = 0 inward: its rule starts from the cell's own f*[opp d]
+            // (loads of this lane alone, once the pulls have arrived)
+            const float uf[3] = {(float)face_u[0], (float)face_u[1], (float)face_u[2]};
+            const bool vel = (rf.vel & 1u) != 0u;
+#pragma unroll
+            for (int d = 0; d < QQ; ++d) {
+                if (kX[d] <= 0) continue;
+                const float o = S::dec(load_half(fin + (int64_t)opp(d) * g.P + row), g.sh);
+                f[d] = vel ? velocity_rule<S>(d, o, uf, rho_prev) : abb_rule<S>(d, o, up, uu2, rho_prev);
+            }
+        }
+        collide27_dev<COLL, FORCED>(f, p);
+        const uint32_t h = sr_cell(c0, g.sh.seed);
+        float sr = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;  // the cell as stored: its face slot
+#pragma unroll
+        for (int d = 0; d < QQ; ++d) {
+            const uint32_t b = S::sr_bits(f[d], g.sh, sr_dir(h, d));
+            w[d] = (w[d] & 0xffff0000u) | b;
+            if (XM == PX_RULE) moment_add<float>(d, S::dec(b, g.sh), sr, sx, sy, sz);
+        }
+        if (XM == PX_RULE && rl) q27_slot_put<S, FORCED>(g, sr, sx, sy, sz, p, x_slot(g, 0, y, z));
+    }
+    // cell x0 + 1: the same with the high halves (face rule at x = nx - 1)
+    {
+#pragma unroll
+        for (int d = 0; d < QQ; ++d) f[d] = S::dec(w[d] >> 16, g.sh);
+        if (XM == PX_RULE && rr) {
+            // d leaves x = nx - 1 inward: own f*[opp d] there
+            const float uf[3] = {(float)face_u[3], (float)face_u[4], (float)face_u[5]};
+            const bool vel = (rf.vel & 2u) != 0u;
+#pragma unroll
+            for (int d = 0; d < QQ; ++d) {
+                if (kX[d] >= 0) continue;
+                const float o = S::dec(load_half(fin + (int64_t)opp(d) * g.P + row + nx - 1), g.sh);
+                f[d] = vel ? velocity_rule<S>(d, o, uf, rho_prev) : abb_rule<S>(d, o, up, uu2, rho_prev);
+            }
+        }
+        collide27_dev<COLL, FORCED>(f, p);
+        const uint32_t h = sr_cell(c0 + 1, g.sh.seed);
+        T* o = fout + c0;
+        float sr = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+        for (int d = 0; d < QQ; ++d) {
+            const uint32_t b = S::sr_bits(f[d], g.sh, sr_dir(h, d));
+            __builtin_nontemporal_store((w[d] & 0xffffu) | (b << 16), reinterpret_cast<uint32_t*>(o + (int64_t)d * g.P));
+            if (XM == PX_RULE) moment_add<float>(d, S::dec(b, g.sh), sr, sx, sy, sz);
+        }
+        if (XM == PX_RULE && rr) q27_slot_put<S, FORCED>(g, sr, sx, sy, sz, p, x_slot(g, 1, y, z));
+    }
+}
+
+// Thread t of the pair blocks: cells 2t and 2t + 1 (one row: nx is even),
+// unless the per-cell blocks take them (percell_cell).
+template <class S, int COLL, bool FORCED>
+__device__ __forceinline__ void pair_block(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                           const Grid& g, const Phys<float>& p, const double* __restrict__ face_u,
+                                           const SplitRows& rows, uint32_t pblk) {
+    const int64_t t = (int64_t)pblk * kBlock + threadIdx.x;
+    const int64_t c0 = 2 * t;
+    if (c0 >= g.N) return;
+    int x0, y, z;
+    xyz_of(c0, g, x0, y, z);
+    if (percell_cell(x0, y, z, g, rows)) return;
+    const bool edge = __ballot(x0 == 0 || x0 + 2 == g.nx) != 0ull;
+    if (!edge) pair_cells<S, COLL, FORCED, PX_NONE>(fin, fout, g, p, face_u, x0, y, z);
+    else if (g.per & 1) pair_cells<S, COLL, FORCED, PX_WRAP>(fin, fout, g, p, face_u, x0, y, z);
+    else if (((rule_faces_of(g).vel | rule_faces_of(g).abb) & 3u) != 0u)
+        pair_cells<S, COLL, FORCED, PX_RULE>(fin, fout, g, p, face_u, x0, y, z);
+    else pair_cells<S, COLL, FORCED, PX_CLAMP>(fin, fout, g, p, face_u, x0, y, z);
+}
+
+// The step kernel (one thread per cell, blocks of 256 cells in cell order).
+template <class S, int COLL, bool FORCED, int MODE, bool IBB>
+__global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                   double* __restrict__ gout, double* __restrict__ rho_out,
+                                                   double* __restrict__ u_out, Grid g, Phys<typename S::C> p,
+                                                   const uint8_t* __restrict__ fl_new, const uint8_t* __restrict__ fl_old,
+                                                   const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
+                                                   int nb, double* __restrict__ partials, int refill,
+                                                   const double* __restrict__ face_u, int split, SplitRows rows) {
+    step_block<S, COLL, FORCED, MODE, IBB>(fin, fout, gout, rho_out, u_out, g, p, fl_new, fl_old, bnew, bold, nb,
+                                           partials, refill, face_u, split, rows, blockIdx.x);
+}
+
+// The split 16-bit step in one launch: the per-cell blocks first (those without
+// a row of theirs leave at once; the others, boundary work bound by latency,
+// start early), then the pair blocks over the plain rows.
+// Block of per-cell slot v of a split step (SplitRows), or -1 for a slot past
+// the end of its row.
+__device__ inline int64_t percell_block(uint32_t v, const Grid& g, const SplitRows& r) {
+    const int ny = g.ny, nz = g.nz;
+    const bool zf = !(g.per & 4);
+    const uint32_t bpr = (uint32_t)r.bpr;
+    int y, z;
+    if (v < (uint32_t)r.nface * bpr) {
+        const uint32_t j = v % bpr, i = v / bpr;
+        // z faces: ny rows at z = 0, ny at z = nz - 1; then y faces: y = 0 and
+        // ny - 1 on every other z
+        uint32_t k = i;
+        if (zf && k < 2u * (uint32_t)ny) {
+            y = (int)(k % (uint32_t)ny);
+            z = k < (uint32_t)ny ? 0 : nz - 1;
+        } else {
+            if (zf) k -= 2u * (uint32_t)ny;
+            y = (k & 1u) ? ny - 1 : 0;
+            z = (int)(k >> 1) + (zf ? 1 : 0);
+        }
+        const int64_t c0 = ((int64_t)y + (int64_t)ny * z) * g.nx;
+        const int64_t first = c0 / kBlock, last = (c0 + g.nx - 1) / kBlock;
+        return first + j <= last ? first + j : -1;
+    }
+    int b = 0;
+    while (b + 1 < r.n && v >= (uint32_t)r.base[b + 1]) ++b;
+    const uint32_t l = v - (uint32_t)r.base[b], bx = (uint32_t)r.bpx[b];
+    const uint32_t jb = l % bx, row = l / bx;
+    y = r.y0[b] + (int)(row % (uint32_t)r.ey[b]);
+    z = r.z0[b] + (int)(row / (uint32_t)r.ey[b]);
+    if (y >= ny) y -= ny;
+    if (z >= nz) z -= nz;
+    const int64_t c0 = ((int64_t)y + (int64_t)ny * z) * g.nx + r.x0[b];
+    const int64_t first = c0 / kBlock, last = (c0 + r.ex[b] - 1) / kBlock;
+    return first + jb <= last ? first + jb : -1;
+}
+
+template <class S, int COLL, bool FORCED, bool IBB>
+__global__ __launch_bounds__(kBlock) void k27_split(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                    Grid g, Phys<float> p, const uint8_t* __restrict__ fl_new,
+                                                    const uint8_t* __restrict__ fl_old,
+                                                    const DevBody* __restrict__ bnew, const DevBody* __restrict__ bold,
+                                                    int nb, double* __restrict__ partials, int refill,
+                                                    const double* __restrict__ face_u, SplitRows rows,
+                                                    uint32_t cell_blocks) {
+    if (blockIdx.x < cell_blocks) {
+        const int64_t b = percell_block(blockIdx.x, g, rows);
+        if (b >= 0)
+            step_block<S, COLL, FORCED, MODE_STEP, IBB>(fin, fout, nullptr, nullptr, nullptr, g, p, fl_new, fl_old,
+                                                        bnew, bold, nb, partials, refill, face_u, 1, rows,
+                                                        (uint32_t)b);
+    } else {
+        pair_block<S, COLL, FORCED>(fin, fout, g, p, face_u, rows, blockIdx.x - cell_blocks);
     }
 }
 
@@ -875,10 +1271,28 @@ static void launch_mode(const StepArgs& a) {
     const dim3 grid((unsigned)nblocks_of(a.g.N)), block(kBlock);
     size_t lds = MODE == MODE_STEP ? (size_t)kWaves * a.nb * 6 * sizeof(double) : 0;
     if (IBB) lds = (size_t)kWaves * a.nb * 6 * sizeof(double) + (size_t)(QQ - 1) * kBlock * sizeof(float);
+    if constexpr (S::kDev && MODE == MODE_STEP) {
+        if (a.split) {
+            const unsigned cb = (unsigned)a.rows.base[a.rows.n], pb = (unsigned)nblocks_of(a.g.N / 2);
+#define KS(COLL, FORCED)                                                                                          \
+    hipLaunchKernelGGL((k27_split<S, COLL, FORCED, IBB>), dim3(cb + pb), block, lds, a.stream, (const T*)a.fin,    \
+                       (T*)a.fout, a.g, p, a.fl_new, a.fl_old, a.bnew, a.bold, a.nb, a.partials, a.refill, a.face_u, \
+                       a.rows, cb)
+            if (a.trt) {
+                if (a.forced) KS(LBM_TRT, true);
+                else KS(LBM_TRT, false);
+            } else {
+                if (a.forced) KS(LBM_BGK, true);
+                else KS(LBM_BGK, false);
+            }
+#undef KS
+            return;
+        }
+    }
 #define K27(COLL, FORCED)                                                                                           \
     hipLaunchKernelGGL((k27_step<S, COLL, FORCED, MODE, IBB>), grid, block, lds, a.stream, (const T*)a.fin, (T*)a.fout, \
                        a.gout, a.rho_out, a.u_out, a.g, p, a.fl_new, a.fl_old, a.bnew, a.bold, a.nb, a.partials,   \
-                       a.refill, a.face_u)
+                       a.refill, a.face_u, a.split, a.rows)
     if (a.trt) {
         if (a.forced) K27(LBM_TRT, true);
         else K27(LBM_TRT, false);
