@@ -113,7 +113,7 @@ struct lbm_ctx {
     int64_t step = 0;
     bool initialised = false;
     bool raw_state = false;  // step 0: fbuf[1-cur] holds f_old_0
-    // D3Q27 16-bit storage, even nx >= 4: the split step (k27_pair + k27_step);
+    // D3Q27 16-bit storage, even nx >= 4: the split step (k27_split: pair and per-cell blocks);
     // LBM_Q27_SPLIT=0 in the environment keeps the single per-cell launch
     bool q27_split = false;
 
@@ -864,17 +864,18 @@ q27::SplitRows split_rows(const lbm_ctx* c, const DevBody* bnew, const DevBody* 
             }
         }
     }
-    // enumeration of the per-cell blocks: face rows, then each box's rows
+    // enumeration of the per-cell blocks: each box's rows, then the face rows
     const bool yf = !((c->g.per >> 1) & 1), zf = !((c->g.per >> 2) & 1);
     r.bpr = L[0] % kBlock == 0 ? L[0] / kBlock : L[0] / kBlock + 2;
     r.nface = (zf ? 2 * L[1] : 0) + (yf ? 2 * (zf ? L[2] - 2 : L[2]) : 0);
-    int64_t v = (int64_t)r.nface * r.bpr;
+    int64_t v = 0;
     for (int k = 0; k < r.n; ++k) {
         r.base[k] = (int)v;
         v += (int64_t)r.ey[k] * r.ez[k] * r.bpx[k];
     }
     r.base[r.n] = (int)v;
-    if (v >= ((int64_t)1 << 31)) r.n = -1;
+    r.total = v + (int64_t)r.nface * r.bpr;
+    if (r.total >= ((int64_t)1 << 30)) r.n = -1;
     return r;
 }
 
@@ -1314,8 +1315,10 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     c->cfg.q = q;
     {
         const char* e = std::getenv("LBM_Q27_SPLIT");
+        // (the pair blocks address a plane with 32-bit byte offsets: N * 2 B
+        // plus the plane pad stays below 4 GiB for N < 2^31 - 2^12)
         c->q27_split = q == 27 && (cfg->storage == LBM_FP16S || cfg->storage == LBM_BF16S) && cfg->nx % 2 == 0 &&
-                       cfg->nx >= 4 && !(e && e[0] == '0');
+                       cfg->nx >= 4 && N < ((int64_t)1 << 31) - 4096 && !(e && e[0] == '0');
     }
     c->q = q;
     c->g.nx = cfg->nx;

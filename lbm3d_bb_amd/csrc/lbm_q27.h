@@ -22,10 +22,10 @@ constexpr int kZ[kQ] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 
 // cell (x, y, z) lies in box k when x - x0[k] < ex[k], (y - y0[k]) mod ny < ey[k]
 // and (z - z0[k]) mod nz < ez[k].
 // The per-cell blocks of a split step are enumerated from these values alone:
-// first the rows on the non-periodic y/z faces (nface rows, bpr block slots
-// per row: the blocks a row of nx cells can span), then the rows of each box
-// (bpx[k] slots per row: the blocks its x range can span); base[k] is the
-// first slot of box k, base[n] the total.  A block reached
+// first the rows of each box (bpx[k] slots per row: the blocks its x range
+// can span; base[k] is the first slot of box k, base[n] the boxes' total),
+// then the rows on the non-periodic y/z faces (nface rows, bpr block slots per
+// row: the blocks a row of nx cells can span); total counts all slots.  A block reached
 // twice (rows that share a block, overlapping boxes) computes the same values
 // twice, which is harmless.
 constexpr int kSplitBoxes = 32;
@@ -36,6 +36,7 @@ struct SplitRows {
     int bpr;
     int nface;
     int base[kSplitBoxes + 1];
+    int64_t total;
 };
 
 // One launch of k27_step: a time step (MODE_STEP), the getters' replay of the

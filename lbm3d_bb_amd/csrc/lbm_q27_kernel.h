@@ -38,6 +38,15 @@ __device__ inline C w27(int d) {
     return n == 0 ? C(8) / C(27) : (n == 1 ? C(2) / C(27) : (n == 2 ? C(1) / C(54) : C(1) / C(216)));
 }
 
+// The random word of direction d for the 16-bit D3Q27 stores, from the cell's
+// hash h (sr_cell): a 24-bit multiply by a per-direction odd constant (full
+// rate, where a 32-bit multiply runs at a quarter) and a shift-xor that brings
+// product bits 15-31 into the low bits the rounding takes.
+__device__ __forceinline__ uint32_t sr_dir27(uint32_t h, int d) {
+    const uint32_t x = __umul24(h, ((0x9e3779u * (2u * (uint32_t)d + 1u)) & 0xffffffu) | 1u);
+    return x ^ (x >> 15);
+}
+
 // 16-bit population storage on D3Q27: the deviation from the rest state
 // f_d - w_d (rho0 = 1) with this lattice's weights, times 2^12 in FP16 or as is
 // in BF16; the step kernels hold and collide the deviations (kDev), loads
@@ -62,9 +71,17 @@ struct StoreF16S27 {
     __device__ static inline C dec(uint32_t h, const Shift& sh) {
         return __half2float(__ushort_as_half((unsigned short)(h & 0xffffu))) * sh.ik;
     }
-    // two cells in one word (low: a), each rounded as store_sr rounds it
-    __device__ static inline uint32_t sr_pair(C a, C b, const Shift& sh, uint32_t ra, uint32_t rb) {
-        return f16x2_sr(a * sh.k, b * sh.k, (ra & 0x1fffu) | ((rb & 0x1fffu) << 16));
+    // a two-cell word in two halves: prep gives each value's rounding input
+    // (the scaled deviation with its random bits added, f16x2_sr), pack
+    // converts two of them (low: a); the halves equal what store_sr writes
+    __device__ static inline uint32_t prep(C dv, const Shift& sh, uint32_t r) {
+        return __float_as_uint(dv * sh.k) + (r & 0x1fffu);
+    }
+    __device__ static inline uint32_t pack(uint32_t a, uint32_t b) {
+        const auto h = __builtin_amdgcn_cvt_pkrtz(__uint_as_float(a), __uint_as_float(b));
+        uint32_t out;
+        __builtin_memcpy(&out, &h, 4);
+        return out;
     }
 };
 
@@ -81,9 +98,8 @@ struct StoreBF16S27 {
     }
     __device__ static inline uint32_t sr_bits(C dv, const Shift&, uint32_t r) { return bf16_sr(dv, r) & 0xffffu; }
     __device__ static inline C dec(uint32_t h, const Shift&) { return __uint_as_float(h << 16); }
-    __device__ static inline uint32_t sr_pair(C a, C b, const Shift&, uint32_t ra, uint32_t rb) {
-        return (bf16_sr(a, ra) & 0xffffu) | (bf16_sr(b, rb) << 16);
-    }
+    __device__ static inline uint32_t prep(C dv, const Shift&, uint32_t r) { return bf16_sr(dv, r) & 0xffffu; }
+    __device__ static inline uint32_t pack(uint32_t a, uint32_t b) { return a | (b << 16); }
 };
 
 // Sums of the FP32 D3Q27 weights (rho_w = sum w_d, s2 = sum w_d c_dx^2; 1 and
@@ -236,24 +252,28 @@ __device__ inline void collide27_dev(float f[QQ], const Phys<float>& p) {
     if (COLL == LBM_TRT) {
         const float wm = p.wm;
         const float km = 1.0f - 0.5f * wm;
+        // The pair (a, b) = (f_d, f_opp) relaxes its symmetric part with wp and
+        // its antisymmetric part with wm:
+        //   f_d = A1 a + B1 b + (P + M),  f_opp = B1 a + A1 b + (P - M),
+        //   A1 = 1 - (wp + wm) / 2,  B1 = (wm - wp) / 2,
+        //   P = wp dp (+ kp Sp),  M = wm dm (+ km Sm),
+        // the same update as ((a+b)/2 relaxed to dp) +- ((a-b)/2 relaxed to dm).
+        const float A1 = 1.0f - 0.5f * (wp + wm), B1 = 0.5f * (wm - wp);
 #pragma unroll
         for (int d = 0; d < QQ - 1; d += 2) {
             const float w = w27<float>(d);
             const float cu = float(kX[d]) * ux + float(kY[d]) * uy + float(kZ[d]) * uz;
             const float cm = float(kX[d]) * mx + float(kY[d]) * my + float(kZ[d]) * mz;
-            const float dp = w * (rm + rh * (a2 * cu * cu - 1.5f * u2));
-            const float dm = w * (c3 * cm);
-            const float fp = 0.5f * (f[d] + f[d + 1]);
-            const float fm = 0.5f * (f[d] - f[d + 1]);
-            float fpn = fp - wp * (fp - dp);
-            float fmn = fm - wm * (fm - dm);
+            float P = (wp * w) * (rm + rh * (a2 * cu * cu - 1.5f * u2));
+            float M = (wm * w * c3) * cm;
             if (FORCED) {
                 const float cF = float(kX[d]) * Fx + float(kY[d]) * Fy + float(kZ[d]) * Fz;
-                fpn += kp * (w * (a9 * cu * cF - 3.0f * uF));
-                fmn += km * (w * (c3 * cF));
+                P += (kp * w) * (a9 * cu * cF - 3.0f * uF);
+                M += (km * w * c3) * cF;
             }
-            f[d] = fpn + fmn;
-            f[d + 1] = fpn - fmn;
+            const float a = f[d], b = f[d + 1];
+            f[d] = fmaf(A1, a, fmaf(B1, b, P + M));
+            f[d + 1] = fmaf(B1, a, fmaf(A1, b, P - M));
         }
         const float w0 = w27<float>(QQ - 1);
         const float d0 = w0 * (rm - rh * (1.5f * u2));
@@ -525,19 +545,18 @@ __device__ inline double wrap_arm(double r, int a, const Grid& g) {
 }
 
 // ---------------------------------------------------------------------------
-// The split 16-bit step: two launches per step.  k27_pair takes every cell of a
-// row off the non-periodic y/z faces, in a block no body comes near
-// (plain_pair), two x-neighbours per thread with 4-byte pair loads and stores:
-// a one-cell-per-thread 16-bit step issues one 2-byte access per lane and
-// direction, as many instructions as FP32 for half the bytes.  Its x-face
-// cells are part of their rows' waves: periodic x wraps with one 2-byte load,
-// a zero-gradient face clamps (a value select), a velocity / anti-bounce-back
-// face applies its rule from the cell's own f*[opp d] (one extra load per
-// direction pair) and the previous-step moments of the cell in its face slot
-// (one 16-byte load), which whichever kernel produced the cell's f* wrote
-// (q27_slot_write).  k27_step (split = 1) takes the rest: cells on the y/z
-// rule rows and every cell of a block within reach of a body, the same
-// partition by the same tests.
+// The split 16-bit step (k27_split, one launch).  A one-cell-per-thread
+// 16-bit step issues one 2-byte access per lane and direction: as many memory
+// instructions as FP32 for half the bytes.  Pair blocks take two x-neighbours
+// per thread with 4-byte loads and stores for every cell off the rows on
+// non-periodic y/z faces and outside the bodies' boxes (percell_cell); the
+// x-face cells stay in their rows' waves: periodic x wraps with one 2-byte
+// load, a zero-gradient face clamps (a value select), a velocity /
+// anti-bounce-back face applies its rule from the cell's own f*[opp d] and
+// the previous-step moments in the cell's face slot, which whichever blocks
+// produced the cell's f* wrote (q27_slot_put).  Per-cell blocks (step_block,
+// split = 1) take the face rows and the boxes, enumerated from kernel
+// arguments (percell_block) so that no block is launched only to leave.
 // ---------------------------------------------------------------------------
 
 // A row on a non-periodic y or z face (the per-cell kernel's).
@@ -964,9 +983,9 @@ __device__ __forceinline__ void step_block(const typename S::T* __restrict__ fin
             else collide27<C, COLL, FORCED>(f, p);
             const uint32_t h = sr_cell(cell, g.sh.seed);
 #pragma unroll
-            for (int d = 0; d < QQ; ++d) S::store_sr(fout + (int64_t)d * g.P + cell, d, f[d], g.sh, sr_dir(h, d));
+            for (int d = 0; d < QQ; ++d) S::store_sr(fout + (int64_t)d * g.P + cell, d, f[d], g.sh, sr_dir27(h, d));
             if constexpr (S::kDev) {
-                // split step: the face slot of an x-face rule cell, for k27_pair's next step
+                // split step: the face slot of an x-face rule cell, for the pair blocks' next step
                 if (split) {
                     const uint32_t xc = (uint32_t)(cell % g.nx);
                     const RuleFaces rf = rule_faces_of(g);
@@ -978,7 +997,7 @@ __device__ __forceinline__ void step_block(const typename S::T* __restrict__ fin
                         C sr = C(0), sx = C(0), sy = C(0), sz = C(0);
 #pragma unroll
                         for (int d = 0; d < QQ; ++d)
-                            moment_add<C>(d, S::dec(S::sr_bits(f[d], g.sh, sr_dir(h, d)), g.sh), sr, sx, sy, sz);
+                            moment_add<C>(d, S::dec(S::sr_bits(f[d], g.sh, sr_dir27(h, d)), g.sh), sr, sx, sy, sz);
                         q27_slot_put<S, FORCED>(g, sr, sx, sy, sz, p, x_slot(g, fx, y, z));
                     }
                 }
@@ -1025,11 +1044,20 @@ __device__ inline uint32_t load_half(const void* p) { return *reinterpret_cast<c
 // x treatment of a wave of pairs
 enum { PX_NONE = 0, PX_WRAP = 1, PX_CLAMP = 2, PX_RULE = 3 };
 
+// Element e of a population plane: the plane's base is uniform (scalar
+// registers) and the lane's offset a 32-bit byte count (a plane of 16-bit
+// populations is below 4 GiB, checked at context creation), so each access
+// takes one 32-bit add instead of 64-bit address arithmetic.
+__device__ __forceinline__ const char* plane_of(const void* f, int d, const Grid& g) {
+    return reinterpret_cast<const char*>(f) + (size_t)d * (size_t)g.P * 2u;
+}
+__device__ __forceinline__ uint32_t load_at(const char* plane, uint32_t e) { return load_word(plane + 2u * e); }
+__device__ __forceinline__ uint32_t load_half_at(const char* plane, uint32_t e) { return load_half(plane + 2u * e); }
+
 template <class S, int COLL, bool FORCED, int XM>
 __device__ __forceinline__ void pair_cells(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                                            const Grid& g, const Phys<float>& p, const double* __restrict__ face_u,
                                            int x0, int y, int z) {
-    using T = typename S::T;
     const int nx = g.nx, nxy = (int)g.nxy;
     const int row = nx * y + nxy * z;
     const bool left = x0 == 0, right = x0 + 2 == nx;
@@ -1062,34 +1090,35 @@ __device__ __forceinline__ void pair_cells(const typename S::T* __restrict__ fin
         if (kY[d] < 0) srow += wyp;
         if (kZ[d] > 0) srow += wzm;
         if (kZ[d] < 0) srow += wzp;
-        const T* plane = fin + (int64_t)d * g.P + srow;
+        const char* pl = plane_of(fin, d, g);
+        const uint32_t e = (uint32_t)(srow + x0);  // the pair's own x in the source row
         if (kX[d] == 0) {
-            w[d] = load_word(plane + x0);
+            w[d] = load_at(pl, e);
         } else if (kX[d] > 0) {
             // sources (x0 - 1, x0); the left pair loads (0, 1) so that no load
             // leaves the row: x = -1 wraps, clamps onto x = 0 or takes the rule
-            const uint32_t v = load_word((XM != PX_NONE && left) ? plane : plane + x0 - 1);
+            const uint32_t v = load_at(pl, (XM != PX_NONE && left) ? e : e - 1u);
             if (XM == PX_NONE) {
                 w[d] = v;
             } else {
-                const uint32_t lo = XM == PX_WRAP ? load_half(left ? plane + nx - 1 : plane + x0 - 1) : (v & 0xffffu);
+                const uint32_t lo = XM == PX_WRAP ? load_half_at(pl, left ? e + (uint32_t)nx - 1u : e - 1u) : (v & 0xffffu);
                 w[d] = left ? (lo | (v << 16)) : ((v & 0xffff0000u) | lo);
             }
         } else {
             // sources (x0 + 1, x0 + 2); the right pair loads (nx - 2, nx - 1)
-            const uint32_t v = load_word((XM != PX_NONE && right) ? plane + x0 : plane + x0 + 1);
+            const uint32_t v = load_at(pl, (XM != PX_NONE && right) ? e : e + 1u);
             if (XM == PX_NONE) {
                 w[d] = v;
             } else {
-                const uint32_t hi = XM == PX_WRAP ? load_half(right ? plane : plane + x0 + 2) : (v >> 16);
+                const uint32_t hi = XM == PX_WRAP ? load_half_at(pl, right ? e + 1u - (uint32_t)nx + 1u : e + 2u) : (v >> 16);
                 w[d] = right ? ((v >> 16) | (hi << 16)) : ((v & 0xffffu) | (hi << 16));
             }
         }
     }
-    const int64_t c0 = (int64_t)row + x0;
+    const uint32_t c0 = (uint32_t)(row + x0);
     float f[QQ];
-    // cell x0: decode, face rule (x = 0), collide; its rounded bits replace the
-    // low halves
+    uint32_t pa[QQ];  // cell x0's rounding inputs (S::prep), converted with cell x0 + 1's
+    // cell x0: decode, face rule (x = 0), collide, rounding inputs
     {
 #pragma unroll
         for (int d = 0; d < QQ; ++d) f[d] = S::dec(w[d], g.sh);
@@ -1101,22 +1130,17 @@ __device__ __forceinline__ void pair_cells(const typename S::T* __restrict__ fin
 #pragma unroll
             for (int d = 0; d < QQ; ++d) {
                 if (kX[d] <= 0) continue;
-                const float o = S::dec(load_half(fin + (int64_t)opp(d) * g.P + row), g.sh);
+                const float o = S::dec(load_half_at(plane_of(fin, opp(d), g), (uint32_t)row), g.sh);
                 f[d] = vel ? velocity_rule<S>(d, o, uf, rho_prev) : abb_rule<S>(d, o, up, uu2, rho_prev);
             }
         }
         collide27_dev<COLL, FORCED>(f, p);
         const uint32_t h = sr_cell(c0, g.sh.seed);
-        float sr = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;  // the cell as stored: its face slot
 #pragma unroll
-        for (int d = 0; d < QQ; ++d) {
-            const uint32_t b = S::sr_bits(f[d], g.sh, sr_dir(h, d));
-            w[d] = (w[d] & 0xffff0000u) | b;
-            if (XM == PX_RULE) moment_add<float>(d, S::dec(b, g.sh), sr, sx, sy, sz);
-        }
-        if (XM == PX_RULE && rl) q27_slot_put<S, FORCED>(g, sr, sx, sy, sz, p, x_slot(g, 0, y, z));
+        for (int d = 0; d < QQ; ++d) pa[d] = S::prep(f[d], g.sh, sr_dir27(h, d));
     }
-    // cell x0 + 1: the same with the high halves (face rule at x = nx - 1)
+    // cell x0 + 1: the same with the high halves (face rule at x = nx - 1), then
+    // both cells' words
     {
 #pragma unroll
         for (int d = 0; d < QQ; ++d) f[d] = S::dec(w[d] >> 16, g.sh);
@@ -1127,21 +1151,24 @@ __device__ __forceinline__ void pair_cells(const typename S::T* __restrict__ fin
 #pragma unroll
             for (int d = 0; d < QQ; ++d) {
                 if (kX[d] >= 0) continue;
-                const float o = S::dec(load_half(fin + (int64_t)opp(d) * g.P + row + nx - 1), g.sh);
+                const float o = S::dec(load_half_at(plane_of(fin, opp(d), g), (uint32_t)(row + nx - 1)), g.sh);
                 f[d] = vel ? velocity_rule<S>(d, o, uf, rho_prev) : abb_rule<S>(d, o, up, uu2, rho_prev);
             }
         }
         collide27_dev<COLL, FORCED>(f, p);
-        const uint32_t h = sr_cell(c0 + 1, g.sh.seed);
-        T* o = fout + c0;
-        float sr = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+        const uint32_t h = sr_cell(c0 + 1u, g.sh.seed);
 #pragma unroll
         for (int d = 0; d < QQ; ++d) {
-            const uint32_t b = S::sr_bits(f[d], g.sh, sr_dir(h, d));
-            __builtin_nontemporal_store((w[d] & 0xffffu) | (b << 16), reinterpret_cast<uint32_t*>(o + (int64_t)d * g.P));
-            if (XM == PX_RULE) moment_add<float>(d, S::dec(b, g.sh), sr, sx, sy, sz);
+            w[d] = S::pack(pa[d], S::prep(f[d], g.sh, sr_dir27(h, d)));
+            __builtin_nontemporal_store(w[d], reinterpret_cast<uint32_t*>(const_cast<char*>(plane_of(fout, d, g)) + 2u * c0));
         }
-        if (XM == PX_RULE && rr) q27_slot_put<S, FORCED>(g, sr, sx, sy, sz, p, x_slot(g, 1, y, z));
+    }
+    if (XM == PX_RULE && (rl || rr)) {
+        // the rule cell as stored: its face slot
+        float sr = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+        for (int d = 0; d < QQ; ++d) moment_add<float>(d, S::dec(rl ? w[d] : (w[d] >> 16), g.sh), sr, sx, sy, sz);
+        q27_slot_put<S, FORCED>(g, sr, sx, sy, sz, p, x_slot(g, rl ? 0 : 1, y, z));
     }
 }
 
@@ -1178,21 +1205,23 @@ __global__ __launch_bounds__(kBlock) void k27_step(const typename S::T* __restri
                                            partials, refill, face_u, split, rows, blockIdx.x);
 }
 
-// The split 16-bit step in one launch: the per-cell blocks first (those without
-// a row of theirs leave at once; the others, boundary work bound by latency,
-// start early), then the pair blocks over the plain rows.
+// The split 16-bit step in one launch: the per-cell blocks first (boundary
+// work, bound by latency: it starts early), then the pair blocks over the
+// plain cells.
 // Block of per-cell slot v of a split step (SplitRows), or -1 for a slot past
-// the end of its row.
+// the end of its row or range: the boxes' slots first (their blocks carry the
+// bodies, the longest work), then the face rows'.
 __device__ inline int64_t percell_block(uint32_t v, const Grid& g, const SplitRows& r) {
     const int ny = g.ny, nz = g.nz;
-    const bool zf = !(g.per & 4);
-    const uint32_t bpr = (uint32_t)r.bpr;
+    const uint32_t nbox = (uint32_t)r.base[r.n];
     int y, z;
-    if (v < (uint32_t)r.nface * bpr) {
-        const uint32_t j = v % bpr, i = v / bpr;
+    if (v >= nbox) {
+        const bool zf = !(g.per & 4);
+        const uint32_t bpr = (uint32_t)r.bpr;
+        const uint32_t j = (v - nbox) % bpr, k0 = (v - nbox) / bpr;
         // z faces: ny rows at z = 0, ny at z = nz - 1; then y faces: y = 0 and
         // ny - 1 on every other z
-        uint32_t k = i;
+        uint32_t k = k0;
         if (zf && k < 2u * (uint32_t)ny) {
             y = (int)(k % (uint32_t)ny);
             z = k < (uint32_t)ny ? 0 : nz - 1;
@@ -1226,14 +1255,17 @@ __global__ __launch_bounds__(kBlock) void k27_split(const typename S::T* __restr
                                                     int nb, double* __restrict__ partials, int refill,
                                                     const double* __restrict__ face_u, SplitRows rows,
                                                     uint32_t cell_blocks) {
-    if (blockIdx.x < cell_blocks) {
-        const int64_t b = percell_block(blockIdx.x, g, rows);
+    const uint32_t i = blockIdx.x;
+    const bool cellb = i < cell_blocks;
+    const uint32_t vi = cellb ? i : i - cell_blocks;
+    if (cellb) {
+        const int64_t b = percell_block(vi, g, rows);
         if (b >= 0)
             step_block<S, COLL, FORCED, MODE_STEP, IBB>(fin, fout, nullptr, nullptr, nullptr, g, p, fl_new, fl_old,
                                                         bnew, bold, nb, partials, refill, face_u, 1, rows,
                                                         (uint32_t)b);
     } else {
-        pair_block<S, COLL, FORCED>(fin, fout, g, p, face_u, rows, blockIdx.x - cell_blocks);
+        pair_block<S, COLL, FORCED>(fin, fout, g, p, face_u, rows, vi);
     }
 }
 
@@ -1273,7 +1305,7 @@ static void launch_mode(const StepArgs& a) {
     if (IBB) lds = (size_t)kWaves * a.nb * 6 * sizeof(double) + (size_t)(QQ - 1) * kBlock * sizeof(float);
     if constexpr (S::kDev && MODE == MODE_STEP) {
         if (a.split) {
-            const unsigned cb = (unsigned)a.rows.base[a.rows.n], pb = (unsigned)nblocks_of(a.g.N / 2);
+            const unsigned cb = (unsigned)a.rows.total, pb = (unsigned)nblocks_of(a.g.N / 2);
 #define KS(COLL, FORCED)                                                                                          \
     hipLaunchKernelGGL((k27_split<S, COLL, FORCED, IBB>), dim3(cb + pb), block, lds, a.stream, (const T*)a.fin,    \
                        (T*)a.fout, a.g, p, a.fl_new, a.fl_old, a.bnew, a.bold, a.nb, a.partials, a.refill, a.face_u, \

@@ -1,6 +1,6 @@
-"""The split 16-bit D3Q27 step (k27_pair over the plain rows, k27_step over the
-rule rows and the blocks near bodies) against the single per-cell launch
-(LBM_Q27_SPLIT=0): the same populations and forces, bit for bit, on every x
+"""The split 16-bit D3Q27 step (k27_split: pair blocks over the plain cells,
+per-cell blocks over the y/z face rows and the bodies' boxes) against the single
+per-cell launch (LBM_Q27_SPLIT=0): the same populations and forces, bit for bit, on every x
 treatment of the pair kernel (velocity inlet, anti-bounce-back outlet,
 zero-gradient faces, periodic wrap), with static and moving bodies, simple and
 interpolated bounce-back, and rows that do not fill a wave (nx = 40, 64).
