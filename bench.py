@@ -21,7 +21,10 @@ profiles/traffic_<workload>.json holds it.  ``cpu_baseline`` times the oracle (a
 plain C restatement, OpenMP) on the same workload for a bounded number of
 steps.  ``config.drag_err_vs_fp64`` compares each body's drag, averaged over
 the last 100 steps of the run, with an untimed FP64-storage run of the same
-workload, start and step count.
+workload, start and step count.  ``companion`` (default workload only) times
+the same case in shifted-FP16 storage right after, with the same warm-up and
+steps, and reports its MLUPS, its ratio to the FP32 line, its roofline and its
+drag error against the same FP64 run.
 
 --copy-probe MiB runs, after the timed region, ten device-to-device copies of
 that many MiB (a plain copy kernel of known size, the calibration of the
@@ -183,38 +186,16 @@ def cpu_baseline(w, budget_s: float) -> dict:
                       f"{prec} oracle, {el:.1f} s"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--workload", default="channel256", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for cpu_baseline")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-profile", action="store_true", help="skip the per-launch HIP events (no roofline)")
-    ap.add_argument("--sample-stride", type=int, default=4,
-                    help="time every n-th step launch with HIP events (an event pair costs ~3 us of the step)")
-    ap.add_argument("--start", default="uniform", choices=["uniform", "net"],
-                    help="net: start from the 128^3 U-Net applied tile-wise (untimed, before the warm-up)")
-    ap.add_argument("--params", default=None, help="U-Net parameters (.npy) for --start net; random init if absent")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
-                    help="after the run, ten device-to-device copies of this many MiB (counter calibration)")
-    ap.add_argument("--no-fp64-ref", action="store_true",
-                    help="skip the FP64 run of the same steps that drag_err_vs_fp64 compares against")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): there is no multi-GPU path; the path "
-              "would shard as x slabs, one per GPU, exchanging the five x-moving populations of each face plane "
-              "every step.", file=sys.stderr)
-        return 2
+# The 16-bit storage of the default workload, timed in the same run and
+# reported beside it (the FP16S throughput claim and its drag error).
+COMPANIONS = {"channel256": "channel256_fp16s"}
 
-    if args.copy_probe > 0:
-        import torch  # initialised before the library so the probe's copies share its HIP runtime
 
-        torch.cuda.init()
+def timed_run(w, args) -> dict:
+    """Set-up, warm-up and the timed region of one workload; the per-launch
+    HIP-event samples of k_step come back in `prof`."""
     from lbm3d_bb_amd.solver import Solver
 
-    w = WORKLOADS[args.workload]
     cfg, laws, bodies, desc = make_case(w)
     s = Solver(cfg)
     setup(s, laws, bodies)
@@ -233,18 +214,98 @@ def main() -> int:
     info = s.info()
     rho, u = s.macros()
     s.close()
+    return {"case": (cfg, laws, bodies, desc), "forces": forces, "warm_forces": warm_forces, "seconds": el,
+            "prof": prof, "info": info, "start": start, "rho": rho, "u": u}
+
+
+def roofline(w, name, ncells, prof) -> tuple[float, dict]:
+    """Mean k_step launch time (s) and the roofline object: k_step is the whole
+    time step (bulk rows, faces and body regions) in one launch."""
+    bpc = bytes_per_cell(w["storage"], w["q"])
+    step_avg_s = prof["bulk_ms"] / max(prof["bulk_launches"], 1) / 1e3
+    achieved = bpc * ncells / step_avg_s / 1e9 if step_avg_s > 0 else None
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", f"traffic_{name}.json")
+    if os.path.exists(tfile):
+        with open(tfile, encoding="utf-8") as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    return step_avg_s, {
+        "bound": "hbm",
+        "achieved": round(achieved, 1) if achieved else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+        "traffic": traffic,
+    }
+
+
+def companion(name, args, primary_mlups, F64) -> dict:
+    """Another storage of the same case, same start, warm-up and steps: its
+    MLUPS, ratio to the primary line's, roofline and drag error against the
+    primary's FP64 run (the case and start are the same, so that run serves)."""
+    w = WORKLOADS[name]
+    run = timed_run(w, args)
+    cfg = run["case"][0]
+    mlups = cfg.ncells * args.steps / run["seconds"] / 1e6
+    step_avg_s, roof = roofline(w, name, cfg.ncells, run["prof"])
+    out = {"workload": name, "value": round(mlups, 1), "unit": "MLUPS", "dtype": w["dtype"],
+           "ms_per_step": round(run["seconds"] / args.steps * 1e3, 5), "ratio_to_primary": round(mlups /
+           primary_mlups, 3), "step_kernel_ms": round(step_avg_s * 1e3, 5),
+           "step_kernel_samples": run["prof"]["bulk_launches"], "roofline": roof}
+    if w["storage"] in ("fp16s", "bf16s"):
+        # 1.7 x the MLUPS of FP32 storage (2 x 19 x 4 bytes per cell) at 60 % of the HBM roofline
+        target = 1.7 * 0.6 * HBM_PEAK_GBS * 1e9 / bytes_per_cell("fp32", w["q"]) / 1e6
+        out["target"] = {"value": round(target, 1), "met": bool(mlups >= target),
+                         "rule": "1.7 x the FP32-storage MLUPS at 60 % of the 8 TB/s roofline"}
+    if F64 is not None:
+        F = run["forces"] if run["warm_forces"] is None else np.concatenate([run["warm_forces"], run["forces"]])
+        out["drag_err_vs_fp64"] = drag_err(F, F64)
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--workload", default="channel256", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for cpu_baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-launch HIP events (no roofline)")
+    ap.add_argument("--sample-stride", type=int, default=4,
+                    help="time every n-th step launch with HIP events (an event pair costs ~3 us of the step)")
+    ap.add_argument("--start", default="uniform", choices=["uniform", "net"],
+                    help="net: start from the 128^3 U-Net applied tile-wise (untimed, before the warm-up)")
+    ap.add_argument("--params", default=None, help="U-Net parameters (.npy) for --start net; random init if absent")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
+                    help="after the run, ten device-to-device copies of this many MiB (counter calibration)")
+    ap.add_argument("--companion", default="auto",
+                    help="another workload timed after the primary and reported beside it ('auto': the 16-bit "
+                         "storage of the default workload; 'none' for no companion)")
+    ap.add_argument("--no-fp64-ref", action="store_true",
+                    help="skip the FP64 run of the same steps that drag_err_vs_fp64 compares against")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): there is no multi-GPU path; the path "
+              "would shard as x slabs, one per GPU, exchanging the five x-moving populations of each face plane "
+              "every step.", file=sys.stderr)
+        return 2
+
+    if args.copy_probe > 0:
+        import torch  # initialised before the library so the probe's copies share its HIP runtime
+
+        torch.cuda.init()
+    w = WORKLOADS[args.workload]
+    run = timed_run(w, args)
+    cfg, laws, bodies, desc = run["case"]
+    forces, warm_forces, el, prof, info, start = (run[k] for k in ("forces", "warm_forces", "seconds", "prof",
+                                                                   "info", "start"))
+    rho, u = run["rho"], run["u"]
 
     N = cfg.ncells
     mlups = N * args.steps / el / 1e6
     bpc = bytes_per_cell(w["storage"], w["q"])
-    # k_step is the whole time step (bulk rows, faces, body regions) in one launch
-    step_avg_s = prof["bulk_ms"] / max(prof["bulk_launches"], 1) / 1e3
-    achieved = bpc * N / step_avg_s / 1e9 if step_avg_s > 0 else None
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tfile):
-        with open(tfile, encoding="utf-8") as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    step_avg_s, roof = roofline(w, args.workload, N, prof)
     out = {
         "metric": "MLUPS",
         "value": round(mlups, 1),
@@ -273,38 +334,33 @@ def main() -> int:
             "rho_min": float(rho.min()),
             "u_max": float(np.abs(u).max()),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1) if achieved else None,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": traffic,
-        },
+        "roofline": roof,
     }
     if args.start == "net":
         out["start"] = start
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+    F64 = None
     if not args.no_fp64_ref and forces is not None:
         F = forces if warm_forces is None else np.concatenate([warm_forces, forces])
-        out["config"]["drag_err_vs_fp64"] = drag_err_vs_fp64(w, F, args, cfg, laws, bodies)
+        F64 = fp64_forces(w, F.shape[0], args, laws, bodies)
+        out["config"]["drag_err_vs_fp64"] = drag_err(F, F64)
+    comp = COMPANIONS.get(args.workload) if args.companion == "auto" else (args.companion or None)
+    if comp and comp != "none":
+        out["companion"] = companion(comp, args, mlups, F64)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)
     return 0
 
 
-def drag_err_vs_fp64(w, F, args, cfg, laws, bodies) -> dict:
-    """The same workload, start and steps in FP64 storage (untimed, after the
-    timed run): relative error of each body's drag (x-force) averaged over the
-    last min(100, steps) steps, against the FP64 run's."""
+def fp64_forces(w, n, args, laws, bodies) -> np.ndarray | None:
+    """Per-step forces of the same workload, start and step count in FP64
+    storage (untimed, after the timed run); None when the run is FP64 itself."""
     from lbm3d_bb_amd.solver import Solver
 
-    n = F.shape[0]
-    win = min(100, n)
     if w["storage"] == "fp64":
-        return {"max": 0.0, "total": 0.0, "steps": n, "window": win, "note": "this run is FP64"}
+        return None
     cfg64 = make_case(dict(w, storage="fp64"))[0]
     r = Solver(cfg64)
     setup(r, laws, bodies)
@@ -312,6 +368,16 @@ def drag_err_vs_fp64(w, F, args, cfg, laws, bodies) -> dict:
         net_start(r, cfg64, laws, bodies, args.params)
     F64 = r.step(n)
     r.close()
+    return F64
+
+
+def drag_err(F, F64) -> dict:
+    """Relative error of each body's drag (x-force) averaged over the last
+    min(100, steps) steps, against the FP64 run's."""
+    n = F.shape[0]
+    win = min(100, n)
+    if F64 is None:
+        return {"max": 0.0, "total": 0.0, "steps": n, "window": win, "note": "this run is FP64"}
     d = F[-win:, :, 0].mean(axis=0)
     d64 = F64[-win:, :, 0].mean(axis=0)
     e = np.abs(d - d64) / np.abs(d64)

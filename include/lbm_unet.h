@@ -28,7 +28,7 @@ extern "C" {
 
 typedef struct {
     int32_t nx, ny, nz;  /* spatial size; each a multiple of 2^(levels-1) */
-    int32_t batch;       /* largest batch; batch * nx * ny * nz < 2^31 */
+    int32_t batch;       /* largest batch; batch * nx * ny * nz < 2^31 and that times base / 8 < 2^32 */
     int32_t cin;         /* input channels (<= 8), e.g. 2: solid mask, Re */
     int32_t cout;        /* output channels (<= 16), e.g. 4: rho, ux, uy, uz */
     int32_t base;        /* channels at level 0 (multiple of 16), e.g. 16 */

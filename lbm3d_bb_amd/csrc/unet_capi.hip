@@ -309,9 +309,9 @@ int forward(lbm_unet* n, int B) {
         if (l < Lv - 1) {
             rc = layer_fwd(n, n->enc2[l], n->E1[l], 0, &n->Cat[l], n->C[l], nullptr, 1, d[l], d[l], d[l]);
             if (rc) return rc;
-            const int64_t tot = (int64_t)B * n->nvox[l + 1] * n->C[l];
+            const int64_t tot = (int64_t)B * n->nvox[l + 1] * (n->C[l] / 8);
             hipLaunchKernelGGL(k_avgpool, dim3(nblk(tot)), dim3(kThreads), 0, n->stream, n->Cat[l].p, n->Cat[l].cs,
-                               n->C[l], n->P[l + 1].p, n->P[l + 1].cs, 0, n->C[l], d[l + 1], d[l], tot);
+                               n->C[l], n->P[l + 1].p, n->P[l + 1].cs, 0, d[l + 1], d[l], n->C[l] / 8, (uint32_t)tot);
         } else {
             rc = layer_fwd(n, n->enc2[l], n->E1[l], 0, &n->D[l], 0, nullptr, 1, d[l], d[l], d[l]);
             if (rc) return rc;
@@ -387,10 +387,10 @@ int backward(lbm_unet* n, int B) {
         if (l > 0) {
             rc = layer_dgrad(n, n->enc1[l], n->dE1[l], 0, n->dP[l], 0, nullptr, 0, d[l], d[l]);
             if (rc) return rc;
-            const int64_t tot = (int64_t)B * n->nvox[l - 1] * n->C[l - 1];
-            hipLaunchKernelGGL(k_avgpool_bwd, dim3(nblk(tot)), dim3(kThreads), 0, n->stream, n->dP[l].p, n->dP[l].cs, 0,
-                               n->dCat[l - 1].p, n->dCat[l - 1].cs, n->C[l - 1], n->Cat[l - 1].p, n->Cat[l - 1].cs,
-                               n->C[l - 1], n->C[l - 1], d[l - 1], d[l], tot);
+            const int64_t tot = (int64_t)B * n->nvox[l - 1] * (n->C[l - 1] / 8);
+            hipLaunchKernelGGL(k_avgpool_bwd, dim3(nblk(tot)), dim3(kThreads), 0, n->stream, n->dP[l].p,
+                               n->dP[l].cs, 0, n->dCat[l - 1].p, n->dCat[l - 1].cs, n->C[l - 1], n->Cat[l - 1].p,
+                               n->Cat[l - 1].cs, n->C[l - 1], d[l - 1], d[l], n->C[l - 1] / 8, (uint32_t)tot);
             UCHK(n, hipGetLastError());
         }
     }
@@ -400,9 +400,8 @@ int backward(lbm_unet* n, int B) {
 int upload_x(lbm_unet* n, const float* x, int B) {
     const int64_t N0 = n->nvox[0];
     UCHK(n, hipMemcpyAsync(n->d_in, x, (size_t)B * n->cfg.cin * N0 * sizeof(float), hipMemcpyHostToDevice, n->stream));
-    const int64_t tot = (int64_t)B * N0 * 8;
-    hipLaunchKernelGGL(k_pack_input, dim3(nblk(tot)), dim3(kThreads), 0, n->stream, n->d_in, n->cfg.cin, 8, N0,
-                       (int64_t)B * N0, n->X0.p);
+    hipLaunchKernelGGL(k_pack_input, dim3(nblk((int64_t)B * N0)), dim3(kThreads), 0, n->stream, n->d_in, n->cfg.cin,
+                       8, (int)N0, (int)((int64_t)B * N0), n->X0.p);
     UCHK(n, hipGetLastError());
     return 0;
 }
@@ -411,11 +410,10 @@ int loss_grad(lbm_unet* n, const float* target, int B, float* loss) {
     const int64_t N0 = n->nvox[0];
     UCHK(n, hipMemcpyAsync(n->d_tgt, target, (size_t)B * n->cfg.cout * N0 * sizeof(float), hipMemcpyHostToDevice,
                            n->stream));
-    const int64_t tot = (int64_t)B * N0 * 16;
-    const int64_t nb = nblk(tot);
+    const int64_t nb = nblk((int64_t)B * N0);
     const float inv = 1.0f / (float)((double)B * N0 * n->cfg.cout);
-    hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(kThreads), 0, n->stream, n->Out, n->d_tgt, n->cfg.cout, 16, N0,
-                       (int64_t)B * N0, inv, n->dOut.p, n->d_part);
+    hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(kThreads), 0, n->stream, n->Out, n->d_tgt, n->cfg.cout, (int)N0,
+                       (int)((int64_t)B * N0), inv, n->dOut.p, n->d_part);
     UCHK(n, hipGetLastError());
     std::vector<float> h((size_t)nb);
     UCHK(n, hipMemcpyAsync(h.data(), n->d_part, (size_t)nb * sizeof(float), hipMemcpyDeviceToHost, n->stream));
@@ -440,8 +438,10 @@ int lbm_unet_create(const lbm_unet_config* cfg, lbm_unet** out) {
     if (cfg->nx % f || cfg->ny % f || cfg->nz % f || cfg->nx < f || cfg->ny < f || cfg->nz < f) return -1;
     // wrapped convolutions run only in the brick kernel, which needs 16 x 4 x 4 voxels at the coarsest level
     if (cfg->periodic && (cfg->nx / f < 16 || cfg->ny / f < 4 || cfg->nz / f < 4)) return -1;
-    // the kernels index voxels (not elements) in 32 bits
+    // the kernels index voxels (not elements) in 32 bits, and the pooling
+    // kernels (voxel, 8-channel group) pairs of level 0 in 32 unsigned bits
     if ((int64_t)cfg->batch * cfg->nx * cfg->ny * cfg->nz >= (int64_t)1 << 31) return -1;
+    if ((int64_t)cfg->batch * cfg->nx * cfg->ny * cfg->nz * (cfg->base / 8) >= (int64_t)1 << 32) return -1;
     lbm_unet* n = new lbm_unet();
     n->cfg = *cfg;
     n->L = cfg->levels;
@@ -617,8 +617,8 @@ int lbm_unet_forward(lbm_unet* n, const float* x, int32_t B, float* y) {
     const int64_t N0 = n->nvox[0];
     float* d_y = n->d_tgt;  // staging for the host layout
     const int64_t tot = (int64_t)B * N0 * n->cfg.cout;
-    hipLaunchKernelGGL(k_unpack_output, dim3(nblk(tot)), dim3(kThreads), 0, n->stream, n->Out, n->cfg.cout, N0,
-                       (int64_t)B * N0, d_y);
+    hipLaunchKernelGGL(k_unpack_output, dim3(nblk((int64_t)B * N0)), dim3(kThreads), 0, n->stream, n->Out,
+                       n->cfg.cout, (int)N0, (int)((int64_t)B * N0), d_y);
     UCHK(n, hipGetLastError());
     UCHK(n, hipMemcpyAsync(y, d_y, (size_t)tot * 4, hipMemcpyDeviceToHost, n->stream));
     UCHK(n, hipStreamSynchronize(n->stream));

@@ -224,12 +224,13 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
 // (w + dz, mt + dy, r16 + dx); halo voxels outside the grid are zeros, or the
 // wrapped-around voxels along the axes a.wrap marks periodic.  The reduction
 // index is k = t * cin + ci as in the weight layout, t = 9 dz + 3 dy + dx: a
-// 32-wide k-step holds one tap column dx of 32 channels (CK = 32, cin a
+// 32-wide k-step holds one tap column of 32 channels (CK = 32, cin a
 // multiple of 32, one chunk of 32 channels at a time), two columns of 16
 // channels (CK = 16) or three of 8 (CK = 8, the fourth lane group reading
-// zeros), and the steps of one (dz, columns) group share their A fragments
-// (the loop below).  B fragments come from global memory (the weights stay
-// in L2).
+// zeros), and the three dy steps of one group share their A fragments (the
+// loop below).  B fragments come from global memory (the weights stay in L2;
+// staging them in LDS beside the brick measured slower: the lower occupancy
+// cost more than the L2 reads it saved).
 constexpr int kBz = 4, kBy = 4, kBx = 16;
 constexpr int kHz = kBz + 2, kHy = kBy + 2, kHx = kBx + 2;
 constexpr int kHvox = kHz * kHy * kHx;
@@ -242,15 +243,62 @@ struct BrickLds {
     static constexpr int S = CK == 32 ? 40 : 24;
 };
 
+// Staging of a brick chunk: thread t holds pieces p = t + i * kThreads
+// (piece = voxel v of the brick with its halo, 8-channel group q) in
+// registers, so all of its global loads are in flight at once, and the next
+// chunk's loads overlap the current chunk's MFMAs.
+template <int CK>
+struct BrickStage {
+    static constexpr int PPV = CK / 8;  // 16-byte pieces per voxel
+    static constexpr int NP = kHvox * PPV;
+    static constexpr int NIT = (NP + kThreads - 1) / kThreads;
+};
+
+template <int CK>
+__device__ inline void brick_load(const ConvArgs& a, int n, int x0, int y0, int z0, int j,
+                                  bf16x8 (&v)[BrickStage<CK>::NIT]) {
+    constexpr int PPV = BrickStage<CK>::PPV, NP = BrickStage<CK>::NP;
+    const bf16* xb = a.x + a.xco + j * 32;
+#pragma unroll
+    for (int i = 0; i < BrickStage<CK>::NIT; ++i) {
+        const int p = threadIdx.x + i * kThreads;
+        v[i] = zero8();
+        if (p < NP) {
+            const int vx = p / PPV, q = p - vx * PPV;
+            const int hz = vx / (kHy * kHx), rem = vx - hz * (kHy * kHx);
+            const int hy = rem / kHx, hx = rem - hy * kHx;
+            const int zz = pad_coord(z0 - 1 + hz, a.in.d, a.wrap & 4), yy = pad_coord(y0 - 1 + hy, a.in.h, a.wrap & 2),
+                      xx = pad_coord(x0 - 1 + hx, a.in.w, a.wrap & 1);
+            // (a brick that overhangs the grid's far end by more than the halo reads only zeros there)
+            if (zz >= 0 && yy >= 0 && xx >= 0 && zz < a.in.d && yy < a.in.h && xx < a.in.w)
+                v[i] = load8(xb + (int64_t)vox(a.in, n, zz, yy, xx) * a.xcs + q * 8);
+        }
+    }
+}
+
+template <int CK>
+__device__ inline void brick_store(bf16* s_in, const bf16x8 (&v)[BrickStage<CK>::NIT]) {
+    constexpr int PPV = BrickStage<CK>::PPV, NP = BrickStage<CK>::NP, S = BrickLds<CK>::S;
+#pragma unroll
+    for (int i = 0; i < BrickStage<CK>::NIT; ++i) {
+        const int p = threadIdx.x + i * kThreads;
+        if (p < NP) {
+            const int vx = p / PPV, q = p - vx * PPV;
+            *reinterpret_cast<bf16x8*>(&s_in[vx * S + q * 8]) = v[i];
+        }
+    }
+}
+
 template <int CK, int NT>
 __global__ __launch_bounds__(kThreads) void k_conv3_lds(ConvArgs a, int nbx, int nby, int nbz) {
     constexpr int S = BrickLds<CK>::S;
-    constexpr int PPV = CK / 8;  // 16-byte pieces per voxel
     __shared__ __attribute__((aligned(16))) bf16 s_in[kHvox * S];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int co_base = blockIdx.y * 16 * NT;
-    // brick coordinates: x fastest, then y, z, sample
+    // brick coordinates: x fastest, then y, z, sample (numbering the bricks
+    // per XCD, so that bricks sharing halo voxels share an L2, measured no
+    // faster)
     uint32_t b = blockIdx.x;
     const int bx = (int)(b % (uint32_t)nbx);
     b /= (uint32_t)nbx;
@@ -259,9 +307,6 @@ __global__ __launch_bounds__(kThreads) void k_conv3_lds(ConvArgs a, int nbx, int
     const int bz = (int)(b % (uint32_t)nbz);
     const int n = (int)(b / (uint32_t)nbz);
     const int x0 = bx * kBx, y0 = by * kBy, z0 = bz * kBz;
-    const Dims in = a.in;
-    const bf16* xb = a.x + a.xco;
-
     f32x4 acc[4][NT];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
@@ -269,76 +314,81 @@ __global__ __launch_bounds__(kThreads) void k_conv3_lds(ConvArgs a, int nbx, int
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int nchunks = CK == 32 ? a.cin / 32 : 1;
-    constexpr int kTapsPerStep = 32 / CK;
-    constexpr int kSteps = CK == 32 ? 27 : (27 + kTapsPerStep - 1) / kTapsPerStep;
     const int rowbase = wave * (kHy * kHx) + r16;  // brick voxel of (w, mt = 0, r16), tap (0, 0, 0)
     const bf16* wrow[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) wrow[nt] = a.wt + (int64_t)(co_base + nt * 16 + r16) * a.kpad;
 
+    bf16x8 stage[BrickStage<CK>::NIT];
+    brick_load<CK>(a, n, x0, y0, z0, 0, stage);
+    brick_store<CK>(s_in, stage);
+    __syncthreads();
     for (int j = 0; j < nchunks; ++j) {
-        if (j) __syncthreads();
-        // stage the brick: piece p = (voxel, 8-channel group)
-        for (int p = threadIdx.x; p < kHvox * PPV; p += kThreads) {
-            const int v = p / PPV, q = p - v * PPV;
-            const int hz = v / (kHy * kHx), rem = v - hz * (kHy * kHx);
-            const int hy = rem / kHx, hx = rem - hy * kHx;
-            const int zz = pad_coord(z0 - 1 + hz, in.d, a.wrap & 4), yy = pad_coord(y0 - 1 + hy, in.h, a.wrap & 2),
-                      xx = pad_coord(x0 - 1 + hx, in.w, a.wrap & 1);
-            bf16x8 val = zero8();
-            // (a brick that overhangs the grid's far end by more than the halo reads only zeros there)
-            if (zz >= 0 && yy >= 0 && xx >= 0 && zz < in.d && yy < in.h && xx < in.w)
-                val = load8(xb + (int64_t)vox(in, n, zz, yy, xx) * a.xcs + j * 32 + q * 8);
-            *reinterpret_cast<bf16x8*>(&s_in[v * S + q * 8]) = val;
-        }
-        __syncthreads();
-        // k-steps grouped by (dz, xg): the lanes of group g take the tap
-        // column dx(g) and channels c(g) of the group (CK = 32: dx = xg, all
-        // 32 channels; CK = 16: dx = 2 xg + g/2, 16 channels; CK = 8: dx = g,
-        // 8 channels; a dx past 2 reads zeros).  The six brick rows
-        // hy = 0..5 of that column serve the three taps dy = 0..2 of all four
+        // the next chunk's loads are in flight during this chunk's MFMAs
+        if (j + 1 < nchunks) brick_load<CK>(a, n, x0, y0, z0, j + 1, stage);
+        // k-steps in groups: in group gi the lanes of group g take one tap
+        // column (dz, dx) of the brick and channels c of it.  CK = 32: nine
+        // groups (dz, dx), all 32 channels.  CK = 16: groups 0-2 take
+        // dx = g/2 at dz = gi; groups 3 and 4 take the dx = 2 columns of
+        // dz = 0, 1 and of dz = 2 (the other half of group 4 reads zeros), so
+        // that 5 groups cover the 9 columns (16 channels each, c = 8 (g & 1)).
+        // CK = 8: group dz, dx = g (g = 3 reads zeros).  The six brick rows
+        // hy = 0..5 of the column serve the three taps dy = 0..2 of all four
         // m-tiles (row mt + dy), so each A fragment feeds two MFMAs on average.
-        constexpr int XG = CK == 32 ? 3 : (CK == 16 ? 2 : 1);
-        int dxg, cgl;
-        if (CK == 32) {
-            dxg = 0;
-            cgl = g * 8;
-        } else if (CK == 16) {
-            dxg = g >> 1;
-            cgl = (g & 1) * 8;
-        } else {
-            dxg = g;
-            cgl = 0;
-        }
-        for (int dz = 0; dz < 3; ++dz) {
+        // A lane's B fragment holds the weights of its own tap (9 dz + 3 dy +
+        // dx) and channels, so lanes of one k-step may take different taps.
+        constexpr int NG = CK == 32 ? 9 : (CK == 16 ? 5 : 3);
+        const int cgl = CK == 32 ? g * 8 : (CK == 16 ? (g & 1) * 8 : 0);
 #pragma unroll
-            for (int xg = 0; xg < XG; ++xg) {
-                const int dx = (CK == 32 ? xg : (CK == 16 ? 2 * xg : 0)) + dxg;
-                // weight row offset of tap (dz, dy = 0, the group's first column) for this lane
-                const int kw = (dz * 9) * a.cin + (CK == 32 ? xg * a.cin + j * 32 : (CK == 16 ? 32 * xg : 0)) + g * 8;
-                bf16x8 B[3][NT];
+        for (int gi = 0; gi < NG; ++gi) {
+            int dz, dx;
+            if (CK == 32) {
+                dz = gi / 3;
+                dx = gi % 3;
+            } else if (CK == 16) {
+                dz = gi < 3 ? gi : 2 * (gi - 3) + (g >> 1);
+                dx = gi < 3 ? (g >> 1) : 2;
+            } else {
+                dz = gi;
+                dx = g;
+            }
+            const bool valid = dz < 3 && dx < 3;
+            bf16x8 B[3][NT];
+            if (!valid) {
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) B[dy][nt] = zero8();
+            } else {
+                // weight row offset k = t * cin + ci of tap (dz, dy = 0, dx) and this lane's channels
+                const int kw = (dz * 9 + dx) * a.cin + (CK == 32 ? j * 32 : 0) + cgl;
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
                     for (int nt = 0; nt < NT; ++nt) B[dy][nt] = load8(wrow[nt] + kw + dy * 3 * a.cin);
-                bf16x8 F[kHy];
-                if (dx <= 2) {
-                    const int base = (rowbase + dz * (kHy * kHx) + dx) * S + cgl;
-#pragma unroll
-                    for (int hy = 0; hy < kHy; ++hy) F[hy] = load8(&s_in[base + hy * kHx * S]);
-                } else {
-#pragma unroll
-                    for (int hy = 0; hy < kHy; ++hy) F[hy] = zero8();
-                }
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                        for (int nt = 0; nt < NT; ++nt)
-                            acc[mt][nt] =
-                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[mt + dy], B[dy][nt], acc[mt][nt], 0, 0, 0);
             }
+            bf16x8 F[kHy];
+            if (valid) {
+                const int base = (rowbase + dz * (kHy * kHx) + dx) * S + cgl;
+#pragma unroll
+                for (int hy = 0; hy < kHy; ++hy) F[hy] = load8(&s_in[base + hy * kHx * S]);
+            } else {
+#pragma unroll
+                for (int hy = 0; hy < kHy; ++hy) F[hy] = zero8();
+            }
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[mt][nt] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[mt + dy], B[dy][nt], acc[mt][nt], 0, 0, 0);
+        }
+        if (j + 1 < nchunks) {
+            __syncthreads();
+            brick_store<CK>(s_in, stage);
+            __syncthreads();
         }
     }
     // epilogue: lane holds x = x0 + 4g + i of row (z0 + w, y0 + mt), column r16 of each tile
@@ -473,82 +523,113 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
         }
 }
 
-// 2x2x2 average pooling: out[v] = mean of the 8 children of v.
+// The elementwise kernels below take one thread per voxel (or per voxel and
+// 8-channel group, blockIdx.y) with 32-bit voxel indices (every tensor holds
+// fewer than 2^31 voxels) and 16-byte accesses to channels-last rows: an
+// index decomposition per element in 64 bits costs more than the access.
+__device__ inline void store8(bf16* p, const bf16x8& v) { *reinterpret_cast<bf16x8*>(p) = v; }
+
+// 2x2x2 average pooling: out[v] = mean of the 8 children of v.  Thread t
+// takes the 8-channel group t % cg8 of coarse voxel t / cg8 (groups fastest,
+// so that neighbouring lanes read neighbouring bytes); t < nv * cg8 < 2^32.
 __global__ __launch_bounds__(kThreads) void k_avgpool(const bf16* __restrict__ x, int xcs, int xco, bf16* __restrict__ y,
-                                                       int ycs, int yco, int c, Dims out, Dims in, int64_t total) {
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= total) return;
-    const int ch = (int)(i % c);
-    const int64_t v = i / c;
+                                                       int ycs, int yco, Dims out, Dims in, int cg8, uint32_t total) {
+    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= total) return;
+    const uint32_t v = t / (uint32_t)cg8;
+    const int c0 = (int)(t - v * (uint32_t)cg8) * 8;
     int n, z, yy, xx;
     unvox(out, v, n, z, yy, xx);
-    float s = 0.f;
+    float s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = 0.f;
     for (int p = 0; p < 8; ++p) {
         const int64_t iv = vox(in, n, 2 * z + (p >> 2), 2 * yy + ((p >> 1) & 1), 2 * xx + (p & 1));
-        s += (float)x[iv * xcs + xco + ch];
+        const bf16x8 a = load8(x + iv * xcs + xco + c0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s[j] += (float)a[j];
     }
-    y[v * ycs + yco + ch] = (bf16)(0.125f * s);
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = (bf16)(0.125f * s[j]);
+    store8(y + (int64_t)v * ycs + yco + c0, o);
 }
 
-// Backward of the pooling, added into dx and masked by (fwd > 0).
+// Backward of the pooling, added into dx and masked by (fwd > 0); thread t
+// takes group t % cg8 of fine voxel t / cg8.
 __global__ __launch_bounds__(kThreads) void k_avgpool_bwd(const bf16* __restrict__ dy, int dcs, int dco,
                                                            bf16* __restrict__ dx, int xcs, int xco,
-                                                           const bf16* __restrict__ fwd, int fcs, int fco, int c,
-                                                           Dims in, Dims out, int64_t total) {
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= total) return;
-    const int ch = (int)(i % c);
-    const int64_t v = i / c;  // fine voxel
+                                                           const bf16* __restrict__ fwd, int fcs, int fco,
+                                                           Dims in, Dims out, int cg8, uint32_t total) {
+    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= total) return;
+    const uint32_t v = t / (uint32_t)cg8;
+    const int c0 = (int)(t - v * (uint32_t)cg8) * 8;
     int n, z, yy, xx;
     unvox(in, v, n, z, yy, xx);
     const int64_t ov = vox(out, n, z >> 1, yy >> 1, xx >> 1);
-    float g = 0.125f * (float)dy[ov * dcs + dco + ch];
-    if (!((float)fwd[v * fcs + fco + ch] > 0.f)) g = 0.f;
-    bf16* p = dx + v * xcs + xco + ch;
-    *p = (bf16)((float)*p + g);
+    const bf16x8 g8 = load8(dy + ov * dcs + dco + c0);
+    const bf16x8 f8 = load8(fwd + (int64_t)v * fcs + fco + c0);
+    bf16* p = dx + (int64_t)v * xcs + xco + c0;
+    bf16x8 d8 = load8(p);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float g = 0.125f * (float)g8[j];
+        if (!((float)f8[j] > 0.f)) g = 0.f;
+        d8[j] = (bf16)((float)d8[j] + g);
+    }
+    store8(p, d8);
 }
 
-// FP32 host-layout input [b][c][N] -> BF16 channels-last with cp >= c channels.
-__global__ __launch_bounds__(kThreads) void k_pack_input(const float* __restrict__ src, int c, int cp, int64_t nvox,
-                                                          int64_t batch_vox, bf16* __restrict__ dst) {
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= batch_vox * cp) return;
-    const int ch = (int)(i % cp);
-    const int64_t v = i / cp;
-    const int64_t b = v / nvox, s = v % nvox;
-    dst[i] = (bf16)(ch < c ? src[(b * c + ch) * nvox + s] : 0.f);
+// FP32 host-layout input [b][c][N] -> BF16 channels-last with cp >= c
+// channels (a multiple of 8), one thread per voxel of the batch.
+__global__ __launch_bounds__(kThreads) void k_pack_input(const float* __restrict__ src, int c, int cp, int nvox,
+                                                          int batch_vox, bf16* __restrict__ dst) {
+    const int v = (int)(blockIdx.x * kThreads + threadIdx.x);
+    if (v >= batch_vox) return;
+    const int b = v / nvox, s = v - b * nvox;
+    for (int c0 = 0; c0 < cp; c0 += 8) {
+        bf16x8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            o[j] = (bf16)(c0 + j < c ? src[((int64_t)b * c + c0 + j) * nvox + s] : 0.f);
+        store8(dst + (int64_t)v * cp + c0, o);
+    }
 }
 
-// FP32 channels-last [v][c] -> host layout [b][c][N].
-__global__ __launch_bounds__(kThreads) void k_unpack_output(const float* __restrict__ src, int c, int64_t nvox,
-                                                             int64_t batch_vox, float* __restrict__ dst) {
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= batch_vox * c) return;
-    const int ch = (int)(i % c);
-    const int64_t v = i / c;
-    const int64_t b = v / nvox, s = v % nvox;
-    dst[(b * c + ch) * nvox + s] = src[i];
+// FP32 channels-last [v][c] -> host layout [b][c][N], one thread per voxel.
+__global__ __launch_bounds__(kThreads) void k_unpack_output(const float* __restrict__ src, int c, int nvox,
+                                                             int batch_vox, float* __restrict__ dst) {
+    const int v = (int)(blockIdx.x * kThreads + threadIdx.x);
+    if (v >= batch_vox) return;
+    const int b = v / nvox, s = v - b * nvox;
+    for (int ch = 0; ch < c; ++ch) dst[((int64_t)b * c + ch) * nvox + s] = src[(int64_t)v * c + ch];
 }
 
-// Mean-squared error: dy (BF16, channel stride cp, zero padded) = 2 (y - t) / count;
-// per-block partial sums of (y - t)^2 in FP32 to partial[blockIdx.x].
+// Mean-squared error, one thread per voxel: dy (BF16, channel stride cp = 16,
+// zero padded) = 2 (y - t) / count; per-block partial sums of (y - t)^2 in
+// FP32 to partial[blockIdx.x].
 __global__ __launch_bounds__(kThreads) void k_mse(const float* __restrict__ y, const float* __restrict__ target, int c,
-                                                   int cp, int64_t nvox, int64_t batch_vox, float inv_count,
+                                                   int nvox, int batch_vox, float inv_count,
                                                    bf16* __restrict__ dy, float* __restrict__ partial) {
     __shared__ float s[kThreads];
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int v = (int)(blockIdx.x * kThreads + threadIdx.x);
     float e2 = 0.f;
-    if (i < batch_vox * cp) {
-        const int ch = (int)(i % cp);
-        const int64_t v = i / cp;
-        float g = 0.f;
-        if (ch < c) {
-            const int64_t b = v / nvox, sidx = v % nvox;
-            const float d = y[v * c + ch] - target[(b * c + ch) * nvox + sidx];
-            e2 = d * d;
-            g = 2.f * d * inv_count;
+    if (v < batch_vox) {
+        const int b = v / nvox, sidx = v - b * nvox;
+        bf16x8 g[2];
+#pragma unroll
+        for (int ch = 0; ch < 16; ++ch) {
+            float gv = 0.f;
+            if (ch < c) {
+                const float d = y[(int64_t)v * c + ch] - target[((int64_t)b * c + ch) * nvox + sidx];
+                e2 += d * d;
+                gv = 2.f * d * inv_count;
+            }
+            g[ch >> 3][ch & 7] = (bf16)gv;
         }
-        dy[i] = (bf16)g;
+        store8(dy + (int64_t)v * 16, g[0]);
+        store8(dy + (int64_t)v * 16 + 8, g[1]);
     }
     s[threadIdx.x] = e2;
     __syncthreads();

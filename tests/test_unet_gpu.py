@@ -141,3 +141,4 @@ def test_periodic_padding_on_a_grid_too_small_fails():
 
     with pytest.raises(_ffi.LbmError):
         UNet((8, 8, 16), batch=1, levels=2, base=16, seed=1, periodic=6)  # level 1 is 8 voxels wide
+
