@@ -54,6 +54,8 @@ struct lbm_unet {
     float* d_tgt = nullptr;
     float* d_part = nullptr;
     float* d_bpart = nullptr;  // bias-gradient partials [kBiasParts][largest cout]
+    float* d_wpart = nullptr;  // weight-gradient block sums [chunks][npar][coutp][kpad] (grown on demand)
+    int64_t wpart_n = 0;
     int64_t part_n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -215,7 +217,7 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     a.dcs = dy.cs;
     a.dco = dco;
     a.coutp = L.coutp;
-    a.dw = n->dwp[li];
+    a.npar = L.geom == G_UP2 ? 8 : 1;
     a.kpad = L.kpad;
     a.ntaps = L.ntaps;
     a.rows = rows;
@@ -231,7 +233,19 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     a.nci_tiles = (L.cinp + TC - 1) / TC;
     const int nco_tiles = (L.coutp + TC - 1) / TC;
     const unsigned gy = L.geom == G_UP2 ? 8 : (unsigned)L.ntaps;
-    const dim3 grid((unsigned)((a.m + a.chunk - 1) / a.chunk), gy, (unsigned)(nco_tiles * a.nci_tiles));
+    const int nchunks = (int)((a.m + a.chunk - 1) / a.chunk);
+    const int64_t stride = (int64_t)a.npar * L.coutp * L.kpad;
+    if ((int64_t)nchunks * stride > n->wpart_n) {
+        // the block sums of the largest layer seen so far (the first training step grows it)
+        UCHK(n, hipStreamSynchronize(n->stream));
+        if (n->d_wpart) UCHK(n, hipFree(n->d_wpart));
+        n->d_wpart = nullptr;
+        n->wpart_n = 0;
+        UCHK(n, hipMalloc(&n->d_wpart, (size_t)nchunks * stride * sizeof(float)));
+        n->wpart_n = (int64_t)nchunks * stride;
+    }
+    a.part = n->d_wpart;
+    const dim3 grid((unsigned)nchunks, gy, (unsigned)(nco_tiles * a.nci_tiles));
 #define LAUNCH_WGRAD(G)                                                                          \
     if (N == 1) hipLaunchKernelGGL((k_wgrad<G, 1>), grid, dim3(kThreads), 0, n->stream, a);    \
     else if (N == 2) hipLaunchKernelGGL((k_wgrad<G, 2>), grid, dim3(kThreads), 0, n->stream, a); \
@@ -242,6 +256,9 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
         default: LAUNCH_WGRAD(G_PW); break;
     }
 #undef LAUNCH_WGRAD
+    UCHK(n, hipGetLastError());
+    hipLaunchKernelGGL(k_wgrad_sum, dim3((unsigned)((stride + kWsumEl - 1) / kWsumEl)), dim3(kThreads), 0, n->stream,
+                       n->d_wpart, nchunks, stride, L.kpad, L.ntaps * L.cinp, n->dwp[li]);
     UCHK(n, hipGetLastError());
     const int64_t ng = (int64_t)L.npar * L.cout * L.ntaps * L.cin;
     hipLaunchKernelGGL(k_unpack_grad, dim3(nblk(ng)), dim3(kThreads), 0, n->stream, n->dwp[li], L, n->d_grad);
@@ -346,11 +363,6 @@ int backward(lbm_unet* n, int B) {
     const int Lv = n->L;
     std::vector<Dims> d(Lv);
     for (int l = 0; l < Lv; ++l) d[l] = with_batch(n->dims[l], B);
-    for (size_t li = 0; li < n->layers.size(); ++li) {
-        const LayerDesc& Ld = n->layers[li];
-        const int64_t c = (int64_t)Ld.npar * Ld.coutp * Ld.kpad;
-        hipLaunchKernelGGL(k_fill_f32, dim3(nblk(c)), dim3(kThreads), 0, n->stream, n->dwp[li], c);
-    }
     int rc;
     // head
     rc = wgrad(n, n->head, n->D[0], 0, n->dOut, 0, d[0], d[0], d[0]);
@@ -556,6 +568,7 @@ int lbm_unet_destroy(lbm_unet* n) {
     if (!n) return 0;
     if (n->stream) (void)hipStreamSynchronize(n->stream);
     for (void* p : n->allocs) (void)hipFree(p);
+    if (n->d_wpart) (void)hipFree(n->d_wpart);
     if (n->ev0) (void)hipEventDestroy(n->ev0);
     if (n->ev1) (void)hipEventDestroy(n->ev1);
     if (n->stream) (void)hipStreamDestroy(n->stream);

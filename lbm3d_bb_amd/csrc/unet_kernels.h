@@ -418,7 +418,10 @@ __global__ __launch_bounds__(kThreads) void k_conv3_lds(ConvArgs a, int nbx, int
     }
 }
 
-// Weight gradient: dw[par][co][t*cin + ci] += sum_rows dy[out(row)][co] * x[in(row,t)][ci].
+// Weight gradient: dw[par][co][t*cin + ci] = sum_rows dy[out(row)][co] * x[in(row,t)][ci],
+// split over chunks of rows: block (chunk, tap, tile) writes its sum to
+// part[chunk][par][co][t*cin + ci] and k_wgrad_sum adds the chunks in order
+// (a fixed-order split-K sum, so gradients are bitwise reproducible).
 // Block: one tap (blockIdx.y), a TC x TC channel tile with TC = 16*N
 // (blockIdx.z), a chunk of rows (blockIdx.x).  The 4 waves are N ci sub-tiles
 // times 4/N row slabs of 32 rows; each step stages R = 128/N rows of TC dy and
@@ -431,7 +434,8 @@ struct WgradArgs {
     int xcs, xco, cin;     // cin = padded input channels (multiple of 8)
     const bf16* dy;
     int dcs, dco, coutp;
-    float* dw;             // [npar][coutp][kpad]
+    float* part;           // [chunks][npar][coutp][kpad] block sums
+    int npar;              // 8 for a transposed 2x2x2 convolution (blockIdx.y = parity), else 1
     int kpad, ntaps;
     Dims rows, in, out;
     int64_t m;
@@ -513,14 +517,42 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
     // acc[ct]: rows co = co0 + ct*16 + 4g + i, column ci = ci0 + cis*16 + r16
     const int ci = ci0 + cis * 16 + r16;
     if (ci >= a.cin) return;
-    float* dw = a.dw + (int64_t)par * a.coutp * a.kpad;
+    float* dw = a.part + ((int64_t)blockIdx.x * a.npar + par) * a.coutp * a.kpad;
 #pragma unroll
     for (int ct = 0; ct < N; ++ct)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int co = co0 + ct * 16 + 4 * g + i;
-            if (co < a.coutp) atomicAdd(&dw[(int64_t)co * a.kpad + t * a.cin + ci], acc[ct][i]);
+            if (co < a.coutp) dw[(int64_t)co * a.kpad + t * a.cin + ci] = acc[ct][i];
         }
+}
+
+// dw[e] = the sum over chunks c of part[c][e] (e = (par, co, k) of an
+// [npar][coutp][kpad] tensor) in a fixed order: a block takes 32 elements;
+// its 8 lane groups g each add the chunks c = g, g + 8, ... in turn, then
+// group 0 adds the 8 group sums in order.  k at or past kval (the taps times
+// the padded input channels) holds no block sum and gets zero.
+constexpr int kWsumEl = 32, kWsumGroups = kThreads / kWsumEl;
+
+__global__ __launch_bounds__(kThreads) void k_wgrad_sum(const float* __restrict__ part, int nchunks, int64_t stride,
+                                                         int kpad, int kval, float* __restrict__ dw) {
+    __shared__ float s_red[kWsumGroups][kWsumEl];
+    const int el = threadIdx.x % kWsumEl, grp = threadIdx.x / kWsumEl;
+    const int64_t e = (int64_t)blockIdx.x * kWsumEl + el;
+    const bool live = e < stride && (int)(e % kpad) < kval;
+    float s = 0.f;
+    if (live) {
+        const float* p = part + e;
+        for (int c = grp; c < nchunks; c += kWsumGroups) s += p[(int64_t)c * stride];
+    }
+    s_red[grp][el] = s;
+    __syncthreads();
+    if (grp == 0 && e < stride) {
+        float t = 0.f;
+#pragma unroll
+        for (int q = 0; q < kWsumGroups; ++q) t += s_red[q][el];
+        dw[e] = t;
+    }
 }
 
 // The elementwise kernels below take one thread per voxel (or per voxel and

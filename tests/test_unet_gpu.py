@@ -142,3 +142,18 @@ def test_periodic_padding_on_a_grid_too_small_fails():
     with pytest.raises(_ffi.LbmError):
         UNet((8, 8, 16), batch=1, levels=2, base=16, seed=1, periodic=6)  # level 1 is 8 voxels wide
 
+
+
+def test_gradient_bitwise_reproducible():
+    """The weight gradient is a fixed-order split-K sum (block sums per chunk of
+    rows, added in chunk order), so two gradient calls on the same inputs
+    return the same bits; 32^3 batch 4 runs hundreds of chunks per layer."""
+    shape = (32, 32, 32)
+    net = _net(shape, batch=4, levels=3, base=16, seed=31)
+    x, t = _data(shape, 4, seed=37)
+    l1, g1 = net.gradient(x, t)
+    l2, g2 = net.gradient(x, t)
+    net.close()
+    assert np.abs(g1).max() > 0
+    assert l1 == l2
+    assert np.array_equal(g1, g2)
