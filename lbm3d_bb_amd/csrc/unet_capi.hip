@@ -186,6 +186,30 @@ int conv(lbm_unet* n, int geom, const Buf& x, int xco, int cin, const bf16* w, i
         return 0;
     }
     if (a.wrap) return fail(n, -2, "periodic padding needs 3x3x3 convolutions of at least 16 x 4 x 4 voxels");
+    if (geom == G_UP2 && kpad % 32 == 0 && kpad / 32 <= 4 && (int64_t)nblk(a.m) * (coutp / 16) >= 256) {
+        // all 8 parities per block, the input rows read once (k_up2); a grid
+        // that would leave most CUs idle keeps one parity per block (k_conv)
+        while (nt > 1 && (int64_t)nblk(a.m) * (coutp / (16 * nt)) < 512) nt >>= 1;
+        const dim3 gridu(nblk(a.m), (unsigned)(coutp / (16 * nt)), 1), blocku(kThreads);
+        const int ks = kpad / 32;
+#define LAUNCH_UP2(NTV, KSV) hipLaunchKernelGGL((k_up2<NTV, KSV>), gridu, blocku, 0, n->stream, a)
+#define UP2_BY_KS(NTV)                     \
+    if (ks == 1) LAUNCH_UP2(NTV, 1);       \
+    else if (ks == 2) LAUNCH_UP2(NTV, 2);  \
+    else if (ks == 3) LAUNCH_UP2(NTV, 3);  \
+    else LAUNCH_UP2(NTV, 4);
+        if (nt == 1) {
+            UP2_BY_KS(1);
+        } else if (nt == 2) {
+            UP2_BY_KS(2);
+        } else {
+            UP2_BY_KS(4);
+        }
+#undef UP2_BY_KS
+#undef LAUNCH_UP2
+        UCHK(n, hipGetLastError());
+        return 0;
+    }
     while (nt > 1 && (int64_t)nblk(a.m) * (coutp / (16 * nt)) * npar < 512) nt >>= 1;
     const dim3 grid(nblk(a.m), (unsigned)(coutp / (16 * nt)), (unsigned)npar), block(kThreads);
 #define LAUNCH_CONV(G, NTV) hipLaunchKernelGGL((k_conv<G, NTV>), grid, block, 0, n->stream, a)

@@ -214,6 +214,87 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
     }
 }
 
+// Transposed 2x2x2 stride-2 convolution with all 8 output parities in one
+// block (k_conv takes one parity per block, blockIdx.z, and so reads every
+// input row 8 times from memory).  Rows are coarse voxels; a wave's 64 rows
+// load their KS 32-channel k-steps of input once into registers and feed the
+// MFMAs of every parity par (weights wt[par]); output row r of parity par is
+// fine voxel (2z + par/4, 2y + (par/2) % 2, 2x + par % 2) of the coarse voxel
+// (z, y, x) of r.  The epilogue is k_conv's.
+template <int NT, int KS>
+__global__ __launch_bounds__(kThreads) void k_up2(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int co_base = blockIdx.y * 16 * NT;
+    const int64_t row0 = (int64_t)blockIdx.x * kThreads + wave * 64;
+    bf16x8 A[KS][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int64_t r = row0 + mt * 16 + r16;
+        const bool ok = r < a.m;
+        int n, z, y, x;
+        unvox(a.rows, ok ? r : 0, n, z, y, x);
+        const int64_t v = vox(a.in, n, z, y, x);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int c0 = ks * 32 + g * 8;
+            A[ks][mt] = (ok && c0 < a.cin) ? load8(a.x + v * a.xcs + a.xco + c0) : zero8();
+        }
+    }
+    // the fine voxel of parity 0 of each output row this lane holds (rows 4g + i of each m-tile)
+    int ovb[4][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t r = row0 + mt * 16 + 4 * g + i;
+            int n, z, y, x;
+            unvox(a.rows, r < a.m ? r : 0, n, z, y, x);
+            ovb[mt][i] = (int)vox(a.out, n, 2 * z, 2 * y, 2 * x);
+        }
+    const int plane = a.out.h * a.out.w;
+    for (int par = 0; par < 8; ++par) {
+        const bf16* wt = a.wt + (int64_t)par * a.coutp * a.kpad;
+        f32x4 acc[4][NT];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            bf16x8 B[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) B[nt] = load8(wt + (int64_t)(co_base + nt * 16 + r16) * a.kpad + ks * 32 + g * 8);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][mt], B[nt], acc[mt][nt], 0, 0, 0);
+        }
+        const int poff = (par >> 2) * plane + ((par >> 1) & 1) * a.out.w + (par & 1);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t r = row0 + mt * 16 + 4 * g + i;
+                if (r >= a.m) continue;
+                const int64_t ov = (int64_t)ovb[mt][i] + poff;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int co = co_base + nt * 16 + r16;
+                    if (co >= a.cstore) continue;
+                    float v = acc[mt][nt][i];
+                    if (a.bias) v += a.bias[co];
+                    if (a.relu) v = v > 0.f ? v : 0.f;
+                    if (a.aux && !((float)a.aux[ov * a.acs + a.aco + co] > 0.f)) v = 0.f;
+                    if (a.yf) a.yf[ov * a.cstore + co] = v;
+                    else a.y[ov * a.ycs + a.yco + co] = (bf16)v;
+                }
+            }
+        }
+    }
+}
+
 // Same-size 3x3x3 convolution with the input staged in LDS.  A block computes
 // a brick of 4 (z) x 4 (y) x 16 (x) output voxels of one sample, 256 rows, for
 // 16*NT output channels: wave w takes the z-plane w of the brick, its m-tile mt
