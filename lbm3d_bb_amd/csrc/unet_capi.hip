@@ -262,9 +262,10 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     if ((int64_t)nchunks * stride > n->wpart_n) {
         // the block sums of the largest layer seen so far (the first training step grows it)
         UCHK(n, hipStreamSynchronize(n->stream));
-        if (n->d_wpart) UCHK(n, hipFree(n->d_wpart));
+        float* old = n->d_wpart;
         n->d_wpart = nullptr;
         n->wpart_n = 0;
+        if (old) UCHK(n, hipFree(old));
         UCHK(n, hipMalloc(&n->d_wpart, (size_t)nchunks * stride * sizeof(float)));
         n->wpart_n = (int64_t)nchunks * stride;
     }
