@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -56,6 +57,9 @@ struct lbm_unet {
     float* d_bpart = nullptr;  // bias-gradient partials [kBiasParts][largest cout]
     float* d_wpart = nullptr;  // weight-gradient block sums [chunks][npar][coutp][kpad] (grown on demand)
     int64_t wpart_n = 0;
+    // weight-gradient blocks numbered per XCD (LBM_UNET_WGRAD_XCD=0: the plain
+    // (chunk, tap, tile) grid, for comparison)
+    bool wgrad_xcd = true;
     int64_t part_n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -270,7 +274,12 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
         n->wpart_n = (int64_t)nchunks * stride;
     }
     a.part = n->d_wpart;
-    const dim3 grid((unsigned)nchunks, gy, (unsigned)(nco_tiles * a.nci_tiles));
+    a.xcd_order = n->wgrad_xcd ? 1 : 0;
+    a.nchunks = nchunks;
+    a.gy = (int)gy;
+    a.ntiles = nco_tiles * a.nci_tiles;
+    const dim3 grid = a.xcd_order ? dim3((unsigned)(((nchunks + 7) / 8) * 8 * (int)gy * a.ntiles))
+                                  : dim3((unsigned)nchunks, gy, (unsigned)a.ntiles);
 #define LAUNCH_WGRAD(G)                                                                          \
     if (N == 1) hipLaunchKernelGGL((k_wgrad<G, 1>), grid, dim3(kThreads), 0, n->stream, a);    \
     else if (N == 2) hipLaunchKernelGGL((k_wgrad<G, 2>), grid, dim3(kThreads), 0, n->stream, a); \
@@ -481,6 +490,7 @@ int lbm_unet_create(const lbm_unet_config* cfg, lbm_unet** out) {
     if ((int64_t)cfg->batch * cfg->nx * cfg->ny * cfg->nz * (cfg->base / 8) >= (int64_t)1 << 32) return -1;
     lbm_unet* n = new lbm_unet();
     n->cfg = *cfg;
+    if (const char* e = std::getenv("LBM_UNET_WGRAD_XCD")) n->wgrad_xcd = std::atoi(e) != 0;
     n->L = cfg->levels;
     for (int l = 0; l < n->L; ++l) {
         n->C.push_back(cfg->base << l);

@@ -523,6 +523,8 @@ struct WgradArgs {
     int64_t chunk;         // rows per block
     int nci_tiles;         // ceil(cin / TC)
     int wrap;              // periodic axes of a 3x3x3 convolution (as ConvArgs::wrap)
+    int xcd_order;         // 1: a 1-D grid numbered per XCD (see k_wgrad), 0: grid (chunks, taps, tiles)
+    int nchunks, gy, ntiles;
 };
 
 template <int GEOM, int N>
@@ -533,11 +535,30 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int cis = wave % N, slab = wave / N;
-    const int t = GEOM == G_UP2 ? 0 : blockIdx.y;
-    const int par = GEOM == G_UP2 ? blockIdx.y : 0;
-    const int co_tile = blockIdx.z / a.nci_tiles, ci_tile = blockIdx.z % a.nci_tiles;
+    // Block -> (chunk, tap or parity, channel tile).  With a.xcd_order the
+    // grid is 1-D and numbered so that every block of one chunk runs on the
+    // same XCD (the hardware deals block ids round-robin to the 8 XCDs) and
+    // close together in time: the chunk's dy rows and the x rows of its 27
+    // taps are then read from that XCD's L2 instead of from memory by each
+    // tap's block.  Grid: ceil(chunks / 8) * 8 * taps * tiles blocks.
+    int chunk, ty, tile;
+    if (a.xcd_order) {
+        const uint32_t b = blockIdx.x, inner = (uint32_t)(a.gy * a.ntiles);
+        const uint32_t q = b >> 3, j = q % inner;
+        chunk = (int)((q / inner) * 8u + (b & 7u));
+        ty = (int)(j % (uint32_t)a.gy);
+        tile = (int)(j / (uint32_t)a.gy);
+        if (chunk >= a.nchunks) return;
+    } else {
+        chunk = blockIdx.x;
+        ty = blockIdx.y;
+        tile = blockIdx.z;
+    }
+    const int t = GEOM == G_UP2 ? 0 : ty;
+    const int par = GEOM == G_UP2 ? ty : 0;
+    const int co_tile = tile / a.nci_tiles, ci_tile = tile % a.nci_tiles;
     const int co0 = co_tile * TC, ci0 = ci_tile * TC;
-    const int64_t rbeg = (int64_t)blockIdx.x * a.chunk;
+    const int64_t rbeg = (int64_t)chunk * a.chunk;
     const int64_t rend = rbeg + a.chunk < a.m ? rbeg + a.chunk : a.m;
 
     f32x4 acc[N];
@@ -598,7 +619,7 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
     // acc[ct]: rows co = co0 + ct*16 + 4g + i, column ci = ci0 + cis*16 + r16
     const int ci = ci0 + cis * 16 + r16;
     if (ci >= a.cin) return;
-    float* dw = a.part + ((int64_t)blockIdx.x * a.npar + par) * a.coutp * a.kpad;
+    float* dw = a.part + ((int64_t)chunk * a.npar + par) * a.coutp * a.kpad;
 #pragma unroll
     for (int ct = 0; ct < N; ++ct)
 #pragma unroll

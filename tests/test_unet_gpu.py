@@ -157,3 +157,21 @@ def test_gradient_bitwise_reproducible():
     assert np.abs(g1).max() > 0
     assert l1 == l2
     assert np.array_equal(g1, g2)
+
+
+@pytest.mark.parametrize("base", [16, 32])
+def test_weight_gradient_block_order_bitwise(base, monkeypatch):
+    """The weight-gradient blocks numbered per XCD compute the same chunk sums
+    as the plain (chunk, tap, tile) grid, so the gradients are bitwise equal;
+    batch 3 at 32^3 gives chunk counts that are not multiples of 8."""
+    shape = (32, 32, 32)
+    x, t = _data(shape, 3, seed=41)
+    res = []
+    for order in ("0", "1"):
+        monkeypatch.setenv("LBM_UNET_WGRAD_XCD", order)
+        net = _net(shape, batch=3, levels=3, base=base, seed=43)
+        res.append(net.gradient(x, t))
+        net.close()
+    assert np.abs(res[0][1]).max() > 0
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1])
