@@ -840,29 +840,47 @@ __global__ __launch_bounds__(kThreads) void k_unpack_grad(const float* __restric
 
 // Bias gradient, sum over voxels of dy[v][co], in two fixed-order passes.
 // Pass 1: block (b, cb) sums its range of `chunk` voxels for channels
-// cb*cp .. cb*cp + cp - 1 (cp a power of two <= kThreads): thread = (channel
-// c = tid % cp, phase tid / cp), so a wave reads whole channel runs of
-// consecutive voxels; the phases are then summed by a tree in LDS.
+// cb*cp .. cb*cp + cp - 1 (cp a power of two <= kThreads): thread = (8-channel
+// group tid % ng, phase tid / ng), ng = ceil(cp / 8), so that a lane loads 16
+// bytes and a wave reads whole channel runs of consecutive voxels; the phases
+// are then summed by a tree in LDS.  The channel stride and offset of dy are
+// multiples of 8 and the rows hold at least 8 channels (zero padded).
 constexpr int kBiasParts = 512;
 
 __global__ __launch_bounds__(kThreads) void k_bias_part(const bf16* __restrict__ dy, int dcs, int dco, int cout,
                                                          int cp, int64_t nvox, int64_t chunk,
                                                          float* __restrict__ part) {
-    __shared__ float s[kThreads];
-    const int nph = kThreads / cp, ph = threadIdx.x / cp;
-    const int c = blockIdx.y * cp + (int)(threadIdx.x % cp);
+    __shared__ float s[8][kThreads];
+    const int ng = (cp + 7) >> 3, nph = kThreads / ng, ph = threadIdx.x / ng;
+    const int c0 = blockIdx.y * cp + (int)(threadIdx.x % ng) * 8;
     const int64_t v0 = (int64_t)blockIdx.x * chunk;
     const int64_t v1 = v0 + chunk < nvox ? v0 + chunk : nvox;
-    float a = 0.f;
-    if (c < cout)
-        for (int64_t v = v0 + ph; v < v1; v += nph) a += (float)dy[v * dcs + dco + c];
-    s[threadIdx.x] = a;
+    float a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = 0.f;
+    if (c0 < cout)
+        for (int64_t v = v0 + ph; v < v1; v += nph) {
+            const bf16x8 x = load8(dy + v * dcs + dco + c0);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] += (float)x[j];
+        }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j][threadIdx.x] = a[j];
     __syncthreads();
     for (int off = nph / 2; off > 0; off >>= 1) {
-        if (ph < off) s[threadIdx.x] += s[threadIdx.x + off * cp];
+        if (ph < off) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j][threadIdx.x] += s[j][threadIdx.x + off * ng];
+        }
         __syncthreads();
     }
-    if (ph == 0 && c < cout) part[(int64_t)blockIdx.x * cout + c] = s[threadIdx.x];
+    if (ph == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int c = c0 + j;
+            if (j < cp && c < cout) part[(int64_t)blockIdx.x * cout + c] = s[j][threadIdx.x];
+        }
+    }
 }
 
 // Pass 2: grad[b_off + co] = sum of the nb partials of channel co (one block per channel).
