@@ -591,16 +591,21 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
             }
         }
         __syncthreads();
-        const bf16x8 B = load8(&s_x[cis * 16 + r16][slab * 32 + g * 8]);
+        // sub-tiles past the channel counts (a 16-channel side of a 32-channel
+        // tile) hold zeros: their waves and MFMAs skip them (wave-uniform)
+        if (ci0 + cis * 16 < a.cin) {
+            const bf16x8 B = load8(&s_x[cis * 16 + r16][slab * 32 + g * 8]);
 #pragma unroll
-        for (int ct = 0; ct < N; ++ct) {
-            const bf16x8 A = load8(&s_dy[ct * 16 + r16][slab * 32 + g * 8]);
-            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[ct], 0, 0, 0);
+            for (int ct = 0; ct < N; ++ct) {
+                if (co0 + ct * 16 >= a.coutp) continue;
+                const bf16x8 A = load8(&s_dy[ct * 16 + r16][slab * 32 + g * 8]);
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[ct], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
     // the row slabs of one ci sub-tile add their sums in LDS (fixed order), so
-    // only the first slab's wave issues the atomics
+    // only the first slab's wave writes the block sums
     if (N < 4) {
         __shared__ float s_acc[4][4 * N][64];
 #pragma unroll
