@@ -58,8 +58,10 @@ struct lbm_unet {
     float* d_wpart = nullptr;  // weight-gradient block sums [chunks][npar][coutp][kpad] (grown on demand)
     int64_t wpart_n = 0;
     // weight-gradient blocks numbered per XCD (LBM_UNET_WGRAD_XCD=0: the plain
-    // (chunk, tap, tile) grid, for comparison)
+    // (chunk, tap, tile) grid, for comparison); the all-taps brick kernel for
+    // the 16-channel layers (LBM_UNET_WGRAD_BRICK=0: the row-chunk kernel)
     bool wgrad_xcd = true;
+    bool wgrad_brick = true;
     int64_t part_n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -261,7 +263,12 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     a.nci_tiles = (L.cinp + TC - 1) / TC;
     const int nco_tiles = (L.coutp + TC - 1) / TC;
     const unsigned gy = L.geom == G_UP2 ? 8 : (unsigned)L.ntaps;
-    const int nchunks = (int)((a.m + a.chunk - 1) / a.chunk);
+    // 16-output-channel 3x3x3 layers with 8 or 16 input channels: all 27 taps
+    // per block from LDS bricks (k_wgrad3_lds), one partial per block
+    const bool brick = n->wgrad_brick && L.geom == G_CONV3 && L.coutp == 16 && (L.cinp == 8 || L.cinp == 16);
+    const int nbx = (in.w + kBx - 1) / kBx, nby = (in.h + kBy - 1) / kBy, nbz = (in.d + kBz - 1) / kBz;
+    const int64_t nbricks = (int64_t)nbx * nby * nbz * in.n;
+    const int nchunks = brick ? (int)std::min<int64_t>(nbricks, 512) : (int)((a.m + a.chunk - 1) / a.chunk);
     const int64_t stride = (int64_t)a.npar * L.coutp * L.kpad;
     if ((int64_t)nchunks * stride > n->wpart_n) {
         // the block sums of the largest layer seen so far (the first training step grows it)
@@ -274,6 +281,14 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
         n->wpart_n = (int64_t)nchunks * stride;
     }
     a.part = n->d_wpart;
+    if (brick) {
+        if (L.cinp == 8)
+            hipLaunchKernelGGL((k_wgrad3_lds<8>), dim3((unsigned)nchunks), dim3(kThreads), 0, n->stream, a, nbx, nby, nbz,
+                               (int)nbricks);
+        else
+            hipLaunchKernelGGL((k_wgrad3_lds<16>), dim3((unsigned)nchunks), dim3(kThreads), 0, n->stream, a, nbx, nby,
+                               nbz, (int)nbricks);
+    }
     a.xcd_order = n->wgrad_xcd ? 1 : 0;
     a.nchunks = nchunks;
     a.gy = (int)gy;
@@ -284,10 +299,12 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     if (N == 1) hipLaunchKernelGGL((k_wgrad<G, 1>), grid, dim3(kThreads), 0, n->stream, a);    \
     else if (N == 2) hipLaunchKernelGGL((k_wgrad<G, 2>), grid, dim3(kThreads), 0, n->stream, a); \
     else hipLaunchKernelGGL((k_wgrad<G, 4>), grid, dim3(kThreads), 0, n->stream, a);
-    switch (L.geom) {
-        case G_CONV3: LAUNCH_WGRAD(G_CONV3); break;
-        case G_UP2: LAUNCH_WGRAD(G_UP2); break;
-        default: LAUNCH_WGRAD(G_PW); break;
+    if (!brick) {
+        switch (L.geom) {
+            case G_CONV3: LAUNCH_WGRAD(G_CONV3); break;
+            case G_UP2: LAUNCH_WGRAD(G_UP2); break;
+            default: LAUNCH_WGRAD(G_PW); break;
+        }
     }
 #undef LAUNCH_WGRAD
     UCHK(n, hipGetLastError());
@@ -491,6 +508,7 @@ int lbm_unet_create(const lbm_unet_config* cfg, lbm_unet** out) {
     lbm_unet* n = new lbm_unet();
     n->cfg = *cfg;
     if (const char* e = std::getenv("LBM_UNET_WGRAD_XCD")) n->wgrad_xcd = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LBM_UNET_WGRAD_BRICK")) n->wgrad_brick = std::atoi(e) != 0;
     n->L = cfg->levels;
     for (int l = 0; l < n->L; ++l) {
         n->C.push_back(cfg->base << l);

@@ -634,6 +634,116 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
         }
 }
 
+// Weight gradient of a same-size 3x3x3 convolution with 16 output channels
+// and CK = 8 or 16 input channels, all 27 taps in one block.  A block walks a
+// fixed range of bricks of 4 (z) x 4 (y) x 16 (x) output voxels; per brick it
+// stages dy (channel-major, so a lane reads 8 consecutive rows of one output
+// channel as the A fragment of dy^T) and the input brick with its one-voxel
+// halo, as three copies shifted by dx = 0, 1, 2 in x (so the B fragment of
+// every tap is an aligned 16-byte read), once, and runs the 27 taps from LDS:
+// wave w takes output plane w (64 rows, two 32-row k-steps), tap t adds
+// dy^T x_t into acc[t] (a 16 x 16 tile, co x ci).  At the end the four waves'
+// tiles are added in order and the block writes them as its partial
+// part[block][co][t * cin + ci] (k_wgrad_sum adds the blocks in order).  The
+// row-chunk kernel (k_wgrad) stages its rows once per tap instead.
+template <int CK>
+struct WgradBrick {
+    static constexpr int CIS = kHz * kHy * 16 + 8;  // x copy: ci stride (elements); the 16 ci lanes of a B read on disjoint banks
+    static constexpr int COPY = CK * CIS;
+    static constexpr int COS = kBz * 64 + 8;       // dy: co stride (elements), likewise
+};
+
+template <int CK>
+__global__ __launch_bounds__(kThreads) void k_wgrad3_lds(WgradArgs a, int nbx, int nby, int nbz, int nbricks) {
+    using WB = WgradBrick<CK>;
+    __shared__ __attribute__((aligned(16))) bf16 s_x[3 * WB::COPY];
+    __shared__ __attribute__((aligned(16))) bf16 s_dy[16 * WB::COS];
+    __shared__ float s_red[kThreads / 64][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int bb0 = (int)((int64_t)blockIdx.x * nbricks / gridDim.x);
+    const int bb1 = (int)((int64_t)(blockIdx.x + 1) * nbricks / gridDim.x);
+    const Dims in = a.in;
+    f32x4 acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int br = bb0; br < bb1; ++br) {
+        uint32_t b = (uint32_t)br;
+        const int bx = (int)(b % (uint32_t)nbx);
+        b /= (uint32_t)nbx;
+        const int by = (int)(b % (uint32_t)nby);
+        b /= (uint32_t)nby;
+        const int bz = (int)(b % (uint32_t)nbz);
+        const int n = (int)(b / (uint32_t)nbz);
+        const int x0 = bx * kBx, y0 = by * kBy, z0 = bz * kBz;
+        __syncthreads();  // the previous brick's reads are done
+        // dy: piece p = (voxel v of the brick, 8-channel group q)
+        for (int p = threadIdx.x; p < kBz * 64 * 2; p += kThreads) {
+            const int v = p >> 1, q = p & 1;
+            const int w = v >> 6, r = v & 63;
+            const int zz = z0 + w, yy = y0 + (r >> 4), xx = x0 + (r & 15);
+            bf16x8 val = zero8();
+            if (zz < a.out.d && yy < a.out.h && xx < a.out.w)
+                val = load8(a.dy + vox(a.out, n, zz, yy, xx) * a.dcs + a.dco + q * 8);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s_dy[(q * 8 + j) * WB::COS + v] = val[j];
+        }
+        // x with its halo: piece p = (halo voxel, 8-channel group q), into the copies it belongs to
+        for (int p = threadIdx.x; p < kHvox * (CK / 8); p += kThreads) {
+            const int v = p / (CK / 8), q = p - v * (CK / 8);
+            const int hz = v / (kHy * kHx), rem = v - hz * (kHy * kHx);
+            const int hy = rem / kHx, hx = rem - hy * kHx;
+            const int zz = pad_coord(z0 - 1 + hz, in.d, a.wrap & 4), yy = pad_coord(y0 - 1 + hy, in.h, a.wrap & 2),
+                      xx = pad_coord(x0 - 1 + hx, in.w, a.wrap & 1);
+            bf16x8 val = zero8();
+            if (zz >= 0 && yy >= 0 && xx >= 0 && zz < in.d && yy < in.h && xx < in.w)
+                val = load8(a.x + vox(in, n, zz, yy, xx) * a.xcs + a.xco + q * 8);
+            const int row = (hz * kHy + hy) * 16;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int e = hx - dx;
+                if (e < 0 || e >= 16) continue;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) s_x[dx * WB::COPY + (q * 8 + j) * WB::CIS + row + e] = val[j];
+            }
+        }
+        __syncthreads();
+        // k-step s of the wave: rows s*32 + g*8 .. + 7 = y-row 2s + g/2, x (g & 1) * 8 .. + 7
+        const bf16x8 A0 = load8(&s_dy[r16 * WB::COS + wave * 64 + g * 8]);
+        const bf16x8 A1 = load8(&s_dy[r16 * WB::COS + wave * 64 + 32 + g * 8]);
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const int dz = t / 9, dyy = (t / 3) % 3, dx = t % 3;
+#pragma unroll
+            for (int st = 0; st < 2; ++st) {
+                const int hz = wave + dz, hy = 2 * st + (g >> 1) + dyy;
+                const bf16x8 B = r16 < CK ? load8(&s_x[dx * WB::COPY + r16 * WB::CIS + (hz * kHy + hy) * 16 + (g & 1) * 8])
+                                          : zero8();
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(st ? A1 : A0, B, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    // the waves' tiles, tap by tap, added in wave order; acc[t] lane (g, r16)
+    // element i: co = 4g + i, ci = r16
+    float* part = a.part + (int64_t)blockIdx.x * a.coutp * a.kpad;
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s_red[wave][i * 64 + lane] = acc[t][i];
+        __syncthreads();
+        if (wave == 0 && r16 < a.cin) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = s_red[0][i * 64 + lane];
+#pragma unroll
+                for (int w = 1; w < kThreads / 64; ++w) v += s_red[w][i * 64 + lane];
+                part[(int64_t)(4 * g + i) * a.kpad + t * a.cin + r16] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // dw[e] = the sum over chunks c of part[c][e] (e = (par, co, k) of an
 // [npar][coutp][kpad] tensor) in a fixed order: a block takes 32 elements;
 // its 8 lane groups g each add the chunks c = g, g + 8, ... in turn, then
