@@ -525,6 +525,7 @@ struct WgradArgs {
     int wrap;              // periodic axes of a 3x3x3 convolution (as ConvArgs::wrap)
     int xcd_order;         // 1: a 1-D grid numbered per XCD (see k_wgrad), 0: grid (chunks, taps, tiles)
     int nchunks, gy, ntiles;
+    int ci_off;            // k_wgrad3_lds: the first input channel of this call (x read from xco + ci_off)
 };
 
 template <int GEOM, int N>
@@ -635,7 +636,8 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
 }
 
 // Weight gradient of a same-size 3x3x3 convolution with 16 output channels
-// and CK = 8 or 16 input channels, all 27 taps in one block.  A block walks a
+// and CK = 8 or 16 input channels (a wider input is run in 16-channel slices,
+// one call each, ci_off = 0, 16, ...), all 27 taps in one block.  A block walks a
 // fixed range of bricks of 4 (z) x 4 (y) x 16 (x) output voxels; per brick it
 // stages dy (channel-major, so a lane reads 8 consecutive rows of one output
 // channel as the A fragment of dy^T) and the input brick with its one-voxel
@@ -697,7 +699,7 @@ __global__ __launch_bounds__(kThreads) void k_wgrad3_lds(WgradArgs a, int nbx, i
                       xx = pad_coord(x0 - 1 + hx, in.w, a.wrap & 1);
             bf16x8 val = zero8();
             if (zz >= 0 && yy >= 0 && xx >= 0 && zz < in.d && yy < in.h && xx < in.w)
-                val = load8(a.x + vox(in, n, zz, yy, xx) * a.xcs + a.xco + q * 8);
+                val = load8(a.x + vox(in, n, zz, yy, xx) * a.xcs + a.xco + a.ci_off + q * 8);
             const int row = (hz * kHy + hy) * 16;
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
@@ -731,13 +733,13 @@ __global__ __launch_bounds__(kThreads) void k_wgrad3_lds(WgradArgs a, int nbx, i
 #pragma unroll
         for (int i = 0; i < 4; ++i) s_red[wave][i * 64 + lane] = acc[t][i];
         __syncthreads();
-        if (wave == 0 && r16 < a.cin) {
+        if (wave == 0 && r16 < CK && a.ci_off + r16 < a.cin) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 float v = s_red[0][i * 64 + lane];
 #pragma unroll
                 for (int w = 1; w < kThreads / 64; ++w) v += s_red[w][i * 64 + lane];
-                part[(int64_t)(4 * g + i) * a.kpad + t * a.cin + r16] = v;
+                part[(int64_t)(4 * g + i) * a.kpad + t * a.cin + a.ci_off + r16] = v;
             }
         }
         __syncthreads();
