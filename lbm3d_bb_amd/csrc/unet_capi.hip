@@ -265,9 +265,11 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     const unsigned gy = L.geom == G_UP2 ? 8 : (unsigned)L.ntaps;
     // 16-output-channel 3x3x3 layers with 8 or 16 input channels: all 27 taps
     // per block from LDS bricks (k_wgrad3_lds), one partial per block
-    const bool brick = n->wgrad_brick && L.geom == G_CONV3 && L.coutp == 16 && (L.cinp == 8 || L.cinp % 16 == 0);
     const int nbx = (in.w + kBx - 1) / kBx, nby = (in.h + kBy - 1) / kBy, nbz = (in.d + kBz - 1) / kBz;
     const int64_t nbricks = (int64_t)nbx * nby * nbz * in.n;
+    // (one call per 16 x 16 channel slice pair, on grids of at least 512 bricks)
+    const bool brick = n->wgrad_brick && L.geom == G_CONV3 && L.coutp % 16 == 0 &&
+                       (L.cinp == 8 || L.cinp % 16 == 0) && (L.coutp == 16 || nbricks >= 512);
     const int nchunks = brick ? (int)std::min<int64_t>(nbricks, 512) : (int)((a.m + a.chunk - 1) / a.chunk);
     const int64_t stride = (int64_t)a.npar * L.coutp * L.kpad;
     if ((int64_t)nchunks * stride > n->wpart_n) {
@@ -282,18 +284,24 @@ int wgrad(lbm_unet* n, int li, const Buf& x, int xco, const Buf& dy, int dco, Di
     }
     a.part = n->d_wpart;
     if (brick) {
-        if (L.cinp == 8) {
-            hipLaunchKernelGGL((k_wgrad3_lds<8>), dim3((unsigned)nchunks), dim3(kThreads), 0, n->stream, a, nbx, nby, nbz,
-                               (int)nbricks);
-        } else {
-            // one call per 16 input channels; each writes its channels of every block partial
+        // one call per (16 output, 16 input) channel slice; each writes its
+        // channels of every block partial
+        for (int o0 = 0; o0 < L.coutp; o0 += 16) {
+            a.co_off = o0;
+            if (L.cinp == 8) {
+                a.ci_off = 0;
+                hipLaunchKernelGGL((k_wgrad3_lds<8>), dim3((unsigned)nchunks), dim3(kThreads), 0, n->stream, a, nbx,
+                                   nby, nbz, (int)nbricks);
+                continue;
+            }
             for (int c0 = 0; c0 < L.cinp; c0 += 16) {
                 a.ci_off = c0;
                 hipLaunchKernelGGL((k_wgrad3_lds<16>), dim3((unsigned)nchunks), dim3(kThreads), 0, n->stream, a, nbx,
                                    nby, nbz, (int)nbricks);
             }
-            a.ci_off = 0;
         }
+        a.ci_off = 0;
+        a.co_off = 0;
     }
     a.xcd_order = n->wgrad_xcd ? 1 : 0;
     a.nchunks = nchunks;

@@ -525,7 +525,7 @@ struct WgradArgs {
     int wrap;              // periodic axes of a 3x3x3 convolution (as ConvArgs::wrap)
     int xcd_order;         // 1: a 1-D grid numbered per XCD (see k_wgrad), 0: grid (chunks, taps, tiles)
     int nchunks, gy, ntiles;
-    int ci_off;            // k_wgrad3_lds: the first input channel of this call (x read from xco + ci_off)
+    int ci_off, co_off;    // k_wgrad3_lds: the first input and output channels of this call
 };
 
 template <int GEOM, int N>
@@ -635,9 +635,10 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(WgradArgs a) {
         }
 }
 
-// Weight gradient of a same-size 3x3x3 convolution with 16 output channels
-// and CK = 8 or 16 input channels (a wider input is run in 16-channel slices,
-// one call each, ci_off = 0, 16, ...), all 27 taps in one block.  A block walks a
+// Weight gradient of a same-size 3x3x3 convolution, 16 output channels and
+// CK = 8 or 16 input channels per call (wider layers run in 16-channel slices
+// of each side, one call per pair: co_off, ci_off = 0, 16, ...), all 27 taps
+// in one block.  A block walks a
 // fixed range of bricks of 4 (z) x 4 (y) x 16 (x) output voxels; per brick it
 // stages dy (channel-major, so a lane reads 8 consecutive rows of one output
 // channel as the A fragment of dy^T) and the input brick with its one-voxel
@@ -686,7 +687,7 @@ __global__ __launch_bounds__(kThreads) void k_wgrad3_lds(WgradArgs a, int nbx, i
             const int zz = z0 + w, yy = y0 + (r >> 4), xx = x0 + (r & 15);
             bf16x8 val = zero8();
             if (zz < a.out.d && yy < a.out.h && xx < a.out.w)
-                val = load8(a.dy + vox(a.out, n, zz, yy, xx) * a.dcs + a.dco + q * 8);
+                val = load8(a.dy + vox(a.out, n, zz, yy, xx) * a.dcs + a.dco + a.co_off + q * 8);
 #pragma unroll
             for (int j = 0; j < 8; ++j) s_dy[(q * 8 + j) * WB::COS + v] = val[j];
         }
@@ -739,7 +740,7 @@ __global__ __launch_bounds__(kThreads) void k_wgrad3_lds(WgradArgs a, int nbx, i
                 float v = s_red[0][i * 64 + lane];
 #pragma unroll
                 for (int w = 1; w < kThreads / 64; ++w) v += s_red[w][i * 64 + lane];
-                part[(int64_t)(4 * g + i) * a.kpad + t * a.cin + a.ci_off + r16] = v;
+                part[(int64_t)(a.co_off + 4 * g + i) * a.kpad + t * a.cin + a.ci_off + r16] = v;
             }
         }
         __syncthreads();

@@ -177,21 +177,22 @@ def test_weight_gradient_block_order_bitwise(base, monkeypatch):
     assert np.array_equal(res[0][1], res[1][1])
 
 
-@pytest.mark.parametrize("periodic", [0, 6])
-def test_weight_gradient_brick_kernel_matches_row_chunks(periodic, monkeypatch):
-    """The all-taps brick kernel of the 16-output-channel 3x3x3 layers (8 and
-    16 input channels) against the row-chunk kernel: the same sums in another
-    order, so the gradients agree to FP32 rounding; batch 3 at 32 x 16 x 48
-    gives bricks that overhang the grid in x and a block count that does not
-    divide the bricks evenly."""
+@pytest.mark.parametrize("shape,batch,base,periodic", [((32, 16, 48), 3, 16, 0), ((32, 16, 48), 3, 16, 6),
+                                                       ((64, 64, 64), 1, 32, 6)])
+def test_weight_gradient_brick_kernel_matches_row_chunks(shape, batch, base, periodic, monkeypatch):
+    """The all-taps brick kernel of the 3x3x3 layers against the row-chunk
+    kernel: the same sums in another order, so the gradients agree to FP32
+    rounding.  Batch 3 at 32 x 16 x 48 (base 16: 8, 16 and 32 input channels
+    into 16) gives bricks that overhang the grid in x and a block count that
+    does not divide the bricks evenly; 64^3 with base 32 runs the 32-output-
+    channel layers of level 0 (1024 bricks) in 16 x 16 channel slices."""
     from lbm3d_bb_amd.warmstart import UNet
 
-    shape = (32, 16, 48)
-    x, t = _data(shape, 3, seed=47)
+    x, t = _data(shape, batch, seed=47)
     res = []
     for brick in ("0", "1"):
         monkeypatch.setenv("LBM_UNET_WGRAD_BRICK", brick)
-        net = UNet(shape, batch=3, levels=2, base=16, seed=53, periodic=periodic)
+        net = UNet(shape, batch=batch, levels=2, base=base, seed=53, periodic=periodic)
         res.append(net.gradient(x, t))
         net.close()
     g0, g1 = res[0][1], res[1][1]
