@@ -65,10 +65,12 @@ __device__ __forceinline__ uint32_t sr_dir(uint32_t h, int d) {
     return x ^ (x >> 15);
 }
 
-// s + a uniform fraction of an FP16 ulp, truncated: the 13 mantissa bits FP16
-// drops from FP32 get random bits added before the round-toward-zero
+// FP16: s + a uniform fraction of an FP16 ulp, truncated: the 13 mantissa
+// bits FP16 drops from FP32 get random bits added before the round-toward-zero
 // conversion (v_cvt_pkrtz_f16_f32), so s rounds up with probability equal to
 // its dropped fraction.  Two values, 13 random bits each from r and r >> 16.
+// (gfx950's v_cvt_sr_f16_f32, two per pair, measured 2 % slower on the FP16
+// step than these adds and the one packed conversion.)
 __device__ __forceinline__ uint32_t f16x2_sr(float a, float b, uint32_t r) {
     const float ta = __uint_as_float(__float_as_uint(a) + (r & 0x1fffu));
     const float tb = __uint_as_float(__float_as_uint(b) + ((r >> 16) & 0x1fffu));
@@ -78,11 +80,26 @@ __device__ __forceinline__ uint32_t f16x2_sr(float a, float b, uint32_t r) {
     return out;
 }
 
-// The same for BF16 (16 dropped bits); NaN stays a quiet NaN.
-__device__ __forceinline__ uint32_t bf16_sr(float x, uint32_t r16) {
-    const uint32_t u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
-    return (u + (r16 & 0xffffu)) >> 16;
+// BF16: gfx950's v_cvt_sr_bf16_f32 adds a field of the random word r (its
+// high 16 bits) to the 16 bits BF16 drops and truncates, so a value rounds
+// away from zero with probability equal to its dropped fraction of an ulp
+// (measured on the device: the rate matches the fraction for hashed words).
+// One instruction per value, where the integer form (NaN test, add, shift)
+// took three: the BF16 step measured 3 % faster.  Two values take r and r
+// rotated by 16 bits, so their fields are disjoint.  bf16_sr_lo converts a
+// into the low half (the high half: old's), bf16_sr_hi b into the high half
+// of lo (the low half kept).
+using bf16x2_t = __bf16 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t bf16_sr_lo(uint32_t old, float a, uint32_t r) {
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf16x2_t, old), a, r, false));
+}
+__device__ __forceinline__ uint32_t bf16_sr_hi(uint32_t lo, float b, uint32_t r) {
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf16x2_t, lo), b, r, true));
+}
+__device__ __forceinline__ uint32_t bf16x2_sr(float a, float b, uint32_t r) {
+    return bf16_sr_hi(bf16_sr_lo(r, a, r), b, __builtin_amdgcn_alignbit(r, r, 16));
 }
 
 // ---------------------------------------------------------------------------
@@ -216,11 +233,10 @@ struct StoreBF16S {
         __builtin_nontemporal_store(lo | (hi << 16), reinterpret_cast<uint32_t*>(p));
     }
     __device__ static inline void store_sr(T* p, int, C dv, const Shift&, uint32_t r) {
-        __builtin_nontemporal_store((uint16_t)bf16_sr(dv, r), p);
+        __builtin_nontemporal_store((uint16_t)(bf16_sr_lo(r, dv, r) & 0xffffu), p);
     }
     __device__ static inline void store2_sr(T* p, int, C a, C b, const Shift&, uint32_t r) {
-        const uint32_t lo = bf16_sr(a, r), hi = bf16_sr(b, r >> 16);
-        __builtin_nontemporal_store(lo | (hi << 16), reinterpret_cast<uint32_t*>(p));
+        __builtin_nontemporal_store(bf16x2_sr(a, b, r), reinterpret_cast<uint32_t*>(p));
     }
 };
 

@@ -41,7 +41,8 @@ __device__ inline C w27(int d) {
 // The random word of direction d for the 16-bit D3Q27 stores, from the cell's
 // hash h (sr_cell): a 24-bit multiply by a per-direction odd constant (full
 // rate, where a 32-bit multiply runs at a quarter) and a shift-xor that brings
-// product bits 15-31 into the low bits the rounding takes.
+// product bits 15-31 into the low bits the FP16 rounding takes (BF16 takes the
+// high bits).
 __device__ __forceinline__ uint32_t sr_dir27(uint32_t h, int d) {
     const uint32_t x = __umul24(h, ((0x9e3779u * (2u * (uint32_t)d + 1u)) & 0xffffffu) | 1u);
     return x ^ (x >> 15);
@@ -71,14 +72,15 @@ struct StoreF16S27 {
     __device__ static inline C dec(uint32_t h, const Shift& sh) {
         return __half2float(__ushort_as_half((unsigned short)(h & 0xffffu))) * sh.ik;
     }
-    // a two-cell word in two halves: prep gives each value's rounding input
-    // (the scaled deviation with its random bits added, f16x2_sr), pack
-    // converts two of them (low: a); the halves equal what store_sr writes
+    // a two-cell word: prep gives cell a's rounding input (the scaled
+    // deviation with its random bits added, as f16x2_sr), pack_sr adds cell
+    // b's and converts both (low half: a); each half equals what store_sr
+    // writes with the same random word
     __device__ static inline uint32_t prep(C dv, const Shift& sh, uint32_t r) {
         return __float_as_uint(dv * sh.k) + (r & 0x1fffu);
     }
-    __device__ static inline uint32_t pack(uint32_t a, uint32_t b) {
-        const auto h = __builtin_amdgcn_cvt_pkrtz(__uint_as_float(a), __uint_as_float(b));
+    __device__ static inline uint32_t pack_sr(uint32_t a, C dv, const Shift& sh, uint32_t r) {
+        const auto h = __builtin_amdgcn_cvt_pkrtz(__uint_as_float(a), __uint_as_float(prep(dv, sh, r)));
         uint32_t out;
         __builtin_memcpy(&out, &h, 4);
         return out;
@@ -94,12 +96,14 @@ struct StoreBF16S27 {
         __builtin_nontemporal_store(StoreBF16S::f32_to_bf16_rne(v - w27<float>(d)), p);
     }
     __device__ static inline void store_sr(T* p, int, C dv, const Shift&, uint32_t r) {
-        __builtin_nontemporal_store((uint16_t)bf16_sr(dv, r), p);
+        __builtin_nontemporal_store((uint16_t)(bf16_sr_lo(r, dv, r) & 0xffffu), p);
     }
-    __device__ static inline uint32_t sr_bits(C dv, const Shift&, uint32_t r) { return bf16_sr(dv, r) & 0xffffu; }
+    __device__ static inline uint32_t sr_bits(C dv, const Shift&, uint32_t r) { return bf16_sr_lo(r, dv, r) & 0xffffu; }
     __device__ static inline C dec(uint32_t h, const Shift&) { return __uint_as_float(h << 16); }
-    __device__ static inline uint32_t prep(C dv, const Shift&, uint32_t r) { return bf16_sr(dv, r) & 0xffffu; }
-    __device__ static inline uint32_t pack(uint32_t a, uint32_t b) { return a | (b << 16); }
+    __device__ static inline uint32_t prep(C dv, const Shift&, uint32_t r) { return bf16_sr_lo(r, dv, r); }
+    __device__ static inline uint32_t pack_sr(uint32_t lo, C dv, const Shift&, uint32_t r) {
+        return bf16_sr_hi(lo, dv, r);
+    }
 };
 
 // Sums of the FP32 D3Q27 weights (rho_w = sum w_d, s2 = sum w_d c_dx^2; 1 and
@@ -1159,7 +1163,7 @@ __device__ __forceinline__ void pair_cells(const typename S::T* __restrict__ fin
         const uint32_t h = sr_cell(c0 + 1u, g.sh.seed);
 #pragma unroll
         for (int d = 0; d < QQ; ++d) {
-            w[d] = S::pack(pa[d], S::prep(f[d], g.sh, sr_dir27(h, d)));
+            w[d] = S::pack_sr(pa[d], f[d], g.sh, sr_dir27(h, d));
             __builtin_nontemporal_store(w[d], reinterpret_cast<uint32_t*>(const_cast<char*>(plane_of(fout, d, g)) + 2u * c0));
         }
     }
