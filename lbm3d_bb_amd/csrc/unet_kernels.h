@@ -319,9 +319,20 @@ constexpr int kHvox = kHz * kHy * kHx;
 template <int CK>
 struct BrickLds {
     // voxel stride in LDS, in elements (a multiple of 8, so every 16-byte read
-    // is aligned): 40 for CK = 32 and 24 for CK = 8 or 16, so that the 16 row
-    // lanes of an A read cover 16 disjoint groups of 4 of the 64 banks
-    static constexpr int S = CK == 32 ? 40 : 24;
+    // is aligned), chosen so that the 16 row lanes of an A read (16 consecutive
+    // voxels, one 16-byte piece each) cover the 64 banks without conflict:
+    // CK = 32: 40 (32 channels and 8 of padding); CK = 8: 8, unpadded (a
+    // 16-byte stride covers the banks once); CK = 16: 16, unpadded, with the
+    // voxel's two 16-byte halves swapped on every other run of 8 voxels, so that
+    // voxels v and v + 8 use different halves of the same 32 banks.  Unpadded,
+    // the 8- and 16-channel bricks take 10 and 21 KB of LDS instead of 31 KB,
+    // and the occupancy is no longer set by the LDS (5 blocks per CU before).
+    static constexpr int S = CK == 32 ? 40 : CK;
+    // element offset of the 8-channel group q of brick voxel v
+    __device__ static inline int at(int v, int q) {
+        if constexpr (CK == 16) return v * S + ((q ^ ((v >> 3) & 1)) << 3);
+        else return v * S + q * 8;
+    }
 };
 
 // Staging of a brick chunk: thread t holds pieces p = t + i * kThreads
@@ -359,13 +370,13 @@ __device__ inline void brick_load(const ConvArgs& a, int n, int x0, int y0, int 
 
 template <int CK>
 __device__ inline void brick_store(bf16* s_in, const bf16x8 (&v)[BrickStage<CK>::NIT]) {
-    constexpr int PPV = BrickStage<CK>::PPV, NP = BrickStage<CK>::NP, S = BrickLds<CK>::S;
+    constexpr int PPV = BrickStage<CK>::PPV, NP = BrickStage<CK>::NP;
 #pragma unroll
     for (int i = 0; i < BrickStage<CK>::NIT; ++i) {
         const int p = threadIdx.x + i * kThreads;
         if (p < NP) {
             const int vx = p / PPV, q = p - vx * PPV;
-            *reinterpret_cast<bf16x8*>(&s_in[vx * S + q * 8]) = v[i];
+            *reinterpret_cast<bf16x8*>(&s_in[BrickLds<CK>::at(vx, q)]) = v[i];
         }
     }
 }
@@ -450,9 +461,9 @@ __global__ __launch_bounds__(kThreads) void k_conv3_lds(ConvArgs a, int nbx, int
             }
             bf16x8 F[kHy];
             if (valid) {
-                const int base = (rowbase + dz * (kHy * kHx) + dx) * S + cgl;
+                const int v0 = rowbase + dz * (kHy * kHx) + dx;
 #pragma unroll
-                for (int hy = 0; hy < kHy; ++hy) F[hy] = load8(&s_in[base + hy * kHx * S]);
+                for (int hy = 0; hy < kHy; ++hy) F[hy] = load8(&s_in[BrickLds<CK>::at(v0 + hy * kHx, cgl >> 3)]);
             } else {
 #pragma unroll
                 for (int hy = 0; hy < kHy; ++hy) F[hy] = zero8();
