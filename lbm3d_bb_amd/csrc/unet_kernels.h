@@ -295,6 +295,48 @@ __global__ __launch_bounds__(kThreads) void k_up2(ConvArgs a) {
     }
 }
 
+// Pointwise (1x1x1) convolution of a narrow layer (cin <= 32 input channels,
+// at most 16 stored outputs, used for the network's output head): one thread
+// per voxel, its CIN input channels in 16-byte loads, the
+// weights and bias as FP32 in LDS (every lane of a wave reads the same word),
+// FP32 sums and k_conv's epilogue.  k_conv's MFMA tiles would pad the 4 head
+// outputs to 16 columns and run 32-wide k-steps over 16 channels.  Rows, input
+// and output share one grid (row r is voxel r of both).
+template <int CIN>
+__global__ __launch_bounds__(kThreads) void k_pw_narrow(ConvArgs a) {
+    __shared__ float s_w[16][CIN];
+    __shared__ float s_b[16];
+    for (int i = threadIdx.x; i < 16 * CIN; i += kThreads) {
+        const int co = i / CIN, ci = i - co * CIN;
+        s_w[co][ci] = (co < a.cstore && ci < a.cin) ? (float)a.wt[(int64_t)co * a.kpad + ci] : 0.f;
+    }
+    if (threadIdx.x < 16) s_b[threadIdx.x] = (a.bias && (int)threadIdx.x < a.cstore) ? a.bias[threadIdx.x] : 0.f;
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= a.m) return;
+    float xv[CIN];
+    const bf16* xp = a.x + r * a.xcs + a.xco;
+#pragma unroll
+    for (int c8 = 0; c8 < CIN / 8; ++c8) {
+        const bf16x8 v = load8(xp + c8 * 8);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) xv[c8 * 8 + j] = (float)v[j];
+    }
+#pragma unroll
+    for (int co = 0; co < 16; ++co) {
+        if (co < a.cstore) {
+            float v = 0.f;
+#pragma unroll
+            for (int ci = 0; ci < CIN; ++ci) v = fmaf(s_w[co][ci], xv[ci], v);
+            v += s_b[co];
+            if (a.relu) v = v > 0.f ? v : 0.f;
+            if (a.aux && !((float)a.aux[r * a.acs + a.aco + co] > 0.f)) v = 0.f;
+            if (a.yf) a.yf[r * a.cstore + co] = v;
+            else a.y[r * a.ycs + a.yco + co] = (bf16)v;
+        }
+    }
+}
+
 // Same-size 3x3x3 convolution with the input staged in LDS.  A block computes
 // a brick of 4 (z) x 4 (y) x 16 (x) output voxels of one sample, 256 rows, for
 // 16*NT output channels: wave w takes the z-plane w of the brick, its m-tile mt
