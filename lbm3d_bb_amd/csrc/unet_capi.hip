@@ -218,12 +218,16 @@ int conv(lbm_unet* n, int geom, const Buf& x, int xco, int cin, const bf16* w, i
         UCHK(n, hipGetLastError());
         return 0;
     }
-    if (geom == G_PW && n->pw_narrow && npar == 1 && cin % 8 == 0 && cin <= 32 && cstore <= 8 && rows.n == in.n &&
-        rows.d == in.d && rows.h == in.h && rows.w == in.w) {
-        // narrow pointwise layers (the output head): one thread per voxel
-        // (k_pw_narrow); wider outputs keep k_conv, whose lanes store 16
-        // consecutive channels of a voxel together (the head's input gradient,
-        // 16 channels, took 329 us one voxel per thread against 79 us there)
+    // BF16 outputs of 16 channels go in 16-byte pieces (k_pw_narrow's vector
+    // path); with scalar 2-byte stores strided across the wave the head's input
+    // gradient took 329 us one voxel per thread against 108 us on k_conv
+    const bool pw_vec = !yf && cstore % 8 == 0 && ycs % 8 == 0 && yco % 8 == 0 &&
+                        (!aux || (aux->cs % 8 == 0 && aco % 8 == 0));
+    if (geom == G_PW && n->pw_narrow && npar == 1 && cin % 8 == 0 && cin <= 32 &&
+        (cstore <= 8 || (cstore <= 16 && pw_vec)) && rows.n == in.n && rows.d == in.d && rows.h == in.h &&
+        rows.w == in.w) {
+        // narrow pointwise layers (the output head and its input gradient):
+        // one thread per voxel (k_pw_narrow)
         const dim3 gp((unsigned)((a.m + kThreads - 1) / kThreads)), bp(kThreads);
         if (cin == 8) hipLaunchKernelGGL((k_pw_narrow<8>), gp, bp, 0, n->stream, a);
         else if (cin == 16) hipLaunchKernelGGL((k_pw_narrow<16>), gp, bp, 0, n->stream, a);

@@ -322,6 +322,32 @@ __global__ __launch_bounds__(kThreads) void k_pw_narrow(ConvArgs a) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) xv[c8 * 8 + j] = (float)v[j];
     }
+    if (!a.yf && (a.cstore & 7) == 0 && (a.ycs & 7) == 0 && (a.yco & 7) == 0 &&
+        (!a.aux || ((a.acs & 7) == 0 && (a.aco & 7) == 0))) {
+        // BF16 outputs 8 channels at a time: the ReLU mask and the output in
+        // 16-byte pieces (one voxel per lane: scalar 2-byte accesses would be
+        // strided across the wave)
+#pragma unroll
+        for (int c8 = 0; c8 < 2; ++c8) {
+            if (c8 * 8 < a.cstore) {
+                const bf16x8 m8 = a.aux ? load8(a.aux + r * a.acs + a.aco + c8 * 8) : zero8();
+                bf16x8 o;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int co = c8 * 8 + j;
+                    float v = 0.f;
+#pragma unroll
+                    for (int ci = 0; ci < CIN; ++ci) v = fmaf(s_w[co][ci], xv[ci], v);
+                    v += s_b[co];
+                    if (a.relu) v = v > 0.f ? v : 0.f;
+                    if (a.aux && !((float)m8[j] > 0.f)) v = 0.f;
+                    o[j] = (bf16)v;
+                }
+                *reinterpret_cast<bf16x8*>(a.y + r * a.ycs + a.yco + c8 * 8) = o;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int co = 0; co < 16; ++co) {
         if (co < a.cstore) {
